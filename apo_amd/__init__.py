@@ -1,0 +1,137 @@
+"""MI355X (gfx950) Apollonius-diagram engine: Python host interface.
+
+This mirrors the construction API of PlathC/apo
+(apo/include/apo/gpu/algorithm.hpp:9-59, algorithm.cuh:12-46):
+
+    AlgorithmGPU(sites, k=16).build()           -> AlgorithmGPU::build
+    AlgorithmGPU.to_host()                      -> AlgorithmGPU::toHost -> VertexDiagram
+    compute_apollonius_diagram(sites)           -> apo::gpu::computeApolloniusDiagram
+
+Everything runs in the HIP library apo_amd/lib/libapo_hip.so through its C
+ABI (include/apo_c.h).  There is no CPU fallback: if the library is missing
+or no GPU is present, construction raises.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import ApoError, library_path  # noqa: F401
+
+__all__ = ["AlgorithmGPU", "VertexDiagram", "compute_apollonius_diagram", "ApoError", "library_path"]
+
+
+@dataclass
+class VertexDiagram:
+    """Host result of apo's toHost (algorithm.hpp:10-16): vertices as (x, y, z, w)
+    with w the empty-sphere radius, and the 4 tangent sites (original indices)."""
+    vertices: np.ndarray      # (V, 4) float64
+    vertices_id: np.ndarray   # (V, 4) uint32
+
+
+class AlgorithmGPU:
+    """One construction context on one GPU (AlgorithmGPU<K>, algorithm.cuh:12-46)."""
+
+    def __init__(self, sites, k: int = 16, max_iterations: int = 10, max_restarts: int = 100,
+                 stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0):
+        self.sites = np.ascontiguousarray(np.asarray(sites, dtype=np.float32).reshape(-1, 4))
+        self.k = int(k)
+        cfg = _lib.Config()
+        _lib.lib().apo_default_config(_lib.byref(cfg))
+        cfg.k = self.k
+        cfg.max_iterations = max_iterations
+        cfg.max_restarts = max_restarts
+        cfg.stage_mask = stage_mask
+        cfg.waves_per_cu = waves_per_cu
+        cfg.device = device
+        self._ctx = _lib.create(cfg)
+        self.built = False
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            _lib.lib().apo_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()
+
+    # -- AlgorithmGPU::build ------------------------------------------------
+    def build(self):
+        _lib.check(self._ctx, _lib.lib().apo_build(self._ctx, _lib.fptr(self.sites), len(self.sites)))
+        self.built = True
+        return self
+
+    # -- AlgorithmGPU::toHost -----------------------------------------------
+    def to_host(self) -> VertexDiagram:
+        L = _lib.lib()
+        cnt = _lib.c_uint64()
+        _lib.check(self._ctx, L.apo_vertex_count(self._ctx, _lib.byref(cnt)))
+        v = int(cnt.value)
+        xyzw = np.zeros((v, 4), np.float64)
+        ids = np.zeros((v, 4), np.uint32)
+        _lib.check(self._ctx, L.apo_get_vertices(self._ctx, _lib.dptr(xyzw), _lib.uptr(ids), v))
+        return VertexDiagram(xyzw, ids)
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        _lib.check(self._ctx, _lib.lib().apo_get_stats(self._ctx, _lib.byref(s)))
+        return {name: getattr(s, name) for name, _ in s._fields_ if name != "reserved"}
+
+    # -- device state (public members of AlgorithmGPU) ------------------------
+    def sorted_sites(self):
+        n = len(self.sites)
+        xyzr = np.zeros((n, 4), np.float32)
+        idx = np.zeros(n, np.uint32)
+        _lib.check(self._ctx, _lib.lib().apo_get_sorted_sites(self._ctx, _lib.fptr(xyzr), _lib.uptr(idx)))
+        return xyzr, idx
+
+    def lbvh(self) -> dict:
+        n = len(self.sites)
+        nodes = 2 * n - 1
+        out = dict(codes=np.zeros(n, np.uint64), parent=np.zeros(nodes, np.int32),
+                   left=np.zeros(nodes, np.int32), right=np.zeros(nodes, np.int32),
+                   aabb=np.zeros((nodes, 6), np.float32), sizes=np.zeros(nodes, np.uint32),
+                   max_radius=np.zeros(nodes, np.float32))
+        _lib.check(self._ctx, _lib.lib().apo_get_lbvh(
+            self._ctx, _lib.u64ptr(out["codes"]), _lib.iptr(out["parent"]), _lib.iptr(out["left"]),
+            _lib.iptr(out["right"]), _lib.fptr(out["aabb"]), _lib.uptr(out["sizes"]), _lib.fptr(out["max_radius"])))
+        return out
+
+    def artificial_sites(self) -> np.ndarray:
+        out = np.zeros((6, 4), np.float64)
+        _lib.check(self._ctx, _lib.lib().apo_get_artificial_sites(self._ctx, _lib.dptr(out)))
+        return out
+
+    def knns(self) -> np.ndarray:
+        out = np.zeros((len(self.sites), self.k), np.uint32)
+        _lib.check(self._ctx, _lib.lib().apo_get_knns(self._ctx, _lib.uptr(out)))
+        return out
+
+    def cell_counts(self) -> dict:
+        n = len(self.sites)
+        vn = np.zeros(n, np.uint32)
+        en = np.zeros(n, np.uint32)
+        st = np.zeros(n, np.uint8)
+        kr = np.zeros(n, np.float32)
+        fl = np.zeros(n, np.uint32)
+        _lib.check(self._ctx, _lib.lib().apo_get_cell_counts(self._ctx, _lib.uptr(vn), _lib.uptr(en),
+                                                             _lib.u8ptr(st), _lib.fptr(kr), _lib.uptr(fl)))
+        return dict(vertex_nb=vn, edge_nb=en, status=st, known_radius=kr, flags=fl)
+
+    def all_cells(self):
+        c = self.cell_counts()
+        V = np.zeros((int(c["vertex_nb"].sum()), 4), np.uint32)
+        E = np.zeros((int(c["edge_nb"].sum()), 4), np.uint32)
+        _lib.check(self._ctx, _lib.lib().apo_get_all_cells(self._ctx, _lib.uptr(V), _lib.uptr(E)))
+        return c, V, E
+
+
+def compute_apollonius_diagram(sites, k: int = 16) -> VertexDiagram:
+    """apo::gpu::computeApolloniusDiagram (apo/src/gpu/algorithm.cu:6-15)."""
+    alg = AlgorithmGPU(sites, k=k)
+    try:
+        return alg.build().to_host()
+    finally:
+        alg.close()

@@ -1,0 +1,533 @@
+// api.hip -- C ABI (include/apo_c.h) over the gfx950 kernels.
+//
+// Orchestration follows AlgorithmGPU<K>::build / toHost
+// (apo/include/apo/gpu/algorithm.inl:1410-1646) with every step on one
+// context-owned stream: no host round trip between the LBVH, the kNN pass,
+// the cell kernel, the validation rounds (task lists are compacted on the
+// device) and the vertex output.  The only synchronisation is at the end of
+// a build, to read the vertex count.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/apo_c.h"
+#include "apo_device.h"
+#include "apo_internal.h"
+
+using namespace apo_hip;
+
+namespace {
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t count, uint32_t value) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+template <typename T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t count = 0;
+};
+
+}  // namespace
+
+struct apo_ctx {
+    apo_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    int cus = 0;
+
+    uint32_t cap = 0;      // site capacity of the buffers below
+    uint32_t capK = 0;
+    uint32_t n = 0;        // sites of the last build
+    bool built = false;
+
+    // input staging
+    F4* sitesIn = nullptr;
+    // LBVH
+    Scene* scene = nullptr;
+    double* boxPartials = nullptr;
+    uint32_t* boxTicket = nullptr;
+    uint32_t* sortScratch = nullptr;
+    uint64_t* codesA = nullptr;
+    uint64_t* codesB = nullptr;
+    uint32_t* idxA = nullptr;
+    uint32_t* idxB = nullptr;
+    F4* sorted = nullptr;
+    Node* nodes = nullptr;
+    int32_t* parent = nullptr;
+    uint32_t* sizes = nullptr;
+    uint32_t* refitCounter = nullptr;
+    float* rootBox = nullptr;
+    // cells
+    uint8_t* status = nullptr;
+    float* knownRadii = nullptr;
+    uint32_t* flags = nullptr;
+    uint4* vtx = nullptr;
+    uint32_t* vnb = nullptr;
+    uint4* edg = nullptr;
+    uint32_t* enb = nullptr;
+    uint32_t* knn = nullptr;
+    uint32_t* counters = nullptr;   // [0] cell queue, [1..15] validation
+    uint32_t* err = nullptr;        // device guard bits
+    uint32_t* tasks = nullptr;
+    // output
+    uint32_t* cellCount = nullptr;
+    uint32_t* blockSums = nullptr;
+    uint32_t* total = nullptr;
+    D4* outXyzw = nullptr;
+    uint4* outIds = nullptr;
+    uint64_t outCap = 0;
+    uint64_t vertexCount = 0;
+
+    hipEvent_t ev[8] = {};
+    apo_stats stats{};
+};
+
+namespace {
+
+apo_status fail(apo_ctx* c, apo_status s, const std::string& msg) {
+    if (c) c->error = msg;
+    return s;
+}
+
+apo_status hipFail(apo_ctx* c, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(c, e == hipErrorOutOfMemory ? APO_ERR_OUT_OF_MEMORY : APO_ERR_HIP, m);
+}
+
+#define APO_HIP(call)                                         \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return hipFail(ctx, e_, #call); \
+    } while (0)
+
+template <typename T>
+hipError_t alloc(T*& p, size_t count) {
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        if (e != hipSuccess) return e;
+    }
+    if (count == 0) count = 1;
+    return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+}
+
+uint32_t tilesFor(uint32_t n) { return (n + 4095u) / 4096u; }
+
+apo_status ensureCapacity(apo_ctx* ctx, uint32_t n) {
+    const uint32_t K = ctx->cfg.k;
+    if (n <= ctx->cap && K <= ctx->capK) return APO_OK;
+    const uint32_t cap = n;
+    const size_t nodes = 2 * size_t(cap) - 1;
+    APO_HIP(alloc(ctx->sitesIn, cap));
+    APO_HIP(alloc(ctx->scene, 1));
+    APO_HIP(alloc(ctx->boxPartials, size_t(kBoxBlocks) * 8));
+    APO_HIP(alloc(ctx->boxTicket, 1));
+    // the ticket is self-resetting; clear it once, ordered on the build stream
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->boxTicket, 1u, 0u);
+    APO_HIP(hipGetLastError());
+    APO_HIP(alloc(ctx->sortScratch, size_t(kRadixPasses) * 256 + kRadixPasses + size_t(kRadixPasses) * tilesFor(cap) * 256));
+    APO_HIP(alloc(ctx->codesA, cap));
+    APO_HIP(alloc(ctx->codesB, cap));
+    APO_HIP(alloc(ctx->idxA, cap));
+    APO_HIP(alloc(ctx->idxB, cap));
+    APO_HIP(alloc(ctx->sorted, cap));
+    APO_HIP(alloc(ctx->nodes, cap));
+    APO_HIP(alloc(ctx->parent, nodes));
+    APO_HIP(alloc(ctx->sizes, nodes));
+    APO_HIP(alloc(ctx->refitCounter, cap));
+    APO_HIP(alloc(ctx->rootBox, 8));
+    APO_HIP(alloc(ctx->status, cap));
+    APO_HIP(alloc(ctx->knownRadii, cap));
+    APO_HIP(alloc(ctx->flags, cap));
+    APO_HIP(alloc(ctx->vtx, size_t(cap) * kVertexCap));
+    APO_HIP(alloc(ctx->vnb, cap));
+    APO_HIP(alloc(ctx->edg, size_t(cap) * kEdgeCap));
+    APO_HIP(alloc(ctx->enb, cap));
+    APO_HIP(alloc(ctx->knn, size_t(cap) * K));
+    APO_HIP(alloc(ctx->counters, 64));
+    APO_HIP(alloc(ctx->err, 1));
+    APO_HIP(alloc(ctx->tasks, cap));
+    APO_HIP(alloc(ctx->cellCount, cap));
+    APO_HIP(alloc(ctx->blockSums, (cap + 255) / 256));
+    APO_HIP(alloc(ctx->total, 1));
+    ctx->outCap = uint64_t(cap) * 8 + 1024;
+    APO_HIP(alloc(ctx->outXyzw, ctx->outCap));
+    APO_HIP(alloc(ctx->outIds, ctx->outCap));
+    ctx->cap = cap;
+    ctx->capK = K;
+    return APO_OK;
+}
+
+CellCtx cellCtx(apo_ctx* ctx) {
+    CellCtx c;
+    c.n = ctx->n;
+    c.sorted = ctx->sorted;
+    c.scene = ctx->scene;
+    c.nodes = ctx->nodes;
+    c.status = ctx->status;
+    c.knownRadii = ctx->knownRadii;
+    c.flags = ctx->flags;
+    c.vtx = ctx->vtx;
+    c.vnb = ctx->vnb;
+    c.edg = ctx->edg;
+    c.enb = ctx->enb;
+    c.err = ctx->err;
+    return c;
+}
+
+uint32_t persistentWaves(apo_ctx* ctx) {
+    const uint32_t perCu = ctx->cfg.waves_per_cu ? ctx->cfg.waves_per_cu : 12u;
+    return uint32_t(ctx->cus) * perCu;
+}
+
+std::string guardText(uint32_t bits) {
+    static const char* names[] = {"sort tile", "sort scatter", "sort look-back spin", "gather index",
+                                  "refit parent", "refit child", "knn child", "cell site id",
+                                  "cell neighbour id", "validation"};
+    std::string out;
+    for (int b = 0; b < 10; b++)
+        if (bits & (1u << b)) out += std::string(out.empty() ? "" : ", ") + names[b];
+    return out;
+}
+
+// With APO_SYNC_STAGES=1 every stage is followed by a synchronisation and a
+// check of the runtime error state and of the device guard word, so a failure
+// is attributed to the stage that produced it.
+apo_status stageCheck(apo_ctx* ctx, const char* stage) {
+    static const bool sync = [] {
+        const char* e = std::getenv("APO_SYNC_STAGES");
+        return e && e[0] == '1';
+    }();
+    if (!sync) return APO_OK;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hipFail(ctx, e, stage);
+    uint32_t bits = 0;
+    e = hipMemcpy(&bits, ctx->err, sizeof(bits), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hipFail(ctx, e, stage);
+    std::fprintf(stderr, "[apo] stage %s ok (guards 0x%x)\n", stage, bits);
+    if (bits) return fail(ctx, APO_ERR_HIP, std::string("device guard tripped after ") + stage + ": " + guardText(bits));
+    return APO_OK;
+}
+
+#define APO_STAGE(name)                              \
+    do {                                             \
+        apo_status s_ = stageCheck(ctx, name);       \
+        if (s_ != APO_OK) return s_;                 \
+    } while (0)
+
+apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
+    hipStream_t st = ctx->stream;
+    ctx->n = n;
+    LbvhBuffers lb{ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA, ctx->codesB,
+                   ctx->idxA,  ctx->idxB,        ctx->sorted,    ctx->nodes,       ctx->parent, ctx->sizes,
+                   ctx->refitCounter, ctx->rootBox, ctx->err};
+    const CellCtx cc = cellCtx(ctx);
+    const uint32_t K = ctx->cfg.k;
+    const uint32_t waves = persistentWaves(ctx);
+
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
+    APO_HIP(hipEventRecord(ctx->ev[0], st));
+    APO_HIP(lbvhBuild(lb, sitesDev, n, st));
+    APO_STAGE("lbvh");
+    APO_HIP(hipEventRecord(ctx->ev[1], st));
+    APO_HIP(knnLaunch(cc, K, ctx->knn, st));
+    APO_STAGE("knn");
+    APO_HIP(hipEventRecord(ctx->ev[2], st));
+    if (ctx->cfg.stage_mask & 8u) {
+        // stop after the kNN pass (introspection of the LBVH and kNN state)
+        APO_HIP(hipStreamSynchronize(st));
+        ctx->vertexCount = 0;
+        ctx->built = true;
+        return APO_OK;
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->counters, 64u, 0u);
+    if (ctx->cfg.stage_mask & 1u) APO_HIP(cellsLaunch(cc, ctx->knn, K, ctx->counters, waves, st));
+    else APO_HIP(cellsLaunch(cc, ctx->knn, 0, ctx->counters, waves, st));
+    APO_STAGE("cells");
+    APO_HIP(hipEventRecord(ctx->ev[3], st));
+    ValidationBuffers vb{ctx->tasks, ctx->counters + 1};
+    if (ctx->cfg.stage_mask & 2u) APO_HIP(validationLaunch(cc, vb, ctx->cfg.max_iterations, waves, false, st));
+    APO_STAGE("vertex/edge validation");
+    APO_HIP(hipEventRecord(ctx->ev[4], st));
+    if (ctx->cfg.stage_mask & 4u) APO_HIP(validationLaunch(cc, vb, 1, waves, true, st));
+    APO_STAGE("bisector validation");
+    APO_HIP(hipEventRecord(ctx->ev[5], st));
+    OutputBuffers ob{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
+    APO_HIP(outputCountLaunch(cc, ob, st));
+    APO_HIP(outputWriteLaunch(cc, ob, ctx->idxA, st));
+    APO_STAGE("output");
+    APO_HIP(hipEventRecord(ctx->ev[6], st));
+    uint32_t total = 0, guard = 0;
+    APO_HIP(hipMemcpyAsync(&total, ctx->total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped: " + guardText(guard));
+    if (total > ctx->outCap) {
+        // more vertices than the estimate: grow the output and write again
+        ctx->outCap = uint64_t(total) + (uint64_t(total) >> 3) + 1024;
+        APO_HIP(alloc(ctx->outXyzw, ctx->outCap));
+        APO_HIP(alloc(ctx->outIds, ctx->outCap));
+        OutputBuffers ob2{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
+        APO_HIP(outputWriteLaunch(cc, ob2, ctx->idxA, st));
+        APO_HIP(hipStreamSynchronize(st));
+    }
+    ctx->vertexCount = total;
+    float ms[7] = {};
+    for (int s = 0; s < 6; s++) APO_HIP(hipEventElapsedTime(&ms[s], ctx->ev[s], ctx->ev[s + 1]));
+    float all = 0;
+    APO_HIP(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[6]));
+    apo_stats& S = ctx->stats;
+    std::memset(&S, 0, sizeof(S));
+    S.ms_total = all;
+    S.ms_lbvh = ms[0];
+    S.ms_knn = ms[1];
+    S.ms_cells = ms[2];
+    S.ms_validation = ms[3];
+    S.ms_bisector = ms[4];
+    S.ms_output = ms[5];
+    S.sites = n;
+    S.vertices = total;
+    ctx->built = true;
+    return APO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void apo_default_config(apo_config* cfg) {
+    if (!cfg) return;
+    cfg->k = 16;
+    cfg->max_iterations = 10;
+    cfg->max_restarts = 100;
+    cfg->stage_mask = 7;
+    cfg->waves_per_cu = 0;
+    cfg->device = 0;
+}
+
+apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
+    if (!out) return APO_ERR_INVALID;
+    *out = nullptr;
+    apo_ctx* ctx = new apo_ctx();
+    if (cfg) ctx->cfg = *cfg;
+    else apo_default_config(&ctx->cfg);
+    const uint32_t k = ctx->cfg.k;
+    if (!(k == 16 || k == 24 || k == 32 || k == 48 || k == 64)) {
+        delete ctx;
+        return APO_ERR_INVALID;
+    }
+    ctx->device = int(ctx->cfg.device);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    for (int s = 0; s < 8 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
+    if (e != hipSuccess) {
+        apo_destroy(ctx);
+        return e == hipErrorOutOfMemory ? APO_ERR_OUT_OF_MEMORY : APO_ERR_HIP;
+    }
+    *out = ctx;
+    return APO_OK;
+}
+
+void apo_destroy(apo_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    void* ptrs[] = {ctx->sitesIn, ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA,
+                    ctx->codesB, ctx->idxA, ctx->idxB, ctx->sorted, ctx->nodes, ctx->parent, ctx->sizes,
+                    ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
+                    ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
+                    ctx->total, ctx->outXyzw, ctx->outIds, ctx->err};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* apo_last_error(const apo_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n) {
+    if (!ctx || !xyzr_device) return APO_ERR_INVALID;
+    if (n < 2) return fail(ctx, APO_ERR_INVALID, "need at least 2 sites");
+    if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
+    APO_HIP(hipSetDevice(ctx->device));
+    ctx->built = false;
+    apo_status s = ensureCapacity(ctx, n);
+    if (s != APO_OK) return s;
+    return runPipeline(ctx, reinterpret_cast<const F4*>(xyzr_device), n);
+}
+
+apo_status apo_build(apo_ctx* ctx, const float* xyzr, uint32_t n) {
+    if (!ctx || !xyzr) return APO_ERR_INVALID;
+    if (n < 2) return fail(ctx, APO_ERR_INVALID, "need at least 2 sites");
+    if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
+    APO_HIP(hipSetDevice(ctx->device));
+    ctx->built = false;
+    apo_status s = ensureCapacity(ctx, n);
+    if (s != APO_OK) return s;
+    APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, ctx->stream));
+    return runPipeline(ctx, ctx->sitesIn, n);
+}
+
+apo_status apo_vertex_count(apo_ctx* ctx, uint64_t* count) {
+    if (!ctx || !count) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    *count = ctx->vertexCount;
+    return APO_OK;
+}
+
+apo_status apo_get_vertices(apo_ctx* ctx, double* xyzw, uint32_t* ids, uint64_t capacity) {
+    if (!ctx) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    if (capacity < ctx->vertexCount) return fail(ctx, APO_ERR_CAPACITY, "output buffer too small");
+    const size_t v = ctx->vertexCount;
+    if (xyzw) APO_HIP(hipMemcpy(xyzw, ctx->outXyzw, v * sizeof(D4), hipMemcpyDeviceToHost));
+    if (ids) APO_HIP(hipMemcpy(ids, ctx->outIds, v * sizeof(uint4), hipMemcpyDeviceToHost));
+    return APO_OK;
+}
+
+apo_status apo_get_device_vertices(apo_ctx* ctx, const double** xyzw, const uint32_t** ids) {
+    if (!ctx) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    if (xyzw) *xyzw = reinterpret_cast<const double*>(ctx->outXyzw);
+    if (ids) *ids = reinterpret_cast<const uint32_t*>(ctx->outIds);
+    return APO_OK;
+}
+
+apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats) {
+    if (!ctx || !stats) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    const uint32_t n = ctx->n;
+    std::vector<uint8_t> st(n);
+    std::vector<uint32_t> fl(n);
+    APO_HIP(hipMemcpy(st.data(), ctx->status, n, hipMemcpyDeviceToHost));
+    APO_HIP(hipMemcpy(fl.data(), ctx->flags, size_t(n) * 4, hipMemcpyDeviceToHost));
+    uint64_t buried = 0, uncertified = 0, flagged = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        buried += st[i] == kStatusBuried;
+        uncertified += st[i] == kStatusInValidation;
+        flagged += fl[i] != 0;
+    }
+    ctx->stats.buried = buried;
+    ctx->stats.uncertified = uncertified;
+    ctx->stats.flagged_cells = flagged;
+    *stats = ctx->stats;
+    return APO_OK;
+}
+
+apo_status apo_get_sorted_sites(apo_ctx* ctx, float* xyzr, uint32_t* indices) {
+    if (!ctx) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    if (xyzr) APO_HIP(hipMemcpy(xyzr, ctx->sorted, size_t(ctx->n) * sizeof(F4), hipMemcpyDeviceToHost));
+    if (indices) APO_HIP(hipMemcpy(indices, ctx->idxA, size_t(ctx->n) * 4, hipMemcpyDeviceToHost));
+    return APO_OK;
+}
+
+apo_status apo_get_lbvh(apo_ctx* ctx, uint64_t* sorted_codes, int32_t* parent, int32_t* left, int32_t* right,
+                        float* aabb, uint32_t* sizes, float* max_radius) {
+    if (!ctx) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    const uint32_t n = ctx->n;
+    const size_t nodes = 2 * size_t(n) - 1;
+    if (sorted_codes) APO_HIP(hipMemcpy(sorted_codes, ctx->codesA, size_t(n) * 8, hipMemcpyDeviceToHost));
+    if (parent) APO_HIP(hipMemcpy(parent, ctx->parent, nodes * 4, hipMemcpyDeviceToHost));
+    if (sizes) APO_HIP(hipMemcpy(sizes, ctx->sizes, nodes * 4, hipMemcpyDeviceToHost));
+    std::vector<Node> nd(n - 1);
+    APO_HIP(hipMemcpy(nd.data(), ctx->nodes, size_t(n - 1) * sizeof(Node), hipMemcpyDeviceToHost));
+    float root[8];
+    APO_HIP(hipMemcpy(root, ctx->rootBox, sizeof(root), hipMemcpyDeviceToHost));
+    std::vector<F4> srt(n);
+    APO_HIP(hipMemcpy(srt.data(), ctx->sorted, size_t(n) * sizeof(F4), hipMemcpyDeviceToHost));
+    // unpack the child-in-parent records into per-node arrays
+    std::vector<float> box(nodes * 6, 0.f), mr(nodes, 0.f);
+    std::vector<int32_t> L(nodes, 0), R(nodes, 0);
+    for (uint32_t i = 0; i < n - 1; i++) {
+        const float* f = reinterpret_cast<const float*>(&nd[i]);
+        const int32_t lc = nd[i].d.x, rc = nd[i].d.y;
+        L[i] = lc;
+        R[i] = rc;
+        for (int c = 0; c < 6; c++) {
+            box[size_t(lc) * 6 + c] = f[c];
+            box[size_t(rc) * 6 + c] = f[6 + c];
+        }
+        mr[lc] = f[14];
+        mr[rc] = f[15];
+    }
+    for (int c = 0; c < 6; c++) box[c] = root[c];
+    mr[0] = root[6];
+    for (uint32_t i = 0; i < n; i++) {
+        L[n - 1 + i] = int32_t(i);
+        R[n - 1 + i] = int32_t(i) + 1;
+    }
+    if (left) std::memcpy(left, L.data(), nodes * 4);
+    if (right) std::memcpy(right, R.data(), nodes * 4);
+    if (aabb) std::memcpy(aabb, box.data(), nodes * 24);
+    if (max_radius) std::memcpy(max_radius, mr.data(), nodes * 4);
+    return APO_OK;
+}
+
+apo_status apo_get_artificial_sites(apo_ctx* ctx, double* xyzr24) {
+    if (!ctx || !xyzr24) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    APO_HIP(hipMemcpy(xyzr24, &ctx->scene->art[0], 6 * sizeof(D4), hipMemcpyDeviceToHost));
+    return APO_OK;
+}
+
+apo_status apo_get_knns(apo_ctx* ctx, uint32_t* knn) {
+    if (!ctx || !knn) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    APO_HIP(hipMemcpy(knn, ctx->knn, size_t(ctx->n) * ctx->cfg.k * 4, hipMemcpyDeviceToHost));
+    return APO_OK;
+}
+
+apo_status apo_get_cell_counts(apo_ctx* ctx, uint32_t* vertex_nb, uint32_t* edge_nb, uint8_t* status,
+                               float* known_radius, uint32_t* flags) {
+    if (!ctx) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    const size_t n = ctx->n;
+    if (vertex_nb) APO_HIP(hipMemcpy(vertex_nb, ctx->vnb, n * 4, hipMemcpyDeviceToHost));
+    if (edge_nb) APO_HIP(hipMemcpy(edge_nb, ctx->enb, n * 4, hipMemcpyDeviceToHost));
+    if (status) APO_HIP(hipMemcpy(status, ctx->status, n, hipMemcpyDeviceToHost));
+    if (known_radius) APO_HIP(hipMemcpy(known_radius, ctx->knownRadii, n * 4, hipMemcpyDeviceToHost));
+    if (flags) APO_HIP(hipMemcpy(flags, ctx->flags, n * 4, hipMemcpyDeviceToHost));
+    return APO_OK;
+}
+
+apo_status apo_get_all_cells(apo_ctx* ctx, uint32_t* vertices, uint32_t* edges) {
+    if (!ctx) return APO_ERR_INVALID;
+    if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
+    const size_t n = ctx->n;
+    std::vector<uint32_t> vn(n), en(n);
+    APO_HIP(hipMemcpy(vn.data(), ctx->vnb, n * 4, hipMemcpyDeviceToHost));
+    APO_HIP(hipMemcpy(en.data(), ctx->enb, n * 4, hipMemcpyDeviceToHost));
+    std::vector<uint4> all;
+    size_t vo = 0, eo = 0;
+    if (vertices) {
+        all.resize(n * kVertexCap);
+        APO_HIP(hipMemcpy(all.data(), ctx->vtx, all.size() * sizeof(uint4), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            std::memcpy(vertices + 4 * vo, all.data() + i * kVertexCap, vn[i] * sizeof(uint4));
+            vo += vn[i];
+        }
+    }
+    if (edges) {
+        all.resize(n * kEdgeCap);
+        APO_HIP(hipMemcpy(all.data(), ctx->edg, all.size() * sizeof(uint4), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            std::memcpy(edges + 4 * eo, all.data() + i * kEdgeCap, en[i] * sizeof(uint4));
+            eo += en[i];
+        }
+    }
+    return APO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// apo_internal.h -- buffers and launchers shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "apo_device.h"
+
+#ifndef DBL_MAX
+#define DBL_MAX __DBL_MAX__
+#endif
+#ifndef FLT_MAX
+#define FLT_MAX __FLT_MAX__
+#endif
+
+namespace apo_hip {
+
+constexpr int kRadixPasses = 8;
+constexpr uint32_t kBoxBlocks = 512;
+
+// Per-cell capacities (apo/include/apo/gpu/algorithm.inl:25-30).
+constexpr uint32_t kVertexCap = 152;
+constexpr uint32_t kEdgeCap = 264;
+constexpr uint32_t kNewEdgeCap = 128;
+constexpr uint32_t kBisectorCap = 512;
+constexpr uint32_t kNeighbourCap = 2 * kEdgeCap;
+constexpr uint32_t kStackDepth = 32;
+
+// Scene box + artificial sites, produced on the device.
+struct Scene {
+    double mn[3], mx[3];
+    double rmin, rmax;
+    float fmn[3], fmx[3];
+    float scale[3];
+    float pad;
+    D4 art[6];
+};
+
+struct LbvhBuffers {
+    Scene* scene;
+    double* boxPartials;   // kBoxBlocks * 8
+    uint32_t* boxTicket;   // 1
+    uint32_t* sortScratch; // hist [8*256] + tile counters [8] + look-back [8*tiles*256]
+    uint64_t* codesA;
+    uint64_t* codesB;
+    uint32_t* idxA;        // sorted position -> original index (final)
+    uint32_t* idxB;
+    F4* sorted;            // sites in Morton order
+    Node* nodes;           // n-1 internal nodes
+    int32_t* parent;       // 2n-1
+    uint32_t* sizes;       // 2n-1
+    uint32_t* refitCounter;// n-1
+    float* rootBox;        // 7 floats: min, max, max radius
+    uint32_t* err;         // device guard bits
+};
+
+hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, hipStream_t st);
+
+// Device view of the construction state.
+struct CellCtx {
+    uint32_t n;
+    const F4* sorted;
+    const Scene* scene;     // artificial sites in scene->art
+    const Node* nodes;
+    uint8_t* status;
+    float* knownRadii;
+    uint32_t* flags;
+    uint4* vtx;             // n * kVertexCap records (j, k, l, type | status << 8)
+    uint32_t* vnb;
+    uint4* edg;             // n * kEdgeCap records (j, k, vertexNb, status)
+    uint32_t* enb;
+    uint32_t* err;          // device guard bits
+};
+
+hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st);
+hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t waves,
+                       hipStream_t st);
+
+// Validation (algorithm.inl:208-1233).  Task lists live on the device.
+struct ValidationBuffers {
+    uint32_t* tasks;       // n
+    uint32_t* counters;    // [0] task count, [1..] queue heads
+};
+hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t waves,
+                            bool bisector, hipStream_t st);
+
+// Output (algorithm.inl:1235-1307): count, scan, write.
+struct OutputBuffers {
+    uint32_t* cellCount;   // n
+    uint32_t* blockSums;   // blocks
+    uint32_t* total;       // 1
+    D4* xyzw;              // capacity records
+    uint4* ids;            // capacity records
+    uint64_t capacity;
+};
+hipError_t outputCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st);
+hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices, hipStream_t st);
+
+}  // namespace apo_hip

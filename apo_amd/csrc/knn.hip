@@ -1,0 +1,176 @@
+// knn.hip -- k nearest sites by sphere gap over the LBVH, one lane per site.
+//
+// Reference: apo/include/apo/gpu/lbvh.inl:8-105 (closest-child-first
+// stackful traversal), apo/include/apo/gpu/heap.hpp:11-43 (bounded max-heap,
+// heapsort) and apo/include/apo/gpu/algorithm.inl:32-99 (getKnns with the
+// buried test on the first pass).  The traversal order, the strict
+// acceptance tests and the heap moves are those of the reference, so the
+// neighbour lists (including the order of equal gaps) are identical.
+//
+// Layout: the heap (keys, gaps) and the traversal stack (node, gap bits) of
+// every lane sit in LDS as [slot][lane] arrays, so any mix of per-lane slot
+// indices is bank-conflict free.  The result is written through LDS so the
+// global stores are coalesced.
+#include "apo_device.h"
+#include "apo_internal.h"
+
+namespace apo_hip {
+
+template <uint32_t K>
+__global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ knn) {
+    __shared__ uint32_t hKey[K][64];
+    __shared__ float hVal[K][64];
+    __shared__ uint32_t stNode[kStackDepth][64];
+    __shared__ float stDist[kStackDepth][64];
+
+    const uint32_t lane = laneId();
+    const uint32_t base = blockIdx.x * 64u;
+    const uint32_t i = base + lane;
+    const uint32_t n = c.n;
+    const bool active = i < n;
+
+    for (uint32_t k = 0; k < K; k++) {
+        hKey[k][lane] = kNone;
+        hVal[k][lane] = FLT_MAX;
+    }
+    uint32_t overflow = 0;
+    if (active) {
+        const F4 s = c.sorted[i];
+        const float startingDistance = -FLT_MAX;
+        uint32_t sp = 0;
+        uint32_t node = 0;
+        while (true) {
+            while (node < n - 1) {
+                const Node nd = c.nodes[node];
+                Box lb, rb;
+                nodeBoxes(nd, lb, rb);
+                const uint32_t L = uint32_t(nd.d.x), R = uint32_t(nd.d.y);
+                if (L >= 2 * n - 1 || R >= 2 * n - 1) {
+                    atomicOr(c.err, kErrKnnChild);
+                    node = 2 * n;  // neither internal nor leaf: ends this descent
+                    break;
+                }
+                const float dL = boxDistance(lb, s), dLmax = boxFarthestDistance(lb, s);
+                const float dR = boxDistance(rb, s), dRmax = boxFarthestDistance(rb, s);
+                const bool leftClosest = dL < dR;
+                const float minFar = leftClosest ? dR : dL;
+                const float maxFar = leftClosest ? dRmax : dLmax;
+                const uint32_t farChild = leftClosest ? R : L;
+                const float minClose = leftClosest ? dL : dR;
+                const float maxClose = leftClosest ? dLmax : dRmax;
+                const uint32_t closeChild = leftClosest ? L : R;
+                const float root = hVal[0][lane];
+                const bool exploreFar = minFar < root && maxFar > startingDistance;
+                const bool exploreClose = minClose < root && maxClose > startingDistance;
+                if (!exploreFar && !exploreClose) break;
+                node = exploreClose ? closeChild : farChild;
+                if (exploreClose && exploreFar) {
+                    if (sp < kStackDepth) {
+                        stNode[sp][lane] = farChild;
+                        stDist[sp][lane] = minFar;
+                        sp++;
+                    } else {
+                        overflow = 1;
+                    }
+                }
+            }
+            if (node >= n - 1 && node < 2 * n - 1) {
+                const uint32_t j = node - (n - 1);
+                const float d = sphereDistF(s, c.sorted[j]);
+                if (j != i && d < hVal[0][lane] && d > startingDistance) {
+                    // replace the root and sift down (heap.hpp:11-31)
+                    hVal[0][lane] = d;
+                    hKey[0][lane] = j;
+                    uint32_t p = 0;
+                    while (true) {
+                        const uint32_t l = 2 * p + 1, r = 2 * p + 2;
+                        uint32_t largest = p;
+                        float vl = hVal[p][lane];
+                        if (l < K && hVal[l][lane] > vl) {
+                            largest = l;
+                            vl = hVal[l][lane];
+                        }
+                        if (r < K && hVal[r][lane] > vl) largest = r;
+                        if (largest == p) break;
+                        const float tv = hVal[p][lane];
+                        hVal[p][lane] = hVal[largest][lane];
+                        hVal[largest][lane] = tv;
+                        const uint32_t tk = hKey[p][lane];
+                        hKey[p][lane] = hKey[largest][lane];
+                        hKey[largest][lane] = tk;
+                        p = largest;
+                    }
+                }
+            }
+            bool found = false;
+            while (sp > 0) {
+                sp--;
+                if (stDist[sp][lane] > hVal[0][lane]) continue;
+                node = stNode[sp][lane];
+                found = true;
+                break;
+            }
+            if (!found) break;
+        }
+        // heapsort into ascending gaps (heap.hpp:33-43)
+        for (uint32_t size = K; size > 0;) {
+            const float tv = hVal[0][lane];
+            hVal[0][lane] = hVal[size - 1][lane];
+            hVal[size - 1][lane] = tv;
+            const uint32_t tk = hKey[0][lane];
+            hKey[0][lane] = hKey[size - 1][lane];
+            hKey[size - 1][lane] = tk;
+            --size;
+            uint32_t p = 0;
+            while (true) {
+                const uint32_t l = 2 * p + 1, r = 2 * p + 2;
+                uint32_t largest = p;
+                if (l < size && hVal[l][lane] > hVal[largest][lane]) largest = l;
+                if (r < size && hVal[r][lane] > hVal[largest][lane]) largest = r;
+                if (largest == p) break;
+                const float t2 = hVal[p][lane];
+                hVal[p][lane] = hVal[largest][lane];
+                hVal[largest][lane] = t2;
+                const uint32_t k2 = hKey[p][lane];
+                hKey[p][lane] = hKey[largest][lane];
+                hKey[largest][lane] = k2;
+                p = largest;
+            }
+        }
+        // buried test against the nearest site (algorithm.inl:85-98)
+        const uint32_t j0 = hKey[0][lane];
+        uint8_t status = kStatusInValidation;
+        if (j0 != kNone) {
+            const F4 cl = c.sorted[j0];
+            const double ri = s.w, rj = cl.w;
+            const double bigger = __builtin_fmax(ri, rj), smaller = __builtin_fmin(ri, rj);
+            const D3 dc{double(s.x) - double(cl.x), double(s.y) - double(cl.y), double(s.z) - double(cl.z)};
+            if (ltD(len(dc), bigger - smaller) && !ltD(rj, ri)) status = kStatusBuried;
+        }
+        // the kNN pass is the first writer of status and flags
+        c.status[i] = status;
+        c.flags[i] = overflow ? kFlagStackOverflow : 0u;
+    }
+    waveSync();
+    // coalesced write of the 64 x K block
+    const uint32_t rows = (n - base) < 64u ? (n - base) : 64u;
+    for (uint32_t t = lane; t < rows * K; t += 64u) {
+        const uint32_t row = t / K, col = t % K;
+        knn[size_t(base + row) * K + col] = hKey[col][row];
+    }
+}
+
+hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st) {
+    const uint32_t blocks = (c.n + 63u) / 64u;
+    switch (k) {
+    case 16: hipLaunchKernelGGL(k_knn<16>, dim3(blocks), dim3(64), 0, st, c, knn); break;
+    case 24: hipLaunchKernelGGL(k_knn<24>, dim3(blocks), dim3(64), 0, st, c, knn); break;
+    case 32: hipLaunchKernelGGL(k_knn<32>, dim3(blocks), dim3(64), 0, st, c, knn); break;
+    case 48: hipLaunchKernelGGL(k_knn<48>, dim3(blocks), dim3(64), 0, st, c, knn); break;
+    case 64: hipLaunchKernelGGL(k_knn<64>, dim3(blocks), dim3(64), 0, st, c, knn); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace apo_hip

@@ -1,0 +1,471 @@
+// lbvh.hip -- LBVH build for gfx950: scene box, Morton codes, onesweep LSD
+// radix sort, Karras hierarchy, atomic bottom-up refit (AABB + max radius).
+//
+// Reference: apo/src/gpu/lbvh.cu:11-323 (Morton, setLeaves, construct,
+// refit) and apo/src/gpu/algorithm.cpp:5-25 (scene box).  The sort replaces
+// thrust::sort_by_key (lbvh.cu:294-295) with a stable hand-written onesweep.
+#include "apo_device.h"
+#include "apo_internal.h"
+
+namespace apo_hip {
+
+// ---------------------------------------------------------------------------
+// Scene box: min/max of sphere extents in FP64 (algorithm.cpp:5-25), then
+// the float box, Morton scale and the 6 artificial sites (algorithm.inl:1437-1464)
+// computed by the last block to finish.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t orderedBits(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double fromOrderedBits(uint64_t o) {
+    const uint64_t b = (o >> 63) ? (o & 0x7fffffffffffffffull) : ~o;
+    return __builtin_bit_cast(double, b);
+}
+
+__device__ __forceinline__ uint32_t pcgWord(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__global__ __launch_bounds__(256) void k_scene_box(const F4* __restrict__ sites, uint32_t n, Scene* scene,
+                                                   double* partials, uint32_t* ticket, uint32_t* zero,
+                                                   uint32_t zeroWords) {
+    // 8 reduced values: min x-r, y-r, z-r, r ; max x+r, y+r, z+r, r
+    double mn[4] = {DBL_MAX, DBL_MAX, DBL_MAX, DBL_MAX};
+    double mx[4] = {-DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const F4 s = sites[i];
+        const double r = s.w;
+        const double c[3] = {s.x, s.y, s.z};
+        for (int a = 0; a < 3; a++) {
+            mn[a] = __builtin_fmin(mn[a], c[a] - r);
+            mx[a] = __builtin_fmax(mx[a], c[a] + r);
+        }
+        mn[3] = __builtin_fmin(mn[3], r);
+        mx[3] = __builtin_fmax(mx[3], r);
+    }
+    // clear the scratch words that later kernels accumulate into
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < zeroWords; w += stride) zero[w] = 0u;
+
+    for (int a = 0; a < 4; a++) {
+        for (int o = 32; o >= 1; o >>= 1) {
+            mn[a] = __builtin_fmin(mn[a], __shfl_xor(mn[a], o, 64));
+            mx[a] = __builtin_fmax(mx[a], __shfl_xor(mx[a], o, 64));
+        }
+    }
+    __shared__ double sred[4][8];
+    const uint32_t wave = threadIdx.x / 64, lane = laneId();
+    if (lane == 0)
+        for (int a = 0; a < 4; a++) {
+            sred[wave][a] = mn[a];
+            sred[wave][4 + a] = mx[a];
+        }
+    __syncthreads();
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        double r[8];
+        for (int a = 0; a < 8; a++) r[a] = sred[0][a];
+        for (uint32_t w = 1; w < blockDim.x / 64; w++)
+            for (int a = 0; a < 4; a++) {
+                r[a] = __builtin_fmin(r[a], sred[w][a]);
+                r[4 + a] = __builtin_fmax(r[4 + a], sred[w][4 + a]);
+            }
+        for (int a = 0; a < 8; a++) partials[blockIdx.x * 8 + a] = r[a];
+        __threadfence();
+        const uint32_t t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    double r[8];
+    for (int a = 0; a < 8; a++) r[a] = __hip_atomic_load(&partials[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t b = 1; b < gridDim.x; b++)
+        for (int a = 0; a < 4; a++) {
+            r[a] = __builtin_fmin(r[a], __hip_atomic_load(&partials[b * 8 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            r[4 + a] = __builtin_fmax(r[4 + a],
+                                      __hip_atomic_load(&partials[b * 8 + 4 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+    *ticket = 0u;  // ready for the next build
+    Scene sc;
+    for (int a = 0; a < 3; a++) {
+        sc.mn[a] = r[a];
+        sc.mx[a] = r[4 + a];
+        sc.fmn[a] = float(r[a]);
+        sc.fmx[a] = float(r[4 + a]);
+        sc.scale[a] = 1.f / (sc.fmx[a] - sc.fmn[a]);
+    }
+    sc.rmin = r[3];
+    sc.rmax = r[7];
+    // artificial sites: centroid +- R * axis, w = min radius, joggled by 0.1
+    const D3 cc{(sc.mn[0] + sc.mx[0]) / 2.0, (sc.mn[1] + sc.mx[1]) / 2.0, (sc.mn[2] + sc.mx[2]) / 2.0};
+    const double aabbRadius = len(sub(cc, D3{sc.mx[0], sc.mx[1], sc.mx[2]}));
+    const double radius = aabbRadius * (__builtin_sqrt(3.0) + 1e-1);
+    const double w = sc.rmin;
+    const double offs[6][4] = {{radius, 0, 0, w}, {-radius, 0, 0, w}, {0, radius, 0, w},
+                               {0, -radius, 0, w}, {0, 0, radius, w}, {0, 0, -radius, w}};
+    for (int s = 0; s < 6; s++) {
+        double v[4] = {cc.x + offs[s][0], cc.y + offs[s][1], cc.z + offs[s][2], 0.0 + offs[s][3]};
+        for (int c = 0; c < 4; c++)
+            v[c] += __builtin_ldexp(double(pcgWord(uint32_t(s * 4 + c))), -32) * (2.0 * 1e-1) + (-1e-1);
+        sc.art[s] = {v[0], v[1], v[2], v[3]};
+    }
+    *scene = sc;
+}
+
+// ---------------------------------------------------------------------------
+// Morton codes (lbvh.cu:11-67): 20 bits per axis, 60-bit code, with the 8
+// digit histograms of the radix sort accumulated on the fly.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t expandBits10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ uint32_t quantize(float v) {
+    if (!(v >= 0.f)) return 0u;
+    if (v >= 4294967040.f) return 0xffffffffu;
+    return uint32_t(v);
+}
+__device__ __forceinline__ uint64_t mortonCode(F4 s, const Scene& sc) {
+    uint32_t u[3];
+    const float c[3] = {s.x, s.y, s.z};
+    for (int a = 0; a < 3; a++) {
+        const float q = (c[a] - sc.fmn[a]) * sc.scale[a];
+        u[a] = quantize(q * 1048575.0f);
+    }
+    const uint64_t lo = expandBits10(u[0]) | (expandBits10(u[1]) << 1) | (expandBits10(u[2]) << 2);
+    const uint64_t hi = expandBits10(u[0] >> 10) | (expandBits10(u[1] >> 10) << 1) | (expandBits10(u[2] >> 10) << 2);
+    return (hi << 30) | lo;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const F4* __restrict__ sites, uint32_t n, const Scene* __restrict__ scene,
+                                                uint64_t* __restrict__ codes, uint32_t* __restrict__ indices,
+                                                uint32_t* __restrict__ hist /* [8][256] */) {
+    __shared__ uint32_t sh[kRadixPasses][256];
+    for (uint32_t t = threadIdx.x; t < kRadixPasses * 256; t += blockDim.x) (&sh[0][0])[t] = 0u;
+    __syncthreads();
+    const Scene sc = *scene;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t code = mortonCode(sites[i], sc);
+        codes[i] = code;
+        indices[i] = i;
+        for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p][(code >> (8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kRadixPasses * 256; t += blockDim.x) {
+        const uint32_t v = (&sh[0][0])[t];
+        if (v) atomicAdd(&hist[t], v);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Onesweep LSD radix sort pass: 8-bit digit, 4096 keys per tile (4 waves x
+// 16 rounds x 64 lanes).  Ranking inside a wave uses 8 ballots to find the
+// lanes holding the same digit, so equal digits keep input order (stable).
+// Tiles take an index from an atomic counter and chain their per-digit
+// prefixes with decoupled look-back.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kTileKeys = 4096;
+constexpr uint32_t kFlagAgg = 1u << 30, kFlagInc = 2u << 30, kValueMask = (1u << 30) - 1u;
+
+__global__ __launch_bounds__(256) void k_onesweep(const uint64_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
+                                                  uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
+                                                  uint32_t n, uint32_t shift, const uint32_t* __restrict__ digitHist,
+                                                  uint32_t* lookback, uint32_t* tileCounter, uint32_t numTiles,
+                                                  uint32_t* err) {
+    __shared__ uint64_t sKeys[kTileKeys];
+    __shared__ uint32_t sVals[kTileKeys];
+    __shared__ uint32_t sWave[4][256];
+    __shared__ uint32_t sTileStart[256];
+    __shared__ uint32_t sGlobal[256];
+    __shared__ uint32_t sScan[4];
+    __shared__ uint32_t sScan2[4];
+    __shared__ uint32_t sTile;
+
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = laneId();
+    if (tid == 0) sTile = atomicAdd(tileCounter, 1u);
+    sWave[0][tid] = sWave[1][tid] = sWave[2][tid] = sWave[3][tid] = 0u;
+    __syncthreads();
+    const uint32_t tile = sTile;
+    if (tile >= numTiles) {
+        // more blocks than tiles cannot happen with grid == numTiles; report it
+        if (tid == 0) atomicOr(err, kErrSortTile);
+        return;
+    }
+    const uint32_t base = tile * kTileKeys + wave * 1024u;
+
+    uint64_t key[16];
+    uint32_t val[16];
+    uint32_t rank[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t idx = base + uint32_t(r) * 64u + lane;
+        const bool valid = idx < n;
+        key[r] = valid ? keysIn[idx] : 0ull;
+        val[r] = valid ? valsIn[idx] : 0u;
+        const uint32_t d = uint32_t(key[r] >> shift) & 255u;
+        uint64_t peers = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t bb = ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t below = countBelow(peers);
+        const uint32_t prev = sWave[wave][d];
+        waveSync();
+        if (valid && below == 0) sWave[wave][d] = prev + popc64(peers);
+        waveSync();
+        rank[r] = prev + below;
+    }
+    __syncthreads();
+
+    // per-digit tile totals, per-wave exclusive offsets
+    const uint32_t d = tid;
+    const uint32_t c0 = sWave[0][d], c1 = sWave[1][d], c2 = sWave[2][d], c3 = sWave[3][d];
+    const uint32_t tileCount = c0 + c1 + c2 + c3;
+    // publish the tile aggregate, then look back for the exclusive prefix
+    uint32_t* lb = lookback + size_t(tile) * 256u + d;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        __hip_atomic_store(lb, kFlagInc | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(lb, kFlagAgg | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int t = int(tile) - 1;
+        uint32_t spins = 0;
+        while (t >= 0) {
+            const uint32_t v = __hip_atomic_load(lookback + size_t(t) * 256u + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t flag = v & ~kValueMask;
+            if (flag == 0u) {
+                if (++spins > (1u << 22)) {
+                    atomicOr(err, kErrSortSpin);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += v & kValueMask;
+            if (flag == kFlagInc) break;
+            t--;
+        }
+        __hip_atomic_store(lb, kFlagInc | (excl + tileCount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // exclusive scans over the 256 digits: global digit starts and tile digit starts
+    const uint32_t g = digitHist[d];
+    uint32_t gInc = waveInclusiveSum(g);
+    uint32_t tInc = waveInclusiveSum(tileCount);
+    if (lane == 63) {
+        sScan[wave] = gInc;
+        sScan2[wave] = tInc;
+    }
+    sWave[1][d] = c0;
+    sWave[2][d] = c0 + c1;
+    sWave[3][d] = c0 + c1 + c2;
+    sWave[0][d] = 0u;
+    __syncthreads();
+    uint32_t gOff = 0, tOff = 0;
+    for (uint32_t w = 0; w < wave; w++) {
+        gOff += sScan[w];
+        tOff += sScan2[w];
+    }
+    sGlobal[d] = gOff + gInc - g + excl;
+    sTileStart[d] = tOff + tInc - tileCount;
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t idx = base + uint32_t(r) * 64u + lane;
+        if (idx < n) {
+            const uint32_t dg = uint32_t(key[r] >> shift) & 255u;
+            const uint32_t pos = sTileStart[dg] + sWave[wave][dg] + rank[r];
+            if (pos < kTileKeys) {
+                sKeys[pos] = key[r];
+                sVals[pos] = val[r];
+            } else {
+                atomicOr(err, kErrSortScatter);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t tileBase = tile * kTileKeys;
+    const uint32_t tileValid = (n - tileBase) < kTileKeys ? (n - tileBase) : kTileKeys;
+    for (uint32_t p = tid; p < tileValid; p += 256u) {
+        const uint64_t k = sKeys[p];
+        const uint32_t dg = uint32_t(k >> shift) & 255u;
+        const uint32_t out = sGlobal[dg] + (p - sTileStart[dg]);
+        if (out < n) {
+            keysOut[out] = k;
+            valsOut[out] = sVals[p];
+        } else {
+            atomicOr(err, kErrSortScatter);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Karras 2012 hierarchy (lbvh.cu:144-223) + gather of the sorted sites
+// (setLeaves, lbvh.cu:88-114).  Thread i handles internal node i and leaf i.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int deltaK(const uint64_t* __restrict__ c, int i, int j, int n) {
+    if (j < 0 || j >= n) return -1;
+    const uint64_t a = c[i], b = c[j];
+    if (a != b) return __builtin_clzll(a ^ b);
+    return __builtin_clzll(uint64_t(int64_t(i ^ j))) + 64;
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ codes, const uint32_t* __restrict__ indices,
+                                                const F4* __restrict__ sitesIn, uint32_t n, F4* __restrict__ sorted,
+                                                Node* __restrict__ nodes, int32_t* __restrict__ parent,
+                                                uint32_t* __restrict__ refitCounter, uint32_t* err) {
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const int ni = int(n);
+    if (i >= ni) return;
+    const uint32_t src = indices[i];
+    if (src < n) {
+        sorted[i] = sitesIn[src];
+    } else {
+        sorted[i] = F4{0.f, 0.f, 0.f, 0.f};
+        atomicOr(err, kErrGatherIndex);
+    }
+    if (i >= ni - 1) return;
+    refitCounter[i] = 0u;
+    const int dd = deltaK(codes, i, i + 1, ni) - deltaK(codes, i, i - 1, ni);
+    const int d = (dd > 0) - (dd < 0);
+    const int deltaMin = deltaK(codes, i, i - d, ni);
+    int lmax = 2;
+    while (deltaK(codes, i, i + lmax * d, ni) > deltaMin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (deltaK(codes, i, i + (l + t) * d, ni) > deltaMin) l += t;
+    const int j = i + l * d;
+    const int deltaNode = deltaK(codes, i, j, ni);
+    int s = 0, k = 2, t;
+    do {
+        t = (l + k - 1) / k;
+        k <<= 1;
+        if (deltaK(codes, i, i + (s + t) * d, ni) > deltaNode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? d : 0);
+    int lc = gamma, rc = gamma + 1;
+    if ((i < j ? i : j) == gamma) lc += ni - 1;
+    if ((i > j ? i : j) == gamma + 1) rc += ni - 1;
+    nodes[i].d.x = lc;
+    nodes[i].d.y = rc;
+    parent[lc] = i;
+    parent[rc] = i;
+    if (i == 0) parent[0] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// Bottom-up refit (lbvh.cu:225-272): one thread per leaf climbs; the second
+// thread to reach a node unions both child boxes.  Child boxes and max radii
+// are written into the parent's 64-byte record.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void writeSlot(Node* nd, bool left, const float* mn, const float* mx, float maxr) {
+    float* f = reinterpret_cast<float*>(nd);
+    if (left) {
+        f[0] = mn[0]; f[1] = mn[1]; f[2] = mn[2];
+        f[3] = mx[0]; f[4] = mx[1]; f[5] = mx[2];
+        f[14] = maxr;
+    } else {
+        f[6] = mn[0]; f[7] = mn[1]; f[8] = mn[2];
+        f[9] = mx[0]; f[10] = mx[1]; f[11] = mx[2];
+        f[15] = maxr;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_refit(const F4* __restrict__ sorted, uint32_t n, Node* nodes,
+                                               const int32_t* __restrict__ parent, uint32_t* sizes,
+                                               uint32_t* refitCounter, float* rootBox, uint32_t* err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const F4 s = sorted[i];
+    const double x = s.x, y = s.y, z = s.z, r = s.w;
+    float mn[3] = {float(x - r), float(y - r), float(z - r)};
+    float mx[3] = {float(x + r), float(y + r), float(z + r)};
+    float maxr = s.w;
+    int node = int(n - 1 + i);
+    sizes[node] = 1u;
+    const int internal = int(n) - 1;
+    for (uint32_t climb = 0; climb < n; climb++) {
+        const int p = parent[node];
+        if (p < 0 || p >= internal) {
+            if (!(p == -1 && node == 0)) atomicOr(err, kErrRefitParent);
+            return;
+        }
+        Node* P = nodes + p;
+        const bool isLeft = P->d.x == node;
+        writeSlot(P, isLeft, mn, mx, maxr);
+        __threadfence();
+        const uint32_t old = atomicAdd(&refitCounter[p], 1u);
+        if (old == 0u) return;
+        __threadfence();
+        const float* f = reinterpret_cast<const float*>(P);
+        float v[16];
+        for (int c = 0; c < 16; c++) v[c] = __hip_atomic_load(f + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int c = 0; c < 3; c++) {
+            mn[c] = __builtin_fminf(v[c], v[6 + c]);
+            mx[c] = __builtin_fmaxf(v[3 + c], v[9 + c]);
+        }
+        maxr = __builtin_fmaxf(v[14], v[15]);
+        const int lc = __builtin_bit_cast(int, v[12]), rc = __builtin_bit_cast(int, v[13]);
+        if (lc < 0 || rc < 0 || lc >= 2 * int(n) - 1 || rc >= 2 * int(n) - 1) {
+            atomicOr(err, kErrRefitChild);
+            return;
+        }
+        const uint32_t sl = __hip_atomic_load(&sizes[lc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t sr = __hip_atomic_load(&sizes[rc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sizes[p] = sl + sr;
+        node = p;
+        if (p == 0) {
+            for (int c = 0; c < 3; c++) {
+                rootBox[c] = mn[c];
+                rootBox[3 + c] = mx[c];
+            }
+            rootBox[6] = maxr;
+            return;
+        }
+    }
+    atomicOr(err, kErrRefitParent);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launcher
+// ---------------------------------------------------------------------------
+hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, hipStream_t st) {
+    const uint32_t tiles = (n + kTileKeys - 1) / kTileKeys;
+    const uint32_t zeroWords = kRadixPasses * 256u + kRadixPasses + kRadixPasses * tiles * 256u;
+    hipLaunchKernelGGL(k_scene_box, dim3(kBoxBlocks), dim3(256), 0, st, sitesIn, n, b.scene, b.boxPartials, b.boxTicket,
+                       b.sortScratch, zeroWords);
+    uint32_t* hist = b.sortScratch;
+    uint32_t* tileCounters = hist + kRadixPasses * 256u;
+    uint32_t* lookback = tileCounters + kRadixPasses;
+    const uint32_t mortonBlocks = (n + 255u) / 256u < 2048u ? (n + 255u) / 256u : 2048u;
+    hipLaunchKernelGGL(k_morton, dim3(mortonBlocks), dim3(256), 0, st, sitesIn, n, b.scene, b.codesA, b.idxA, hist);
+    uint64_t* kin = b.codesA;
+    uint32_t* vin = b.idxA;
+    uint64_t* kout = b.codesB;
+    uint32_t* vout = b.idxB;
+    for (int p = 0; p < kRadixPasses; p++) {
+        hipLaunchKernelGGL(k_onesweep, dim3(tiles), dim3(256), 0, st, kin, vin, kout, vout, n, uint32_t(8 * p),
+                           hist + p * 256, lookback + size_t(p) * tiles * 256u, tileCounters + p, tiles, b.err);
+        uint64_t* tk = kin; kin = kout; kout = tk;
+        uint32_t* tv = vin; vin = vout; vout = tv;
+    }
+    // after an even number of passes the sorted data is back in A
+    const uint32_t blocks = (n + 255u) / 256u;
+    hipLaunchKernelGGL(k_karras, dim3(blocks), dim3(256), 0, st, b.codesA, b.idxA, sitesIn, n, b.sorted, b.nodes,
+                       b.parent, b.refitCounter, b.err);
+    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, b.sorted, n, b.nodes, b.parent, b.sizes,
+                       b.refitCounter, b.rootBox, b.err);
+    return hipGetLastError();
+}
+
+}  // namespace apo_hip

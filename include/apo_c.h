@@ -1,0 +1,115 @@
+/*
+ * apo_c.h -- C ABI of the MI355X (gfx950) Apollonius-diagram engine.
+ *
+ * Plain C types only: pointers, sizes and status codes.  One context owns
+ * all device memory and one HIP stream; calls on one context are not
+ * thread-safe, distinct contexts may be used from distinct threads.
+ *
+ * Each entry point names the reference interface it replaces
+ * (PlathC/apo, paths relative to the repository root).
+ */
+#ifndef APO_C_H
+#define APO_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum apo_status {
+    APO_OK = 0,
+    APO_ERR_HIP = 1,          /* a HIP runtime call failed (apo_last_error has the text) */
+    APO_ERR_OUT_OF_MEMORY = 2,
+    APO_ERR_INVALID = 3,      /* bad argument (null pointer, n < 2, unsupported k ...) */
+    APO_ERR_STATE = 4,        /* call out of order (e.g. results before a build) */
+    APO_ERR_CAPACITY = 5      /* caller buffer smaller than the reported count */
+} apo_status;
+
+typedef struct apo_ctx apo_ctx;
+
+/* Construction parameters.  Defaults (apo_default_config) reproduce
+ * AlgorithmGPU<16>(sites, knnValidationStep = 1)
+ * (apo/include/apo/gpu/algorithm.cuh:12-16, algorithm.inl:1404-1594). */
+typedef struct apo_config {
+    uint32_t k;               /* neighbours per cell (16, 24, 32, 48 or 64)           */
+    uint32_t max_iterations;  /* vertex + edge validation rounds (10, algorithm.inl:1554) */
+    uint32_t max_restarts;    /* restarts per cell per validation kernel (100, :272)  */
+    uint32_t stage_mask;      /* bit0 cells, bit1 vertex/edge validation, bit2 bisector,
+                                 bit3 stop after the kNN pass */
+    uint32_t waves_per_cu;    /* persistent-grid occupancy of the cell kernels, 0 = auto */
+    uint32_t device;          /* HIP device ordinal */
+} apo_config;
+
+/* Per-build statistics (the stage split of AlgorithmGPU::Sample,
+ * algorithm.cuh:18-27, in milliseconds, plus counts). */
+typedef struct apo_stats {
+    double ms_total;
+    double ms_lbvh;
+    double ms_knn;
+    double ms_cells;
+    double ms_validation;
+    double ms_bisector;
+    double ms_output;
+    uint64_t sites;
+    uint64_t vertices;          /* emitted Apollonius vertices */
+    uint64_t uncertified;       /* cells not certified after the kNN pass */
+    uint64_t buried;
+    uint64_t flagged_cells;     /* cells with an overflow / cap flag */
+    uint64_t reserved[4];
+} apo_stats;
+
+void apo_default_config(apo_config* cfg);
+
+/* Replaces the AlgorithmGPU<K> constructor (algorithm.inl:1404-1408). */
+apo_status apo_create(apo_ctx** ctx, const apo_config* cfg);
+void apo_destroy(apo_ctx* ctx);
+const char* apo_last_error(const apo_ctx* ctx);
+
+/* Replaces AlgorithmGPU<K>::build() (algorithm.inl:1410-1594) including the
+ * H2D copy of computeApolloniusDiagram (src/gpu/algorithm.cu:6-15).
+ * xyzr: host array of n (x, y, z, r) FP32 quadruples, read during the call. */
+apo_status apo_build(apo_ctx* ctx, const float* xyzr, uint32_t n);
+
+/* Same, for sites already resident in device memory (n float4 records). */
+apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n);
+
+/* Vertex output of AlgorithmGPU<K>::toHost (algorithm.inl:1596-1646):
+ * each Apollonius vertex once, from the cell of its smallest site,
+ * centre + empty-sphere radius in FP64, the 4 tangent sites as original
+ * input indices.  Two-call pattern: query the count, then copy. */
+apo_status apo_vertex_count(apo_ctx* ctx, uint64_t* count);
+apo_status apo_get_vertices(apo_ctx* ctx, double* xyzw, uint32_t* ids, uint64_t capacity);
+/* Device pointers to the vertex records (valid until the next build). */
+apo_status apo_get_device_vertices(apo_ctx* ctx, const double** xyzw, const uint32_t** ids);
+
+apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats);
+
+/* ---- introspection of the device state (the public members dSites,
+ * bvh, dStatus, dVertices, dEdges of AlgorithmGPU, algorithm.cuh:147-155) ---- */
+
+/* Sorted sites (Morton order) and the sorted->original permutation. */
+apo_status apo_get_sorted_sites(apo_ctx* ctx, float* xyzr, uint32_t* indices);
+/* LBVH in the reference node numbering (lbvh.cuh:12-37): internal nodes
+ * [0, n-1), leaves [n-1, 2n-1).  aabb: 6 floats per node (min, max);
+ * max_radius: largest site radius below each node. */
+apo_status apo_get_lbvh(apo_ctx* ctx, uint64_t* sorted_codes, int32_t* parent, int32_t* left, int32_t* right,
+                        float* aabb, uint32_t* sizes, float* max_radius);
+/* The 6 artificial bounding sites appended after the n input sites
+ * (algorithm.inl:1437-1464), 6 x (x, y, z, r) in FP64. */
+apo_status apo_get_artificial_sites(apo_ctx* ctx, double* xyzr24);
+/* kNN lists of the first pass, n*k sorted-space indices. */
+apo_status apo_get_knns(apo_ctx* ctx, uint32_t* knn);
+/* Per-cell counts and status (sorted space). */
+apo_status apo_get_cell_counts(apo_ctx* ctx, uint32_t* vertex_nb, uint32_t* edge_nb, uint8_t* status,
+                               float* known_radius, uint32_t* flags);
+/* Records of all cells concatenated in cell order: vertices as 4 u32
+ * (j, k, l, type | status << 8), edges as 4 u32 (j, k, vertexNb, status). */
+apo_status apo_get_all_cells(apo_ctx* ctx, uint32_t* vertices, uint32_t* edges);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APO_C_H */

@@ -1,0 +1,121 @@
+/*
+ * apo_oracle.h -- CPU restatement of apo's Apollonius-diagram construction.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle and the
+ * reported CPU baseline.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path (the HIP
+ * library under apo_amd/) never links or calls it.
+ *
+ * It restates, in plain host C++17 with IEEE FP32/FP64 arithmetic and no FMA
+ * contraction, the method of PlathC/apo:
+ *   - Morton codes, stable sort, Karras hierarchy, refit  (apo/src/gpu/lbvh.cu:11-323)
+ *   - kNN traversal + heap                                (apo/include/apo/gpu/lbvh.inl:8-105, heap.hpp:11-43)
+ *   - quadrisector / trisectorVertex / tangent planes      (apo/include/apo/gpu/math.cuh:390-780)
+ *   - updateCell / cellRadius / initial octahedron        (apo/include/apo/gpu/topology_update.inl:5-653)
+ *   - getKnns, getCells, vertex/edge/bisector validation, output
+ *                                                         (apo/include/apo/gpu/algorithm.inl:32-1307,1410-1646)
+ * The reference's 32-lane warp order is replaced by a fixed 64-wide step
+ * order with ascending lane order (the order the HIP kernels use).
+ */
+#ifndef APO_ORACLE_H
+#define APO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference sample stream (app/common/samples.cpp:64-69) ---------- */
+double orc_pcg_unit(uint32_t v);          /* ((w>>22)^w) / 4294967295 */
+double orc_pcg_ldexp(uint32_t v);         /* ldexp((w>>22)^w, -32)  (apo/src/core/utils.cpp:6-12) */
+
+/* ---- geometry (FP64), exported for property tests ------------------- */
+/* s = 4 doubles each (x,y,z,r).  x1,x2 = 4 doubles.  Returns the mask /
+ * count exactly as the reference functions do.                        */
+uint32_t orc_quadrisector(const double* s1, const double* s2, const double* s3, const double* s4,
+                          double* x1, double* x2);
+uint32_t orc_trisector(const double* s1, const double* s2, const double* s3, double* x1, double* x2);
+void     orc_tangent_planes(const double* s1, const double* s2, const double* s3, double* n1, double* n2);
+void     orc_sphere_from_points(const double* p1, const double* p2, const double* p3, const double* p4,
+                                double* out);
+
+/* ---- Morton / LBVH -------------------------------------------------- */
+/* Scene box of sphere extents in FP64, cast to float as the reference does
+ * (apo/src/gpu/algorithm.cpp:5-25, algorithm.inl:1509).                */
+void orc_scene_box(const float* xyzr, uint32_t n, float* box6 /* minx,miny,minz,maxx,maxy,maxz */,
+                   double* minmax_radius /* 2 */);
+uint64_t orc_morton64(uint32_t x, uint32_t y, uint32_t z);
+/* The 6 joggled artificial sites (algorithm.inl:1437-1464), 6 x (x,y,z,r). */
+void     orc_artificial_sites(const float* xyzr, uint32_t n, double* out24);
+void     orc_morton_codes(const float* xyzr, uint32_t n, const float* box6, uint64_t* codes);
+
+/* Full LBVH build.  Node layout as the reference: root 0, internal
+ * [0,n-1), leaves [n-1,2n-1).  aabb = 6 floats per node (min,max).     */
+int orc_lbvh_build(const float* xyzr, uint32_t n, uint64_t* sorted_codes, uint32_t* indices, int32_t* parent,
+                   int32_t* left, int32_t* right, float* aabb, uint32_t* sizes, float* max_radius);
+
+/* kNN over the BVH of sorted sites (sorted_xyzr = sites in Morton order).
+ * One pass from scratch (iteration 0).  knn = n*k indices (sorted space),
+ * knn_dist = n*k gaps.  Returns max traversal stack depth used.          */
+int orc_knn(const float* sorted_xyzr, uint32_t n, const int32_t* left, const int32_t* right, const float* aabb,
+            uint32_t k, uint32_t* knn, float* knn_dist);
+/* Brute-force kNN with the same strict acceptance (tests only). */
+void orc_knn_brute(const float* sorted_xyzr, uint32_t n, uint32_t k, uint32_t* knn, float* knn_dist);
+
+/* ---- full construction ------------------------------------------------ */
+typedef struct orc_config {
+    uint32_t k;                 /* neighbours per kNN pass (reference default 16) */
+    uint32_t vertex_capacity;   /* per-cell vertex records (152) */
+    uint32_t edge_capacity;     /* per-cell edge records (264) */
+    uint32_t new_edge_capacity; /* new-edge entries per update (128) */
+    uint32_t bisector_capacity; /* bisectors per cell in bisector validation (512) */
+    uint32_t max_iterations;    /* vertex+edge validation rounds (10) */
+    uint32_t max_restarts;      /* restarts per cell per validation kernel (100) */
+    uint32_t threads;           /* OpenMP threads, 0 = default */
+    uint32_t stage_mask;        /* bit0 knn+cells, bit1 vertex/edge validation, bit2 bisector validation */
+} orc_config;
+
+void orc_default_config(orc_config* cfg);
+
+typedef struct orc_diagram orc_diagram;
+
+orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfg);
+void         orc_free(orc_diagram* d);
+
+/* Emitted Apollonius vertices (apo toHost semantics). */
+uint64_t orc_vertex_count(const orc_diagram* d);
+void     orc_get_vertices(const orc_diagram* d, double* xyzw, uint32_t* ids);
+
+/* Per-cell state after the pipeline (sorted space). */
+void orc_get_sorted_sites(const orc_diagram* d, float* xyzr, uint32_t* indices);
+void orc_get_cell_counts(const orc_diagram* d, uint32_t* vertex_nb, uint32_t* edge_nb, uint8_t* status,
+                         float* known_radius, uint32_t* flags);
+/* Records of cell i: vertices as 4 u32 (j,k,l,type|status<<8), edges as 4 u32 (j,k,vertexNb,status). */
+void orc_get_cell(const orc_diagram* d, uint32_t i, uint32_t* vertices, uint32_t* edges);
+/* Concatenated records of all cells in cell order (counts from orc_get_cell_counts). */
+void orc_get_all_cells(const orc_diagram* d, uint32_t* vertices, uint32_t* edges);
+void orc_get_knns(const orc_diagram* d, uint32_t* knn);
+/* Stage statistics: [0] uncertified after cells, [1] tasks per validation
+ * iteration summed, [2] iterations run, [3] overflowed cells, [4] vertices
+ * emitted, [5] buried cells, [6] cells changed by bisector validation.   */
+void orc_get_stats(const orc_diagram* d, uint64_t* stats8);
+/* Certified k per cell (first neighbour count at which the security-radius
+ * test passes, 0xffffffff when the K neighbours never certify), computed
+ * with the per-neighbour test of app/benchmarks/benchmark_contributing_nearest.cu:75-113. */
+void orc_certified_k(const float* xyzr, uint32_t n, uint32_t kmax, uint32_t* out_k, uint32_t threads);
+
+/* ---- checks ------------------------------------------------------------ */
+/* Number of emitted vertices violated by some site (FP64 weighted distance
+ * strictly below the vertex radius minus tol), checked by brute force.     */
+uint64_t orc_empty_sphere_violations(const float* xyzr, uint32_t n, const double* xyzw, const uint32_t* ids,
+                                     uint64_t v, double tol, uint32_t threads);
+/* All empty tangent spheres of 4 sites, by enumeration of quadruples
+ * (tiny n only).  Returns the count; writes up to cap id quadruples sorted. */
+uint64_t orc_bruteforce_vertices(const float* xyzr, uint32_t n, uint32_t* ids, uint64_t cap, double tol);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APO_ORACLE_H */

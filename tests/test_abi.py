@@ -1,0 +1,38 @@
+"""The C-ABI library loads and exports every entry point include/apo_c.h declares
+(CPU only: no compute call needs a GPU here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from apo_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "apo_c.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(apo_[a-z_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.library_path()):
+        pytest.skip("libapo_hip.so not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_lib.library_path())
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_default_config_matches_reference_defaults():
+    if not os.path.exists(_lib.library_path()):
+        pytest.skip("libapo_hip.so not built")
+    cfg = _lib.Config()
+    _lib.lib().apo_default_config(ctypes.byref(cfg))
+    # AlgorithmGPU<16>(sites, knnValidationStep = 1), 10 validation rounds, <100 restarts
+    assert (cfg.k, cfg.max_iterations, cfg.max_restarts, cfg.stage_mask) == (16, 10, 100, 7)

@@ -79,6 +79,20 @@ class AlgorithmGPU:
         _lib.check(self._ctx, _lib.lib().apo_get_stats(self._ctx, _lib.byref(s)))
         return {name: getattr(s, name) for name, _ in s._fields_ if name != "reserved"}
 
+    def empty_sphere_violations(self, xyzw=None, ids=None, tol: float = 1e-9) -> int:
+        """FP64 empty-sphere check through the LBVH (the criterion of
+        app/precision/precision.cpp:26-100).  Default: the build's own output."""
+        cnt = _lib.c_uint64()
+        if xyzw is None:
+            st = _lib.lib().apo_check_empty_spheres(self._ctx, None, None, 0, tol, _lib.byref(cnt))
+        else:
+            x = np.ascontiguousarray(xyzw, np.float64)
+            i = np.ascontiguousarray(ids, np.uint32)
+            st = _lib.lib().apo_check_empty_spheres(self._ctx, _lib.dptr(x), _lib.uptr(i), len(x), tol,
+                                                    _lib.byref(cnt))
+        _lib.check(self._ctx, st)
+        return int(cnt.value)
+
     # -- device state (public members of AlgorithmGPU) ------------------------
     def sorted_sites(self):
         n = len(self.sites)

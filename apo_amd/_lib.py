@@ -32,7 +32,8 @@ class Stats(ctypes.Structure):
                 ("ms_bisector", ctypes.c_double), ("ms_output", ctypes.c_double),
                 ("sites", ctypes.c_uint64), ("vertices", ctypes.c_uint64), ("uncertified", ctypes.c_uint64),
                 ("buried", ctypes.c_uint64), ("flagged_cells", ctypes.c_uint64),
-                ("reserved", ctypes.c_uint64 * 4)]
+                ("vertex_records", ctypes.c_uint64), ("edge_records", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 2)]
 
 
 _vp = ctypes.c_void_p
@@ -56,6 +57,7 @@ SIGNATURES = {
     "apo_get_vertices": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64]),
     "apo_get_device_vertices": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "apo_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
+    "apo_check_empty_spheres": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64, ctypes.c_double, _u64p]),
     "apo_get_sorted_sites": (ctypes.c_int, [_vp, _f32p, _u32p]),
     "apo_get_lbvh": (ctypes.c_int, [_vp, _u64p, _i32p, _i32p, _i32p, _f32p, _u32p, _f32p]),
     "apo_get_artificial_sites": (ctypes.c_int, [_vp, _f64p]),

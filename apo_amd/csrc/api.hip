@@ -253,10 +253,10 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     APO_STAGE("cells");
     APO_HIP(hipEventRecord(ctx->ev[3], st));
     ValidationBuffers vb{ctx->tasks, ctx->counters + 1};
-    if (ctx->cfg.stage_mask & 2u) APO_HIP(validationLaunch(cc, vb, ctx->cfg.max_iterations, waves, false, st));
+    if (ctx->cfg.stage_mask & 2u) APO_HIP(validationLaunch(cc, vb, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st));
     APO_STAGE("vertex/edge validation");
     APO_HIP(hipEventRecord(ctx->ev[4], st));
-    if (ctx->cfg.stage_mask & 4u) APO_HIP(validationLaunch(cc, vb, 1, waves, true, st));
+    if (ctx->cfg.stage_mask & 4u) APO_HIP(validationLaunch(cc, vb, 1, ctx->cfg.max_restarts, true, st));
     APO_STAGE("bisector validation");
     APO_HIP(hipEventRecord(ctx->ev[5], st));
     OutputBuffers ob{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
@@ -411,16 +411,61 @@ apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats) {
     std::vector<uint32_t> fl(n);
     APO_HIP(hipMemcpy(st.data(), ctx->status, n, hipMemcpyDeviceToHost));
     APO_HIP(hipMemcpy(fl.data(), ctx->flags, size_t(n) * 4, hipMemcpyDeviceToHost));
-    uint64_t buried = 0, uncertified = 0, flagged = 0;
+    std::vector<uint32_t> vn(n), en(n);
+    APO_HIP(hipMemcpy(vn.data(), ctx->vnb, size_t(n) * 4, hipMemcpyDeviceToHost));
+    APO_HIP(hipMemcpy(en.data(), ctx->enb, size_t(n) * 4, hipMemcpyDeviceToHost));
+    uint64_t buried = 0, uncertified = 0, flagged = 0, vr = 0, er = 0;
     for (uint32_t i = 0; i < n; i++) {
         buried += st[i] == kStatusBuried;
         uncertified += st[i] == kStatusInValidation;
         flagged += fl[i] != 0;
+        vr += vn[i];
+        er += en[i];
     }
     ctx->stats.buried = buried;
     ctx->stats.uncertified = uncertified;
     ctx->stats.flagged_cells = flagged;
+    ctx->stats.vertex_records = vr;
+    ctx->stats.edge_records = er;
     *stats = ctx->stats;
+    return APO_OK;
+}
+
+apo_status apo_check_empty_spheres(apo_ctx* ctx, const double* xyzw, const uint32_t* ids, uint64_t count, double tol,
+                                   uint64_t* violations) {
+    if (!ctx || !violations || (xyzw == nullptr) != (ids == nullptr)) return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    hipStream_t st = ctx->stream;
+    const D4* dx = ctx->outXyzw;
+    const uint4* di = ctx->outIds;
+    D4* ux = nullptr;
+    uint4* ui = nullptr;
+    if (xyzw) {
+        APO_HIP(hipMalloc(reinterpret_cast<void**>(&ux), (count + 1) * sizeof(D4)));
+        APO_HIP(hipMalloc(reinterpret_cast<void**>(&ui), (count + 1) * sizeof(uint4)));
+        APO_HIP(hipMemcpy(ux, xyzw, count * sizeof(D4), hipMemcpyHostToDevice));
+        APO_HIP(hipMemcpy(ui, ids, count * sizeof(uint4), hipMemcpyHostToDevice));
+        dx = ux;
+        di = ui;
+    } else {
+        count = ctx->vertexCount;
+    }
+    unsigned long long* dv = nullptr;
+    APO_HIP(hipMalloc(reinterpret_cast<void**>(&dv), sizeof(unsigned long long)));
+    APO_HIP(hipMemsetAsync(dv, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
+    APO_HIP(checkEmptyLaunch(ctx->sorted, ctx->idxA, ctx->cellCount, ctx->nodes, ctx->n, dx, di, count, tol, dv,
+                             ctx->err, st));
+    unsigned long long bad = 0;
+    uint32_t guard = 0;
+    APO_HIP(hipMemcpyAsync(&bad, dv, sizeof(bad), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(guard), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    APO_HIP(hipFree(dv));
+    if (ux) APO_HIP(hipFree(ux));
+    if (ui) APO_HIP(hipFree(ui));
+    if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped in the empty-sphere check: " + guardText(guard));
+    *violations = bad;
     return APO_OK;
 }
 

@@ -81,7 +81,7 @@ struct ValidationBuffers {
     uint32_t* tasks;       // n
     uint32_t* counters;    // [0] task count, [1..] queue heads
 };
-hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t waves,
+hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t maxRestarts,
                             bool bisector, hipStream_t st);
 
 // Output (algorithm.inl:1235-1307): count, scan, write.
@@ -95,5 +95,9 @@ struct OutputBuffers {
 };
 hipError_t outputCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st);
 hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices, hipStream_t st);
+
+hipError_t checkEmptyLaunch(const F4* sorted, const uint32_t* indices, uint32_t* rank, const Node* nodes, uint32_t n,
+                            const D4* xyzw, const uint4* ids, uint64_t count, double tol,
+                            unsigned long long* violations, uint32_t* err, hipStream_t st);
 
 }  // namespace apo_hip

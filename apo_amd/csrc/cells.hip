@@ -17,377 +17,9 @@
 //  * a persistent grid pulls cells from an atomic queue in Morton order.
 #include "apo_device.h"
 #include "apo_internal.h"
+#include "cell_update.h"
 
 namespace apo_hip {
-
-constexpr uint32_t kListCap = 2 * kEdgeCap;
-
-struct CellLds {
-    uint4 vtx[kVertexCap];
-    uint4 edg[kEdgeCap];
-    uint32_t nb[kNeighbourCap];
-    uint32_t ne[kNewEdgeCap];
-    uint32_t list[kListCap];
-    uint32_t knn[64];
-};
-
-struct WaveCell {
-    uint32_t V, E, F, flags;
-};
-
-__device__ __forceinline__ D4 loadSite(const CellCtx& c, uint32_t id) {
-    if (id < c.n) return toD(c.sorted[id]);
-    if (id - c.n < 6u) return c.scene->art[id - c.n];
-    atomicOr(c.err, kErrCellSite);
-    return D4{0, 0, 0, 0};
-}
-
-__device__ __forceinline__ D4 vertexPosition(const CellCtx& c, const D4& si, uint4 r) {
-    D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
-    quadrisector(si, loadSite(c, r.x), loadSite(c, r.y), loadSite(c, r.z), x0, x1);
-    return (r.w & 0xffu) ? x1 : x0;
-}
-
-// Distinct sites over the edges [0, E), in first-occurrence order.
-__device__ void buildNeighbours(CellLds& L, WaveCell& w) {
-    const uint32_t lane = laneId();
-    uint32_t F = 0;
-    for (uint32_t b = 0; b < w.E; b += 64) {
-        const uint32_t e = b + lane;
-        const bool has = e < w.E;
-        uint32_t x0 = kNone, x1 = kNone;
-        if (has) {
-            const uint4 ed = L.edg[e];
-            x0 = ed.x;
-            x1 = ed.y;
-        }
-        bool f0 = has, f1 = has && x1 != x0;
-        const uint32_t lim = (w.E - b) < 64u ? w.E : b + 64u;
-        for (uint32_t e2 = 0; e2 < lim; e2++) {
-            const uint4 y = L.edg[e2];
-            if (e2 < e) {
-                if (y.x == x0 || y.y == x0) f0 = false;
-                if (y.x == x1 || y.y == x1) f1 = false;
-            }
-        }
-        const uint64_t m0 = ballot(f0), m1 = ballot(f1);
-        const uint32_t off = F + countBelow(m0) + countBelow(m1);
-        if (f0) L.nb[off] = x0;
-        if (f1) L.nb[off + (f0 ? 1u : 0u)] = x1;
-        F += popc64(m0) + popc64(m1);
-    }
-    w.F = F;
-    waveSync();
-}
-
-// Candidate sphere is clear of every neighbour except skip1/skip2
-// (topology_update.inl:578-592).  Per lane, serial over the neighbour set.
-__device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLds& L, uint32_t F, const D4& cand,
-                                                  uint32_t skip1, uint32_t skip2) {
-    for (uint32_t q = 0; q < F; q++) {
-        const uint32_t id = __builtin_amdgcn_readfirstlane(L.nb[q]);
-        if (id == skip1 || id == skip2) continue;
-        if (intersectD(loadSite(c, id), cand)) return false;
-    }
-    return true;
-}
-
-// Stable in-place removal of records flagged by `drop`, step by step.
-template <typename Drop>
-__device__ __forceinline__ uint32_t compact(uint4* rec, uint32_t count, Drop drop) {
-    const uint32_t lane = laneId();
-    uint32_t removed = 0;
-    for (uint32_t b = 0; b < count; b += 64) {
-        const uint32_t idx = b + lane;
-        const bool has = idx < count;
-        uint4 r = {0, 0, 0, 0};
-        if (has) r = rec[idx];
-        const bool d = has && drop(idx, r);
-        const uint64_t m = ballot(d);
-        waveSync();
-        if (has && !d) rec[idx - removed - countBelow(m)] = r;
-        removed += popc64(m);
-        waveSync();
-    }
-    return count - removed;
-}
-
-__device__ __forceinline__ void sort3(uint32_t& a, uint32_t& b, uint32_t& c) {
-    uint32_t t;
-    if (a > c) { t = a; a = c; c = t; }
-    if (a > b) { t = a; a = b; b = t; }
-    if (b > c) { t = b; b = c; c = t; }
-}
-
-// One incremental update of cell i by site j (topology_update.inl:49-555).
-__device__ bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, const D4& si, uint32_t j,
-                           const D4& sj) {
-    const uint32_t lane = laneId();
-    if (w.V == 0 && w.E == 0) return false;
-    {
-        const double ri = si.w, rj = sj.w;
-        const double bigger = __builtin_fmax(ri, rj), smaller = __builtin_fmin(ri, rj);
-        if (ltD(len(D3{si.x - sj.x, si.y - sj.y, si.z - sj.z}), bigger - smaller)) {
-            if (ltD(rj, ri)) return false;
-            w.V = 0;
-            w.E = 0;
-            w.F = 0;
-            return false;
-        }
-    }
-    const uint32_t baseV = w.V, baseE = w.E;
-    buildNeighbours(L, w);
-
-    // (1) vertices whose empty sphere intersects s_j
-    uint32_t nInv = 0;
-    for (uint32_t b = 0; b < baseV; b += 64) {
-        const uint32_t v = b + lane;
-        bool inv = false;
-        if (v < baseV) {
-            const uint4 r = L.vtx[v];
-            inv = intersectD(vertexPosition(c, si, r), sj);
-            if (inv) L.vtx[v].w = (r.w & 0xffu) | (0xffu << 8);
-        }
-        const uint64_t m = ballot(inv);
-        if (inv) L.list[nInv + countBelow(m)] = v;
-        nInv += popc64(m);
-    }
-    waveSync();
-    if (nInv) {
-        for (uint32_t b = 0; b < baseE; b += 64) {
-            const uint32_t e = b + lane;
-            if (e < baseE) {
-                const uint4 ed = L.edg[e];
-                uint32_t dec = 0;
-                for (uint32_t u = 0; u < nInv; u++) {
-                    const uint4 vx = L.vtx[L.list[u]];
-                    const bool e1 = ed.x == vx.x && ed.y == vx.y;
-                    const bool e2 = ed.x == vx.x && ed.y == vx.z;
-                    const bool e3 = ed.x == vx.y && ed.y == vx.z;
-                    if (e1 || e2 || e3) dec++;
-                }
-                if (dec) L.edg[e].z = ed.z - dec;
-            }
-        }
-        waveSync();
-        w.V = compact(L.vtx, w.V, [&](uint32_t idx, uint4 r) { return ((r.w >> 8) & 0xffu) == 0xffu && idx < baseV; });
-    }
-
-    // (2) candidate vertices on every edge (k, l)
-    uint32_t NE = 0, nClosed = 0;
-    for (uint32_t b = 0; b < baseE; b += 64) {
-        const uint32_t e = b + lane;
-        const bool has = e < baseE;
-        uint4 ed = {0, 0, 0, 0};
-        if (has) ed = L.edg[e];
-        const uint32_t k = ed.x, l = ed.y;
-        D4 q0{0, 0, 0, 0}, q1{0, 0, 0, 0};
-        uint32_t mask = 0;
-        D4 sk{0, 0, 0, 0}, sl{0, 0, 0, 0};
-        if (has) {
-            sk = loadSite(c, k);
-            sl = loadSite(c, l);
-            mask = quadrisector(si, sj, sk, sl, q0, q1);
-        }
-        uint32_t vnb = ed.z;
-        if (has && mask == 0 && vnb == kNone) {
-            D4 t = q0;
-            trisectorAliased(si, sk, sl, t);
-            if (intersectD(t, sj)) vnb = 0;
-        }
-        uint32_t valid = mask;
-        if (mask & 1u)
-            if (!clearOfNeighbours(c, L, w.F, q0, k, l)) valid &= ~1u;
-        if (mask & 2u)
-            if (!clearOfNeighbours(c, L, w.F, q1, k, l)) valid &= ~2u;
-        const uint32_t nNew = uint32_t(__builtin_popcount(valid));
-        if (has && mask != 0 && nNew > 0) {
-            if (vnb == kNone) vnb = 0;
-            vnb += nNew;
-        }
-        if (has && vnb != ed.z) L.edg[e].z = vnb;
-
-        const bool closedCand = has && mask == 0;
-        const uint64_t mc = ballot(closedCand);
-        if (closedCand) L.list[nClosed + countBelow(mc)] = e;
-        nClosed += popc64(mc);
-
-        const uint64_t b0 = ballot(nNew >= 1), b1 = ballot(nNew >= 2);
-        const uint32_t off = countBelow(b0) + countBelow(b1);
-        uint32_t jj = j, kk = k, ll = l;
-        sort3(jj, kk, ll);
-        uint32_t slot = w.V + off;
-        uint32_t eslot = NE + 2 * off;
-        const uint32_t two = nNew == 2 ? (1u << 30) : 0u;
-        for (uint32_t nv = 0; nv < 2; nv++) {
-            if (!((valid >> nv) & 1u)) continue;
-            if (slot < kVertexCap) L.vtx[slot] = make_uint4(jj, kk, ll, nv);
-            slot++;
-            if (eslot < kNewEdgeCap) L.ne[eslot] = k | two;
-            eslot++;
-            if (eslot < kNewEdgeCap) L.ne[eslot] = l | two;
-            eslot++;
-        }
-        const uint32_t tot = popc64(b0) + popc64(b1);
-        w.V += tot;
-        NE += 2 * tot;
-    }
-    if (w.V > kVertexCap) {
-        w.V = kVertexCap;
-        w.flags |= kFlagVertexOverflow;
-    }
-    if (NE > kNewEdgeCap) {
-        NE = kNewEdgeCap;
-        w.flags |= kFlagNewEdgeOverflow;
-    }
-    waveSync();
-
-    // (3) closed elliptic trisectors (topology_update.inl:325-433).  The
-    // candidates (edge order, endpoint order) are tested in parallel; an id
-    // already among the new-edge entries, or accepted by a lower candidate,
-    // is skipped, which yields the reference's sequential result.
-    if (nClosed) {
-        const uint32_t cands = 2 * nClosed;
-        for (uint32_t qb = 0; qb < cands; qb += 64) {
-            const uint32_t q = qb + lane;
-            const bool act = q < cands;
-            uint32_t id = kNone;
-            if (act) {
-                const uint4 ed = L.edg[L.list[q >> 1]];
-                id = (q & 1u) ? ed.y : ed.x;
-            }
-            bool ok = act;
-            for (uint32_t a = 0; a < NE && ok; a++)
-                if ((L.ne[a] & 0x3fffffffu) == id) ok = false;
-            if (ok) {
-                D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
-                const uint32_t cnt = trisector(si, sj, loadSite(c, id), x0, x1);
-                ok = cnt == 2 && clearOfNeighbours(c, L, w.F, x0, id, i);
-            }
-            // keep the first accepted lane per id
-            uint64_t m = ballot(ok);
-            uint64_t mm = m;
-            while (mm) {
-                const uint32_t t = uint32_t(__builtin_ctzll(mm));
-                mm &= mm - 1;
-                const uint32_t idt = readLane(id, t);
-                if (ok && lane > t && id == idt) ok = false;
-            }
-            m = ballot(ok);
-            const uint32_t pos = NE + countBelow(m);
-            if (ok) {
-                if (pos < kNewEdgeCap) L.ne[pos] = id | (1u << 31);
-            }
-            NE += popc64(m);
-            if (NE > kNewEdgeCap) {
-                NE = kNewEdgeCap;
-                w.flags |= kFlagNewEdgeOverflow;
-            }
-            waveSync();
-        }
-    }
-
-    // (4) old edges whose vertex count dropped to zero leave the cell
-    w.E = compact(L.edg, w.E, [&](uint32_t idx, uint4 r) { return r.z == 0u && idx < baseE; });
-
-    // (5) one new edge (j, k) per distinct k, ascending, vertexNb = run length
-    //     (or kNone for a closed trisector)
-    // first-occurrence flags go to bit 30 of the entry
-    for (uint32_t ab = 0; ab < NE; ab += 64) {
-        const uint32_t a = ab + lane;
-        bool first = a < NE;
-        uint32_t key = 0;
-        if (first) {
-            key = L.ne[a] & 0x3fffffffu;
-            for (uint32_t a2 = 0; a2 < a; a2++)
-                if ((L.ne[a2] & 0x3fffffffu) == key) {
-                    first = false;
-                    break;
-                }
-        }
-        waveSync();
-        if (a < NE) L.list[a] = first ? 1u : 0u;
-        waveSync();
-    }
-    uint32_t D = 0;
-    for (uint32_t ab = 0; ab < NE; ab += 64) {
-        const uint32_t a = ab + lane;
-        const bool first = a < NE && L.list[a];
-        if (first) {
-            const uint32_t key = L.ne[a] & 0x3fffffffu;
-            uint32_t count = 0, rank = 0;
-            bool closed = false;
-            for (uint32_t a2 = 0; a2 < NE; a2++) {
-                const uint32_t en = L.ne[a2];
-                const uint32_t k2 = en & 0x3fffffffu;
-                if (k2 == key) {
-                    count++;
-                    closed = (en >> 31) != 0u;
-                }
-                if (k2 < key && L.list[a2]) rank++;
-            }
-            const uint32_t slot = w.E + rank;
-            if (slot < kEdgeCap)
-                L.edg[slot] = make_uint4(j < key ? j : key, j < key ? key : j, closed ? kNone : count, 0u);
-        }
-        D += popc64(ballot(first));
-    }
-    w.E += D;
-    if (w.E > kEdgeCap) {
-        w.E = kEdgeCap;
-        w.flags |= kFlagEdgeOverflow;
-    }
-    waveSync();
-    return D != 0;
-}
-
-// cellRadius (topology_update.inl:595-653)
-__device__ double cellRadius(const CellCtx& c, const CellLds& L, const WaveCell& w, const D4& si) {
-    const uint32_t lane = laneId();
-    double maxR = 0.0;
-    for (uint32_t b = 0; b < w.V; b += 64) {
-        const uint32_t v = b + lane;
-        if (v < w.V) maxR = __builtin_fmax(maxR, vertexPosition(c, si, L.vtx[v]).w);
-    }
-    for (uint32_t b = 0; b < w.E; b += 64) {
-        const uint32_t e = b + lane;
-        if (e < w.E) {
-            const uint4 ed = L.edg[e];
-            D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
-            if (trisector(si, loadSite(c, ed.x), loadSite(c, ed.y), x0, x1) == 2)
-                maxR = __builtin_fmax(maxR, __builtin_fmax(x0.w, x1.w));
-        }
-    }
-    return waveMaxD(maxR);
-}
-
-// The initial cell: octahedron of the 6 artificial sites (algorithm.inl:1472-1499)
-__device__ __forceinline__ void initCell(CellLds& L, WaveCell& w, uint32_t n) {
-    const uint32_t lane = laneId();
-    const uint8_t av[8][3] = {{0, 2, 4}, {0, 2, 5}, {0, 3, 4}, {0, 3, 5}, {1, 2, 4}, {1, 2, 5}, {1, 3, 4}, {1, 3, 5}};
-    const uint8_t ae[12][2] = {{0, 2}, {2, 4}, {1, 2}, {2, 5}, {1, 4}, {1, 5},
-                               {0, 5}, {0, 4}, {1, 3}, {3, 5}, {0, 3}, {3, 4}};
-    if (lane < 8) L.vtx[lane] = make_uint4(n + av[lane][0], n + av[lane][1], n + av[lane][2], 0u);
-    if (lane < 12) L.edg[lane] = make_uint4(n + ae[lane][0], n + ae[lane][1], 2u, 0u);
-    w.V = 8;
-    w.E = 12;
-    w.F = 0;
-    w.flags = 0;
-    waveSync();
-}
-
-__device__ __forceinline__ void storeCell(const CellCtx& c, const CellLds& L, const WaveCell& w, uint32_t i) {
-    const uint32_t lane = laneId();
-    uint4* gv = c.vtx + size_t(i) * kVertexCap;
-    uint4* ge = c.edg + size_t(i) * kEdgeCap;
-    for (uint32_t v = lane; v < w.V; v += 64) gv[v] = L.vtx[v];
-    for (uint32_t e = lane; e < w.E; e += 64) ge[e] = L.edg[e];
-    if (lane == 0) {
-        c.vnb[i] = w.V;
-        c.enb[i] = w.E;
-        if (w.flags) c.flags[i] |= w.flags;
-    }
-}
 
 // Block b -> cell index so that each of the 8 XCDs (which receive blocks
 // round-robin) works through one contiguous Morton range of cells and its

@@ -1,8 +1,572 @@
-// validation.hip -- placeholder, replaced by the validation kernels.
+// validation.hip -- completion of uncertified cells on gfx950.
+//
+// Reference: apo/include/apo/gpu/algorithm.inl:208-476 (globalVertexValidation),
+// 478-857 (globalEdgeValidation), 859-1233 (globalBisectorValidation) and the
+// iteration loop of AlgorithmGPU::build, :1551-1593.
+//
+// One wave per task cell (the grid is the task list, no persistent loop).
+// Each lane runs its own BVH query (vertex / edge / bisector) with its stack
+// in LDS ([slot][lane]); the contributors it finds are applied to the cell
+// in ascending lane order with the wave-wide updateCell of cells.hip.  Steps
+// are 64 records wide.
 #include "apo_device.h"
 #include "apo_internal.h"
+#include "cell_update.h"
+
 namespace apo_hip {
-hipError_t validationLaunch(const CellCtx&, const ValidationBuffers&, uint32_t, uint32_t, bool, hipStream_t) {
+
+struct ValLds {
+    CellLds cell;
+    uint32_t stNode[kStackDepth][64];
+    float stDist[kStackDepth][64];
+    uint32_t bis[kBisectorCap];
+};
+
+__device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
+    const uint32_t lane = laneId();
+    const uint32_t E = c.enb[i];
+    const uint32_t V = withVertices ? c.vnb[i] : 0u;
+    const uint4* ge = c.edg + size_t(i) * kEdgeCap;
+    const uint4* gv = c.vtx + size_t(i) * kVertexCap;
+    for (uint32_t e = lane; e < E; e += 64) L.edg[e] = ge[e];
+    for (uint32_t v = lane; v < V; v += 64) L.vtx[v] = gv[v];
+    w.V = V;
+    w.E = E;
+    w.F = 0;
+    w.flags = 0;
+    waveSync();
+}
+
+// j is an endpoint of a current edge (wave-wide; algorithm.inl:368-376)
+__device__ __forceinline__ bool knownSite(const CellLds& L, const WaveCell& w, uint32_t j) {
+    const uint32_t lane = laneId();
+    bool hit = false;
+    for (uint32_t b = 0; b < w.E; b += 64) {
+        const uint32_t e = b + lane;
+        if (e < w.E) {
+            const uint4 ed = L.edg[e];
+            hit = hit || ed.x == j || ed.y == j;
+        }
+    }
+    return ballot(hit) != 0;
+}
+
+// same test from one lane, over the neighbour set
+__device__ __forceinline__ bool knownLane(const CellLds& L, uint32_t F, uint32_t j) {
+    for (uint32_t q = 0; q < F; q++)
+        if (L.nb[q] == j) return true;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Nearest contributing site to a vertex sphere (algorithm.inl:305-414)
+// ---------------------------------------------------------------------------
+__device__ uint32_t vertexSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, uint4 vx, F4 cur,
+                                 uint32_t& flags) {
+    const uint32_t lane = laneId();
+    const uint32_t n = c.n;
+    ValLds& M = const_cast<ValLds&>(S);
+    uint32_t sp = 0, node = 0, nb = kNone;
+    float nd = FLT_MAX;
+    while (true) {
+        while (node < n - 1) {
+            const Node nn = c.nodes[node];
+            Box lb, rb;
+            nodeBoxes(nn, lb, rb);
+            const uint32_t L = uint32_t(nn.d.x), R = uint32_t(nn.d.y);
+            if (L >= 2 * n - 1 || R >= 2 * n - 1) {
+                atomicOr(c.err, kErrKnnChild);
+                node = 2 * n;
+                break;
+            }
+            const float dL = boxDistance(lb, cur), dR = boxDistance(rb, cur);
+            const bool lc = dL < dR;
+            const float minFar = lc ? dR : dL;
+            const uint32_t farChild = lc ? R : L;
+            const float minClose = lc ? dL : dR;
+            const uint32_t closeChild = lc ? L : R;
+            const bool ef = minFar < nd, ec = minClose < nd;
+            if (!ef && !ec) break;
+            node = ec ? closeChild : farChild;
+            if (ec && ef) {
+                if (sp < kStackDepth) {
+                    M.stNode[sp][lane] = farChild;
+                    M.stDist[sp][lane] = minFar;
+                    sp++;
+                } else {
+                    flags |= kFlagStackOverflow;
+                }
+            }
+        }
+        if (node >= n - 1 && node < 2 * n - 1) {
+            const uint32_t j = node - (n - 1);
+            if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried &&
+                !knownLane(S.cell, F, j)) {
+                const float d = sphereDistF(c.sorted[j], cur);
+                if (d < nd) {
+                    nd = d;
+                    nb = j;
+                }
+            }
+        }
+        bool found = false;
+        while (sp > 0) {
+            sp--;
+            if (M.stDist[sp][lane] > nd) continue;
+            node = M.stNode[sp][lane];
+            found = true;
+            break;
+        }
+        if (!found) break;
+    }
+    return nb;
+}
+
+__global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint32_t* __restrict__ tasks,
+                                                          uint32_t maxRestarts) {
+    __shared__ ValLds S;
+    CellLds& L = S.cell;
+    const uint32_t lane = laneId();
+    const uint32_t i = tasks[blockIdx.x];
+    const uint8_t st = c.status[i];
+    if (st == kStatusCertified || st == kStatusBuried) return;
+    WaveCell w;
+    loadCell(c, L, w, i, true);
+    const float kr = c.knownRadii[i];
+    const D4 si = loadSite(c, i);
+    uint32_t vs = 0, restarts = 0;
+    uint32_t flags = 0;
+    while (vs < w.V / 64 + 1 && restarts < maxRestarts) {
+        const uint32_t v = vs * 64 + lane;
+        vs++;
+        buildNeighbours(L, w);
+        bool need = v < w.V && ((L.vtx[v].w >> 8) & 0xffu) == 0u;
+        D4 cur{0, 0, 0, 0};
+        uint32_t nb = kNone;
+        if (need) {
+            const uint4 r = L.vtx[v];
+            cur = vertexPosition(c, si, r);
+            if (2.0 * cur.w < double(kr)) need = false;
+            if (need) nb = vertexSearch(c, S, w.F, i, r, toF(cur), flags);
+            L.vtx[v].w = (r.w & 0xffu) | (1u << 8);
+        }
+        bool invalid = false;
+        if (nb != kNone) invalid = intersectD(cur, loadSite(c, nb));
+        const uint64_t m = ballot(invalid);
+        waveSync();
+        uint64_t mm = m;
+        while (mm) {
+            const uint32_t t = uint32_t(__builtin_ctzll(mm));
+            mm &= mm - 1;
+            const uint32_t j = readLane(nb, t);
+            if (knownSite(L, w, j)) continue;
+            updateCell(c, L, w, i, si, j, loadSite(c, j));
+        }
+        if (m) {
+            vs = 0;
+            restarts++;
+        }
+    }
+    if (restarts >= maxRestarts) flags |= kFlagRestartCap;
+    uint32_t allFlags = flags;
+    for (int o = 32; o >= 1; o >>= 1) allFlags |= __shfl_xor(allFlags, o, 64);
+    w.flags |= allFlags;
+    storeCell(c, L, w, i);
+}
+
+// ---------------------------------------------------------------------------
+// Contributor search around an edge (algorithm.inl:478-857)
+// ---------------------------------------------------------------------------
+struct EdgeQuery {
+    F4 bs;
+    bool hyperbola;
+    F3 n1, n2, pp1, pp2;
+};
+
+__device__ uint32_t edgeSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, const D4& si, uint32_t k,
+                               uint32_t l, const EdgeQuery& q, uint32_t& flags) {
+    const uint32_t lane = laneId();
+    const uint32_t n = c.n;
+    ValLds& M = const_cast<ValLds&>(S);
+    uint32_t sp = 0, node = 0;
+    const D4 sk = loadSite(c, k), sl = loadSite(c, l);
+    while (true) {
+        while (node < n - 1) {
+            const Node nn = c.nodes[node];
+            Box lb, rb;
+            nodeBoxes(nn, lb, rb);
+            const uint32_t L = uint32_t(nn.d.x), R = uint32_t(nn.d.y);
+            if (L >= 2 * n - 1 || R >= 2 * n - 1) {
+                atomicOr(c.err, kErrKnnChild);
+                node = 2 * n;
+                break;
+            }
+            const F3 cpL = closestPoint(lb, q.bs), cpR = closestPoint(rb, q.bs);
+            const float dL = boxDistance(lb, q.bs), dR = boxDistance(rb, q.bs);
+            const bool lc = dL < dR;
+            const float minFar = lc ? dR : dL;
+            const F3 farPoint = lc ? cpR : cpL;
+            const uint32_t farChild = lc ? R : L;
+            const float minClose = lc ? dL : dR;
+            const F3 closePoint = lc ? cpL : cpR;
+            const uint32_t closeChild = lc ? L : R;
+            bool ef = minFar < 0.f;
+            if (q.hyperbola && ef)
+                ef = dotf(q.n1, subf(farPoint, q.pp1)) < 0.f && dotf(q.n2, subf(farPoint, q.pp2)) < 0.f;
+            bool ec = minClose < 0.f;
+            if (q.hyperbola && ec)
+                ec = dotf(q.n1, subf(closePoint, q.pp1)) < 0.f && dotf(q.n2, subf(closePoint, q.pp2)) < 0.f;
+            if (!ef && !ec) break;
+            node = ec ? closeChild : farChild;
+            if (ec && ef) {
+                if (sp < kStackDepth) {
+                    M.stNode[sp][lane] = farChild;
+                    sp++;
+                } else {
+                    flags |= kFlagStackOverflow;
+                }
+            }
+        }
+        if (node >= n - 1 && node < 2 * n - 1) {
+            const uint32_t j = node - (n - 1);
+            bool cc = j != i && j != k && j != l && c.status[j] != kStatusBuried;
+            if (cc) {
+                const F4 fsj = c.sorted[j];
+                cc = intersectF(fsj, q.bs);
+                if (q.hyperbola) {
+                    const F3 cj = xyzf(fsj);
+                    cc = cc && dotf(q.n1, subf(cj, q.pp1)) < -fsj.w && dotf(q.n2, subf(cj, q.pp2)) < -fsj.w;
+                }
+            }
+            if (cc && knownLane(S.cell, F, j)) cc = false;
+            if (cc) {
+                D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
+                uint32_t mask = quadrisector(si, loadSite(c, j), sk, sl, x0, x1);
+                if (mask != 3) mask = 0;
+                uint32_t valid = mask;
+                if ((mask & 1u) && !clearOfNeighbours(c, S.cell, F, x0, k, l)) valid &= ~1u;
+                if ((mask & 2u) && !clearOfNeighbours(c, S.cell, F, x1, k, l)) valid &= ~2u;
+                if (valid) return j;
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        node = M.stNode[sp][lane];
+    }
+    return kNone;
+}
+
+__device__ __forceinline__ uint32_t waveOr(uint32_t v) {
+    for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Store the edges and append the vertices created during this kernel after
+// the cell's stored vertices (algorithm.inl:840-855, 1216-1231).
+__device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L, WaveCell& w, uint32_t i,
+                                              bool didChange) {
+    const uint32_t lane = laneId();
+    const uint32_t baseV = c.vnb[i];
+    uint32_t total = baseV + w.V;
+    if (total > kVertexCap) {
+        total = kVertexCap;
+        w.flags |= kFlagVertexOverflow;
+    }
+    uint4* gv = c.vtx + size_t(i) * kVertexCap;
+    uint4* ge = c.edg + size_t(i) * kEdgeCap;
+    for (uint32_t v = lane; baseV + v < total; v += 64) gv[baseV + v] = L.vtx[v];
+    for (uint32_t e = lane; e < w.E; e += 64) ge[e] = L.edg[e];
+    waveSync();
+    if (lane == 0) {
+        c.status[i] = didChange ? kStatusInValidation : kStatusCertified;
+        c.enb[i] = w.E;
+        c.vnb[i] = total;
+        if (w.flags) c.flags[i] |= w.flags;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_t* __restrict__ tasks,
+                                                        uint32_t maxRestarts) {
+    __shared__ ValLds S;
+    CellLds& L = S.cell;
+    const uint32_t lane = laneId();
+    const uint32_t i = tasks[blockIdx.x];
+    const uint8_t st = c.status[i];
+    if (st == kStatusCertified || st == kStatusBuried) return;
+    WaveCell w;
+    loadCell(c, L, w, i, false);
+    for (uint32_t e = lane; e < w.E; e += 64)
+        if (L.edg[e].w != 1u) L.edg[e].w = 2u;
+    waveSync();
+    const float kr = c.knownRadii[i];
+    const D4 si = loadSite(c, i);
+    const F4 fsi = toF(si);
+    uint32_t flags = 0;
+    bool didChange = false;
+    const uint32_t stepNb = w.E / 64 + 1;
+    uint32_t es = 0, restarts = 0;
+    while (es < stepNb) {
+        buildNeighbours(L, w);
+        const uint32_t e = es * 64 + lane;
+        const bool inRange = e < w.E;
+        bool need = inRange && L.edg[e].w == 2u;
+        uint32_t k = kNone, l = kNone;
+        if (inRange) {
+            k = L.edg[e].x;
+            l = L.edg[e].y;
+        }
+        EdgeQuery q{{0, 0, 0, 0}, false, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (need) {
+            const D4 sk = loadSite(c, k), sl = loadSite(c, l);
+            D4 ev0{0, 0, 0, 0}, ev1{0, 0, 0, 0};
+            const uint32_t type = trisector(si, sk, sl, ev0, ev1);
+            q.hyperbola = type == 1;
+            if (type == 2) {
+                const F4 v1 = toF(ev0), v2 = toF(ev1);
+                const F3 c1 = xyzf(v1), c2 = xyzf(v2);
+                const F3 center = mulf(addf(c1, c2), .5f);
+                const float radius = lenf(subf(center, c1)) + v1.w;
+                q.bs = {center.x, center.y, center.z, radius};
+            } else {
+                D3 nn1, nn2;
+                tangentPlanes(si, sk, sl, nn1, nn2);
+                q.n1 = toF3(nn1);
+                q.n2 = toF3(nn2);
+                const D3 p1 = add(xyz(si), mul(nn1, si.w));
+                const D3 p2 = add(xyz(sk), mul(nn1, sk.w));
+                const D3 p3 = add(xyz(sl), mul(nn1, sl.w));
+                const D3 p4 = add(xyz(si), mul(nn2, si.w));
+                const D3 p5 = add(xyz(sk), mul(nn2, sk.w));
+                q.bs = toF(sphereFromPoints(p1, p2, p3, p5));
+                q.pp1 = toF3(p1);
+                q.pp2 = toF3(p4);
+            }
+        }
+        const bool inSec = ltF(lenf(subf(xyzf(fsi), xyzf(q.bs))), kr - q.bs.w);
+        if (inRange && inSec) {
+            L.edg[e].w = 1u;
+            need = false;
+        }
+        uint32_t found = kNone;
+        if (need) {
+            found = edgeSearch(c, S, w.F, i, si, k, l, q, flags);
+            if (found == kNone) L.edg[e].w = 1u;
+        }
+        const uint64_t m = ballot(found != kNone);
+        waveSync();
+        bool changed = false;
+        uint64_t mm = m;
+        while (mm) {
+            const uint32_t t = uint32_t(__builtin_ctzll(mm));
+            mm &= mm - 1;
+            const uint32_t j = readLane(found, t);
+            if (knownSite(L, w, j)) continue;
+            const bool ch = updateCell(c, L, w, i, si, j, loadSite(c, j));
+            didChange = didChange || ch;
+            changed = changed || ch;
+        }
+        if (changed && restarts < maxRestarts) {
+            es = 1;
+            restarts++;
+        } else {
+            if (changed) flags |= kFlagRestartCap;
+            es++;
+        }
+    }
+    w.flags |= waveOr(flags);
+    storeAppended(c, L, w, i, didChange);
+}
+
+// ---------------------------------------------------------------------------
+// Contributor search along a bisector (algorithm.inl:859-1233)
+// ---------------------------------------------------------------------------
+__device__ uint32_t bisectorSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, const D4& si,
+                                   uint32_t bk, F3 fci, F3 fck, F4 bs, float cylR, uint32_t& flags) {
+    const uint32_t lane = laneId();
+    const uint32_t n = c.n;
+    ValLds& M = const_cast<ValLds&>(S);
+    const D4 sk = loadSite(c, bk);
+    uint32_t sp = 0, node = 0;
+    while (true) {
+        while (node < n - 1) {
+            const Node nn = c.nodes[node];
+            Box lb, rb;
+            nodeBoxes(nn, lb, rb);
+            const uint32_t L = uint32_t(nn.d.x), R = uint32_t(nn.d.y);
+            if (L >= 2 * n - 1 || R >= 2 * n - 1) {
+                atomicOr(c.err, kErrKnnChild);
+                node = 2 * n;
+                break;
+            }
+            const float dL = boxDistance(lb, bs), dR = boxDistance(rb, bs);
+            const bool lc = dL < dR;
+            const float minFar = lc ? dR : dL;
+            const uint32_t farChild = lc ? R : L;
+            const float minClose = lc ? dL : dR;
+            const uint32_t closeChild = lc ? L : R;
+            const bool ef = minFar < 0.f, ec = minClose < 0.f;
+            if (!ef && !ec) break;
+            node = ec ? closeChild : farChild;
+            if (ec && ef) {
+                if (sp < kStackDepth) {
+                    M.stNode[sp][lane] = farChild;
+                    sp++;
+                } else {
+                    flags |= kFlagStackOverflow;
+                }
+            }
+        }
+        if (node >= n - 1 && node < 2 * n - 1) {
+            const uint32_t j = node - (n - 1);
+            bool cc = j != i && j != bk;
+            if (cc) {
+                const F4 fsj = c.sorted[j];
+                const F3 fcj = xyzf(fsj);
+                cc = bs.w >= fsj.w && (lenf(subf(xyzf(bs), fcj)) <= (bs.w - fsj.w));
+                if (cc) {
+                    const F3 itoj = subf(fcj, fci);
+                    const float v = dotf(itoj, normalizef(subf(fck, fci)));
+                    const float d = __builtin_sqrtf(dotf(itoj, itoj) - v * v);
+                    cc = cc && (d + fsj.w) < cylR;
+                }
+            }
+            if (cc && knownLane(S.cell, F, j)) cc = false;
+            if (cc) {
+                D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
+                const uint32_t cnt = trisector(si, loadSite(c, j), sk, x0, x1);
+                cc = cnt == 2 && clearOfNeighbours(c, S.cell, F, x1, bk, bk);
+                if (cc) return j;
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        node = M.stNode[sp][lane];
+    }
+    return kNone;
+}
+
+__global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t maxRestarts) {
+    __shared__ ValLds S;
+    CellLds& L = S.cell;
+    const uint32_t lane = laneId();
+    const uint32_t i = blockIdx.x;
+    const uint8_t st = c.status[i];
+    if (st == kStatusBuried) return;
+    WaveCell w;
+    loadCell(c, L, w, i, false);
+    uint32_t flags = 0;
+    // sorted distinct neighbours above i (algorithm.inl:903-968)
+    buildNeighbours(L, w);
+    const uint32_t F = w.F;
+    uint32_t B = 0;
+    for (uint32_t b = 0; b < F; b += 64) {
+        const uint32_t q = b + lane;
+        uint32_t id = kNone;
+        bool keep = false;
+        if (q < F) {
+            id = L.nb[q];
+            keep = id < c.n && id > i;
+        }
+        uint32_t rank = 0;
+        if (keep)
+            for (uint32_t q2 = 0; q2 < F; q2++) {
+                const uint32_t o = L.nb[q2];
+                if (o < c.n && o > i && o < id) rank++;
+            }
+        if (keep && rank < kBisectorCap) S.bis[rank] = id;
+        B += popc64(ballot(keep));
+    }
+    if (B > kBisectorCap) {
+        B = kBisectorCap;
+        flags |= kFlagBisectorOverflow;
+    }
+    waveSync();
+    // the vertex list of this kernel starts empty (algorithm.inl:970-971)
+    w.V = 0;
+    const float kr = c.knownRadii[i];
+    const D4 si = loadSite(c, i);
+    const F3 fci{float(si.x), float(si.y), float(si.z)};
+    const float fri = float(si.w);
+    bool didChange = false;
+    const uint32_t stepNb = B / 64 + 1;
+    uint32_t bsi = 0, restarts = 0;
+    while (bsi < stepNb) {
+        buildNeighbours(L, w);
+        const uint32_t bid = bsi * 64 + lane;
+        bool need = bid < B && S.bis[bid] != kNone;
+        uint32_t found = kNone;
+        if (need) {
+            const uint32_t bk = S.bis[bid];
+            const D4 sk = loadSite(c, bk);
+            const F3 fck{float(sk.x), float(sk.y), float(sk.z)};
+            const float frk = float(sk.w);
+            const float dist = lenf(subf(fci, fck)) + fri + frk;
+            if (ltF(dist, kr)) {
+                S.bis[bid] = kNone;
+                need = false;
+            }
+            if (need) {
+                const F3 center = mulf(addf(fci, fck), .5f);
+                const float radius = lenf(subf(center, fci)) + __builtin_fmaxf(fri, frk);
+                const F4 bsph{center.x, center.y, center.z, radius};
+                const float cylR = __builtin_fmaxf(fri, frk);
+                found = bisectorSearch(c, S, w.F, i, si, bk, fci, fck, bsph, cylR, flags);
+                if (found == kNone) S.bis[bid] = kNone;
+            }
+        }
+        const uint64_t m = ballot(found != kNone);
+        waveSync();
+        bool changed = false;
+        uint64_t mm = m;
+        while (mm) {
+            const uint32_t t = uint32_t(__builtin_ctzll(mm));
+            mm &= mm - 1;
+            const uint32_t j = readLane(found, t);
+            if (knownSite(L, w, j)) continue;
+            const bool ch = updateCell(c, L, w, i, si, j, loadSite(c, j));
+            didChange = didChange || ch;
+            changed = changed || ch;
+        }
+        if (changed && restarts < maxRestarts) {
+            bsi = 1;
+            restarts++;
+        } else {
+            if (changed) flags |= kFlagRestartCap;
+            bsi++;
+        }
+    }
+    w.flags |= waveOr(flags);
+    storeAppended(c, L, w, i, didChange);
+}
+
+__global__ void k_compact_tasks(const uint8_t* __restrict__ status, uint32_t n, uint32_t* tasks, uint32_t* count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && status[i] == kStatusInValidation) tasks[atomicAdd(count, 1u)] = i;
+}
+
+__global__ void k_zero1(uint32_t* p) { *p = 0u; }
+
+hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t maxRestarts,
+                            bool bisector, hipStream_t st) {
+    if (bisector) {
+        hipLaunchKernelGGL(k_bisector_validation, dim3(c.n), dim3(64), 0, st, c, maxRestarts);
+        return hipGetLastError();
+    }
+    for (uint32_t it = 0; it < iterations; it++) {
+        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, v.counters);
+        hipLaunchKernelGGL(k_compact_tasks, dim3((c.n + 255) / 256), dim3(256), 0, st, c.status, c.n, v.tasks,
+                           v.counters);
+        uint32_t count = 0;
+        hipError_t e = hipMemcpyAsync(&count, v.counters, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+        if (count == 0) break;
+        hipLaunchKernelGGL(k_vertex_validation, dim3(count), dim3(64), 0, st, c, v.tasks, maxRestarts);
+        hipLaunchKernelGGL(k_edge_validation, dim3(count), dim3(64), 0, st, c, v.tasks, maxRestarts);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     return hipSuccess;
 }
+
 }  // namespace apo_hip

@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Throughput of the gfx950 Apollonius-diagram construction on one MI355X.
+
+Workload: 1,000,000 sites, centres uniform in [0,1]^3, radii log-uniform in
+[1e-4, 1e-2] (FP32, seed 3).  One step = one full construction (scene box,
+LBVH, kNN, cell update, vertex/edge/bisector validation, vertex output
+compaction) from sites resident in HBM to the compacted vertex buffer in
+HBM.  value = cells per second over the timed steps.
+
+Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=60_000, help="sites in the CPU-baseline sub-cube")
+    return ap.parse_args()
+
+
+def cpu_baseline(sites: np.ndarray, target: int):
+    """Time the oracle (host C++/OpenMP restatement of the same construction)
+    on a sub-cube of the workload holding about `target` sites: same density
+    and radius distribution, bounded CPU time."""
+    import oracle
+    edge = (target / len(sites)) ** (1.0 / 3.0)
+    sub = sites[np.all(sites[:, :3] < edge, axis=1)]
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    d = oracle.Diagram(sub, threads=threads)
+    dt = time.perf_counter() - t0
+    v = d.stats()["vertices"]
+    return {"value": len(sub) / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"{len(sub)} sites of the workload inside [0,{edge:.4f}]^3 (same density and radii), "
+                      f"full construction in {dt:.2f} s, {v} vertices"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import apo_amd
+    from apo_amd import _lib, datasets
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    n = args.n
+    sites = datasets.loguniform_sites(n, seed=3)
+    dev = torch.from_numpy(sites).to("cuda")
+    torch.cuda.synchronize()
+
+    cfg = _lib.Config()
+    _lib.lib().apo_default_config(_lib.byref(cfg))
+    cfg.k = args.k
+    ctx = _lib.create(cfg)
+    L = _lib.lib()
+    ptr = _lib.ctypes.c_void_p(dev.data_ptr())
+
+    def step():
+        _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stage = {k: 0.0 for k in ("ms_total", "ms_lbvh", "ms_knn", "ms_cells", "ms_validation", "ms_bisector",
+                              "ms_output")}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1000.0 * wall / args.steps
+
+    s = _lib.Stats()
+    _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+    vertices = int(s.vertices)
+    for k in stage:
+        stage[k] = float(getattr(s, k))
+
+    # stage times are HIP events recorded on the library's stream (last step);
+    # the dominant kernel is the stage with the largest time
+    parts = {"lbvh": s.ms_lbvh, "knn": s.ms_knn, "cells": s.ms_cells, "validation": s.ms_validation,
+             "bisector": s.ms_bisector, "output": s.ms_output}
+    dominant = max(parts, key=parts.get)
+    # algorithmic HBM bytes per cell of each stage
+    cell_bytes = 16 + args.k * (4 + 16) + 16 * ((s.vertex_records + s.edge_records) / max(n, 1)) + 5
+    bytes_per_cell = {"lbvh": 440.0, "knn": 16 + 4 * args.k, "cells": cell_bytes,
+                      "validation": cell_bytes, "bisector": cell_bytes, "output": 48.0 * vertices / n}
+    achieved = bytes_per_cell[dominant] * n / (parts[dominant] * 1e-3) / 1e9
+    lbvh_gbs = 440.0 * n / (s.ms_lbvh * 1e-3) / 1e9
+
+    violations = None
+    if not args.no_check:
+        cnt = _lib.c_uint64()
+        _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
+        violations = int(cnt.value)
+
+    out = {
+        "metric": "cells/sec at 1M sites: learned-k vs plain on 1 GPU, 0 empty-sphere violations",
+        "value": n / (ms_per_step / 1000.0),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n} sites, centres U[0,1]^3, radii log-uniform [1e-4,1e-2], seed 3, plain k={args.k}",
+                   "path": "plain", "sites": n, "k": args.k},
+        "vertices_per_s": vertices / (ms_per_step / 1000.0),
+        "vertices": vertices,
+        "stage_ms": stage,
+        "buried_sites": int(s.buried),
+        "flagged_cells": int(s.flagged_cells),
+        "empty_sphere_violations": violations,
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(sites, args.cpu_sample)
+    L.apo_destroy(ctx)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -57,7 +57,9 @@ typedef struct apo_stats {
     uint64_t uncertified;       /* cells not certified after the kNN pass */
     uint64_t buried;
     uint64_t flagged_cells;     /* cells with an overflow / cap flag */
-    uint64_t reserved[4];
+    uint64_t vertex_records;    /* sum over cells of stored vertex records */
+    uint64_t edge_records;      /* sum over cells of stored edge records */
+    uint64_t reserved[2];
 } apo_stats;
 
 void apo_default_config(apo_config* cfg);
@@ -85,6 +87,14 @@ apo_status apo_get_vertices(apo_ctx* ctx, double* xyzw, uint32_t* ids, uint64_t 
 apo_status apo_get_device_vertices(apo_ctx* ctx, const double** xyzw, const uint32_t** ids);
 
 apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats);
+
+/* FP64 empty-sphere check through the LBVH of the last build, with the
+ * criterion of app/precision/precision.cpp:26-100: a vertex is violated when
+ * a site other than its 4 tangent sites lies closer than (its largest
+ * tangent distance - tol).  xyzw/ids: host arrays of `count` vertices
+ * (original site ids); pass NULL for both to check the build's own output. */
+apo_status apo_check_empty_spheres(apo_ctx* ctx, const double* xyzw, const uint32_t* ids, uint64_t count, double tol,
+                                   uint64_t* violations);
 
 /* ---- introspection of the device state (the public members dSites,
  * bvh, dStatus, dVertices, dEdges of AlgorithmGPU, algorithm.cuh:147-155) ---- */

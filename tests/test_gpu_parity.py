@@ -13,9 +13,9 @@ from apo_amd import AlgorithmGPU, datasets
 pytestmark = pytest.mark.gpu
 
 CASES = {
+    "uniform_small_257": lambda: datasets.uniform_sites(257, seed=11, r_min=0.01, r_max=0.05),
     "uniform_10k": lambda: datasets.uniform_sites(10_000, seed=1),
     "loguniform_20k": lambda: datasets.loguniform_sites(20_000, seed=3),
-    "uniform_small_257": lambda: datasets.uniform_sites(257, seed=11, r_min=0.01, r_max=0.05),
 }
 
 
@@ -27,9 +27,9 @@ def _say(msg):
 def built(request):
     name = request.param
     sites = CASES[name]()
-    _say(f"[case] {name} n={len(sites)} stage_mask=1")
-    gpu = AlgorithmGPU(sites, stage_mask=1).build()
-    orc = oracle.Diagram(sites, stage_mask=1)
+    _say(f"[case] {name} n={len(sites)} full pipeline")
+    gpu = AlgorithmGPU(sites).build()
+    orc = oracle.Diagram(sites)
     yield name, sites, gpu, orc
     gpu.close()
 
@@ -46,6 +46,7 @@ def test_lbvh_bit_exact(built):
         assert np.array_equal(got[key], ref[key]), (name, key)
     assert np.array_equal(got["aabb"].view(np.uint32), ref["aabb"].view(np.uint32)), name
     assert np.array_equal(got["max_radius"], ref["max_radius"]), name
+    assert np.array_equal(gpu.artificial_sites().view(np.uint64), oracle.artificial_sites(sites).view(np.uint64))
 
 
 def test_knn_exact(built):
@@ -70,5 +71,32 @@ def test_vertices_exact(built):
     name, sites, gpu, orc = built
     g = gpu.to_host()
     ox, oi = orc.vertices()
+    assert len(g.vertices) > 0
     assert np.array_equal(g.vertices_id, oi), name
     assert np.array_equal(g.vertices.view(np.uint64), ox.view(np.uint64)), name
+
+
+def test_empty_sphere(built):
+    name, sites, gpu, orc = built
+    assert gpu.empty_sphere_violations() == 0, name
+    if len(sites) <= 10_000:
+        g = gpu.to_host()
+        assert oracle.empty_sphere_violations(sites, g.vertices, g.vertices_id) == 0, name
+
+
+def test_checker_counts_planted_violations(built):
+    # move a third of the vertex centres: the tangent-site radius is
+    # recomputed from the moved centre, so other sites fall inside; the BVH
+    # checker must report exactly the brute-force count of the oracle
+    name, sites, gpu, orc = built
+    if len(sites) > 10_000:
+        pytest.skip("brute-force reference limited to small sets")
+    g = gpu.to_host()
+    x = g.vertices.copy()
+    rng = np.random.default_rng(7)
+    pick = rng.random(len(x)) < 0.33
+    x[pick, :3] += rng.uniform(-0.03, 0.03, (pick.sum(), 3))
+    want = oracle.empty_sphere_violations(sites, x, g.vertices_id)
+    got = gpu.empty_sphere_violations(x, g.vertices_id)
+    assert want > 0
+    assert got == want, (name, got, want)

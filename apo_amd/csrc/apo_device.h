@@ -72,6 +72,15 @@ __device__ __forceinline__ F3 normalizef(F3 v) {
     return {v.x * inv, v.y * inv, v.z * inv};
 }
 __device__ __forceinline__ F3 xyzf(F4 a) { return {a.x, a.y, a.z}; }
+// Component-wise selects: a conditional on aggregates would select between
+// addresses and push both operands to private memory.
+__device__ __forceinline__ F3 selF3(bool c, F3 a, F3 b) { return {c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+__device__ __forceinline__ F4 selF4(bool c, F4 a, F4 b) {
+    return {c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w};
+}
+__device__ __forceinline__ D4 selD4(bool c, D4 a, D4 b) {
+    return {c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w};
+}
 __device__ __forceinline__ F4 toF(D4 a) { return {float(a.x), float(a.y), float(a.z), float(a.w)}; }
 __device__ __forceinline__ F3 toF3(D3 a) { return {float(a.x), float(a.y), float(a.z)}; }
 __device__ __forceinline__ D4 toD(F4 a) { return {double(a.x), double(a.y), double(a.z), double(a.w)}; }
@@ -100,6 +109,7 @@ __device__ __forceinline__ bool intersectD(D4 a, D4 b) { return ltD(sphereDistD(
 // boxes (apo/include/apo/gpu/math.inl:79-101), FP32
 // ---------------------------------------------------------------------------
 struct Box { F3 mn, mx; };
+__device__ __forceinline__ Box selBox(bool c, const Box& a, const Box& b) { return {selF3(c, a.mn, b.mn), selF3(c, a.mx, b.mx)}; }
 
 __device__ __forceinline__ F3 closestPoint(const Box& b, F4 s) {
     return {__builtin_fminf(__builtin_fmaxf(s.x, b.mn.x), b.mx.x),

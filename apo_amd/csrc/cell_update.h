@@ -39,7 +39,7 @@ __device__ __forceinline__ D4 loadSite(const CellCtx& c, uint32_t id) {
 __device__ __forceinline__ D4 vertexPosition(const CellCtx& c, const D4& si, uint4 r) {
     D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
     quadrisector(si, loadSite(c, r.x), loadSite(c, r.y), loadSite(c, r.z), x0, x1);
-    return (r.w & 0xffu) ? x1 : x0;
+    return selD4((r.w & 0xffu) != 0u, x1, x0);
 }
 
 // Distinct sites over the edges [0, E), in first-occurrence order.

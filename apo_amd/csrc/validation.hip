@@ -12,14 +12,21 @@
 #include "apo_device.h"
 #include "apo_internal.h"
 #include "cell_update.h"
+#include "edge_bound.h"
 
 namespace apo_hip {
 
 struct ValLds {
     CellLds cell;
-    uint32_t stNode[kStackDepth][64];
-    float stDist[kStackDepth][64];
+    union {
+        struct {
+            uint32_t stNode[kStackDepth][64];
+            float stDist[kStackDepth][64];
+        };
+        CoverLds cover;  // edge covers are built while no traversal runs
+    };
     uint32_t bis[kBisectorCap];
+    CoverOut coverOut;
 };
 
 __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
@@ -181,9 +188,11 @@ struct EdgeQuery {
     F4 bs;
     bool hyperbola;
     F3 n1, n2, pp1, pp2;
+    int cover;                // balls in `ball` (0: no pruning)
+    F4 ball[kCoverBalls];     // every contributor j has |p_j - c| - r_j <= r for one ball (edge_bound.h)
 };
 
-__device__ uint32_t edgeSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, const D4& si, uint32_t k,
+__device__ __forceinline__ uint32_t edgeSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, const D4& si, uint32_t k,
                                uint32_t l, const EdgeQuery& q, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
@@ -205,10 +214,10 @@ __device__ uint32_t edgeSearch(const CellCtx& c, const ValLds& S, uint32_t F, ui
             const float dL = boxDistance(lb, q.bs), dR = boxDistance(rb, q.bs);
             const bool lc = dL < dR;
             const float minFar = lc ? dR : dL;
-            const F3 farPoint = lc ? cpR : cpL;
+            const F3 farPoint = selF3(lc, cpR, cpL);
             const uint32_t farChild = lc ? R : L;
             const float minClose = lc ? dL : dR;
-            const F3 closePoint = lc ? cpL : cpR;
+            const F3 closePoint = selF3(lc, cpL, cpR);
             const uint32_t closeChild = lc ? L : R;
             bool ef = minFar < 0.f;
             if (q.hyperbola && ef)
@@ -216,6 +225,21 @@ __device__ uint32_t edgeSearch(const CellCtx& c, const ValLds& S, uint32_t F, ui
             bool ec = minClose < 0.f;
             if (q.hyperbola && ec)
                 ec = dotf(q.n1, subf(closePoint, q.pp1)) < 0.f && dotf(q.n2, subf(closePoint, q.pp2)) < 0.f;
+            if (q.cover) {
+                const float lr = __builtin_bit_cast(float, nn.d.z), rr = __builtin_bit_cast(float, nn.d.w);
+                const Box fb = selBox(lc, rb, lb), cb = selBox(lc, lb, rb);
+                const float fr = lc ? rr : lr, cr = lc ? lr : rr;
+                bool tf = false, tc = false;
+#pragma unroll
+                for (int m = 0; m < kCoverBalls; m++) {
+                    if (m < q.cover) {
+                        tf = tf || boxDistance(fb, q.ball[m]) <= fr;
+                        tc = tc || boxDistance(cb, q.ball[m]) <= cr;
+                    }
+                }
+                ef = ef && tf;
+                ec = ec && tc;
+            }
             if (!ef && !ec) break;
             node = ec ? closeChild : farChild;
             if (ec && ef) {
@@ -315,7 +339,13 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
             k = L.edg[e].x;
             l = L.edg[e].y;
         }
-        EdgeQuery q{{0, 0, 0, 0}, false, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        EdgeQuery q;
+        q.bs = {0, 0, 0, 0};
+        q.hyperbola = false;
+        q.n1 = q.n2 = q.pp1 = q.pp2 = {0, 0, 0};
+        q.cover = 0;
+#pragma unroll
+        for (int m = 0; m < kCoverBalls; m++) q.ball[m] = {0, 0, 0, 0};
         if (need) {
             const D4 sk = loadSite(c, k), sl = loadSite(c, l);
             D4 ev0{0, 0, 0, 0}, ev1{0, 0, 0, 0};
@@ -346,6 +376,19 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
         if (inRange && inSec) {
             L.edg[e].w = 1u;
             need = false;
+        }
+        // searches whose sphere is large get a cover of the edge's empty
+        // spheres, built by the whole wave one edge at a time
+        uint64_t bm = ballot(need && q.bs.w > kCoverRatio * kr);
+        while (bm) {
+            const uint32_t t = uint32_t(__builtin_ctzll(bm));
+            bm &= bm - 1;
+            const int cnt = waveEdgeCover(c, L, S.cover, S.coverOut, w.F, si, readLane(k, t), readLane(l, t));
+            const bool mine = cnt > 0 && lane == t;
+            q.cover = mine ? cnt : q.cover;
+#pragma unroll
+            for (int m = 0; m < kCoverBalls; m++) q.ball[m] = selF4(mine, S.coverOut.ball[m], q.ball[m]);
+            waveSync();
         }
         uint32_t found = kNone;
         if (need) {

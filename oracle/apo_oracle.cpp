@@ -868,7 +868,241 @@ struct EdgeQuery {
     F4 bs;
     bool hyperbola;
     F3 n1, n2, pp1, pp2;
+    int cover = 0;  // balls in `ball` (0: no pruning)
+    F4 ball[8];     // every contributor j has |p_j - c| - r_j <= r for one ball
 };
+constexpr int kCoverBalls = 8;
+// edge searches whose sphere exceeds this multiple of the known radius get a cover
+constexpr float kCoverRatio = 2.0f;
+struct EdgeCover {
+    int count;
+    F4 ball[kCoverBalls];
+};
+
+// ---------------------------------------------------------------------------
+// Cover of the empty spheres of a cell edge.  No counterpart in the
+// reference: an exact pruning aid for the edge contributor search, which
+// must leave every search result unchanged (the traversal keeps its order
+// and only skips subtrees that hold no possible contributor).
+//
+// The centres of the spheres tangent to s1, s2, s3 with radius w are
+// c(w) = p1 + u(w) +- h(w) n, where n is the unit normal of the centres'
+// plane, u(w) = P + Q w lies in that plane and h(w)^2 = A w^2 + B w + Cc
+// (u is linear in w because subtracting the tangency equations pairwise
+// leaves two linear equations).  Each sign of h traces a piece of the
+// trisector parametrised by w.  A neighbour s becomes tangent where
+//     |c(w) - p_s| = r_s + w  <=>  L0 + L1 w = +-2 sigma h(w),
+// sigma = n.(p_s - p1); squared, a quadratic in w whose roots are all the
+// points where the gap of s changes sign.
+// ---------------------------------------------------------------------------
+struct TriCurve {
+    D3 P, Q, n;  // P, Q relative to p1
+    double A, B, Cc;
+};
+
+bool triCurve(const D4& s1, const D4& s2, const D4& s3, TriCurve& t) {
+    const D3 q2 = sub(xyz(s2), xyz(s1)), q3 = sub(xyz(s3), xyz(s1));
+    const D3 nc = cross(q2, q3);
+    const double nl = len(nc);
+    if (!(nl > 1e-12 * len(q2) * len(q3))) return false;  // collinear centres
+    t.n = mul(nc, 1.0 / nl);
+    // rows: -q2 . u = b0x + b1x w, -q3 . u = b0y + b1y w, n . u = 0
+    const D3 r1{-q2.x, -q2.y, -q2.z}, r2{-q3.x, -q3.y, -q3.z}, r3 = t.n;
+    const double b0x = .5 * (-dot(q2, q2) - s1.w * s1.w + s2.w * s2.w);
+    const double b0y = .5 * (-dot(q3, q3) - s1.w * s1.w + s3.w * s3.w);
+    const double b1x = -(s1.w - s2.w), b1y = -(s1.w - s3.w);
+    const D3 c23 = cross(r2, r3), c31 = cross(r3, r1);
+    const double det = dot(r1, c23);
+    if (!(std::fabs(det) > 0.0)) return false;
+    const double inv = 1.0 / det;
+    t.P = mul(add(mul(c23, b0x), mul(c31, b0y)), inv);
+    t.Q = mul(add(mul(c23, b1x), mul(c31, b1y)), inv);
+    t.A = 1.0 - dot(t.Q, t.Q);
+    t.B = 2.0 * (s1.w - dot(t.P, t.Q));
+    t.Cc = s1.w * s1.w - dot(t.P, t.P);
+    return std::isfinite(t.A) && std::isfinite(t.B) && std::isfinite(t.Cc) && std::isfinite(dot(t.P, t.P) + dot(t.Q, t.Q));
+}
+
+// Real roots of a w^2 + b w + c, counting a slightly negative discriminant
+// (rounding) as a double root.  Returns -1 when every w is a root.
+int realRoots(double a, double b, double c, double r[2]) {
+    const double sa = std::fabs(a), sb = std::fabs(b), sc = std::fabs(c);
+    const double scale = std::fmax(sa, std::fmax(sb, sc));
+    if (!(scale > 0.0)) return -1;
+    if (sa <= 1e-13 * scale) {
+        if (sb <= 1e-13 * scale) return -1;
+        r[0] = -c / b;
+        return 1;
+    }
+    double disc = b * b - 4.0 * a * c;
+    if (disc < 0.0) {
+        if (disc < -1e-10 * (b * b + std::fabs(4.0 * a * c))) return 0;
+        disc = 0.0;
+    }
+    const double q = -.5 * (b + std::copysign(std::sqrt(disc), b));
+    r[0] = q / a;
+    r[1] = (q != 0.0) ? c / q : r[0];
+    return 2;
+}
+
+template <class Nb>
+bool clearOfAll(const Context& C, const Nb& N, const D4& x, uint32_t skip, double tol) {
+    for (uint32_t s : N)
+        if (s != skip && sphereDistD(C.S[s], x) < -tol) return false;
+    return true;
+}
+
+// Cover of the empty spheres of edge (k,l): balls such that every sphere the
+// contributor test can accept lies inside one of them.  The neighbours'
+// tangency roots split each piece of the trisector into intervals on which
+// no neighbour changes side, so one midpoint test per interval and half tells
+// whether the interval can hold an accepted sphere.  An interval [wa, wb] of
+// one half (h monotone on it) lies in the ball centred at c(wm), wm the
+// midpoint, of radius |Q| (wb - wa) / 2 + max |h(w) - h(wm)| + wb.  A
+// contributor j touches an accepted sphere, so |p_j - centre| - r_j <= radius
+// for one ball.  false: no cover could be certified (search unpruned).
+bool edgeCover(const Context& C, const Cell& cell, const D4& si, uint32_t k, uint32_t l, EdgeCover& cov) {
+    cov.count = 0;
+    TriCurve t;
+    if (!triCurve(si, C.S[k], C.S[l], t)) return false;
+    std::vector<uint32_t> N;
+    N.reserve(2 * cell.E.size());
+    for (const Edg& e : cell.E) {
+        for (uint32_t s : {e.j, e.k}) {
+            if (s == k || s == l) continue;
+            bool dup = false;
+            for (uint32_t m : N) dup = dup || m == s;
+            if (!dup) N.push_back(s);
+        }
+    }
+    const D3 p1 = xyz(si);
+    const double r1 = si.w;
+    const double tolBase = r1 + len(t.P);
+    auto hAt = [&](double w) { return std::sqrt(std::fmax(t.A * w * w + t.B * w + t.Cc, 0.0)); };
+    auto point = [&](double w, double sgn) -> D4 {
+        const D3 c = add(p1, add(add(t.P, mul(t.Q, w)), mul(t.n, sgn * hAt(w))));
+        return {c.x, c.y, c.z, w};
+    };
+    auto tolAt = [&](double w) { return 1e-9 + 1e-7 * (std::fabs(w) + tolBase); };
+    // some neighbour intersects the sphere at x by more than margin + tol
+    auto clearlyInvalid = [&](const D4& x, double margin) {
+        for (uint32_t s : N)
+            if (sphereDistD(C.S[s], x) < -(margin + tolAt(x.w))) return true;
+        return false;
+    };
+
+    // pieces of the w axis where the trisector is real (h^2 >= 0); accepted
+    // spheres have w = |p1 - c| - r1 >= -r1
+    double tr[2];
+    const int nt = realRoots(t.A, t.B, t.Cc, tr);
+    if (nt < 0 || nt == 1) return false;
+    if (nt == 2 && !(std::isfinite(tr[0]) && std::isfinite(tr[1]))) return false;
+    if (nt == 2 && tr[0] > tr[1]) std::swap(tr[0], tr[1]);
+    const double inf = std::numeric_limits<double>::infinity();
+    struct Span { double a, b; };
+    std::vector<Span> pieces;
+    if (t.A < 0.0) {
+        if (nt != 2) return false;
+        const double a = std::fmax(tr[0], -r1);
+        if (!(tr[1] > a)) return false;
+        pieces.push_back({a, tr[1]});
+    } else {
+        if (nt == 2) {
+            if (tr[0] > -r1) pieces.push_back({-r1, tr[0]});
+            pieces.push_back({std::fmax(tr[1], -r1), inf});
+        } else {
+            pieces.push_back({-r1, inf});
+        }
+    }
+    // breakpoints: tangency roots of every neighbour and the extremum of h^2
+    std::vector<double> roots;
+    for (uint32_t s : N) {
+        const D4 ss = C.S[s];
+        const D3 qs = sub(xyz(ss), p1);
+        const double sigma = dot(t.n, qs);
+        const double L0 = dot(qs, qs) - 2.0 * dot(t.P, qs) + r1 * r1 - ss.w * ss.w;
+        const double L1 = -2.0 * dot(t.Q, qs) + 2.0 * (r1 - ss.w);
+        const double s4 = 4.0 * sigma * sigma;
+        double r[2];
+        const int nr = realRoots(L1 * L1 - s4 * t.A, 2.0 * L0 * L1 - s4 * t.B, L0 * L0 - s4 * t.Cc, r);
+        if (nr < 0) return false;
+        for (int m = 0; m < nr; m++) {
+            if (!std::isfinite(r[m])) return false;
+            roots.push_back(r[m]);
+        }
+    }
+    roots.push_back(-t.B / (2.0 * t.A));
+    if (!std::isfinite(roots.back())) return false;
+    std::sort(roots.begin(), roots.end());
+
+    // possibly-valid runs of intervals, split at the extremum of h^2
+    struct Run { double a, b, sgn; };
+    std::vector<Run> runs;
+    const double wv = -t.B / (2.0 * t.A);
+    for (const Span& pc : pieces) {
+        std::vector<double> bp{pc.a};
+        for (double r : roots)
+            if (r > pc.a && r < pc.b) bp.push_back(r);
+        bp.push_back(pc.b);
+        for (double sgn : {1.0, -1.0}) {
+            bool open = false;
+            for (size_t m = 0; m + 1 < bp.size(); m++) {
+                const double u = bp[m], v = bp[m + 1];
+                bool keep;
+                if (v == inf) {
+                    const double ws = u + .5 * std::fmax(u + r1, 0.0) + 1e-6 * (1.0 + r1);
+                    if (!clearlyInvalid(point(ws, sgn), 0.0)) return false;  // may run to infinity
+                    keep = false;
+                } else {
+                    // A wide interval has one sign per neighbour; on a narrow one
+                    // the computed roots may be off by more than its width, so
+                    // the midpoint must clear every point of it: the spheres
+                    // on [u, v] move by at most the displacement bound plus the
+                    // change of w.
+                    const double wm = .5 * (u + v), hm = hAt(wm);
+                    double margin = 0.0;
+                    if (v - u <= 1e-6 * (1.0 + std::fabs(u) + r1))
+                        margin = len(t.Q) * (v - u) * .5 + std::fmax(std::fabs(hAt(u) - hm), std::fabs(hAt(v) - hm)) +
+                                 (v - u) * .5;
+                    keep = !clearlyInvalid(point(wm, sgn), margin);
+                }
+                if (keep && open && u != wv) {
+                    runs.back().b = v;
+                } else if (keep) {
+                    runs.push_back({u, v, sgn});
+                }
+                open = keep;
+            }
+        }
+    }
+    if (runs.empty()) return false;
+    const double m1 = std::fmax(std::fabs(si.x), std::fmax(std::fabs(si.y), std::fabs(si.z)));
+    if (int(runs.size()) > kCoverBalls) {
+        // one ball around s_i reaching the largest accepted radius
+        double W = -DBL_MAX;
+        for (const Run& r : runs) W = std::fmax(W, r.b);
+        cov.ball[0] = {float(si.x), float(si.y), float(si.z),
+                       float((r1 + 2.0 * W) * (1.0 + 1e-4) + 1e-6 * (1.0 + m1))};
+        cov.count = 1;
+        return true;
+    }
+    const int split = kCoverBalls / int(runs.size());
+    const double qn = len(t.Q);
+    for (const Run& r : runs) {
+        for (int m = 0; m < split; m++) {
+            const double wa = r.a + (r.b - r.a) * double(m) / double(split);
+            const double wb = (m + 1 == split) ? r.b : r.a + (r.b - r.a) * double(m + 1) / double(split);
+            const double wm = .5 * (wa + wb);
+            const D4 cm = point(wm, r.sgn);
+            const double hm = hAt(wm);
+            const double rad = qn * (wb - wa) * .5 + std::fmax(std::fabs(hAt(wa) - hm), std::fabs(hAt(wb) - hm)) + wb;
+            const double mc = std::fmax(std::fabs(cm.x), std::fmax(std::fabs(cm.y), std::fabs(cm.z)));
+            cov.ball[cov.count++] = {float(cm.x), float(cm.y), float(cm.z),
+                                     float(rad * (1.0 + 1e-4) + 1e-6 * (1.0 + mc + std::fabs(rad)))};
+        }
+    }
+    return true;
+}
 
 // Contributor search around an edge (algorithm.inl:616-793)
 uint32_t edgeSearch(const Context& C, const Cell& cell, uint32_t i, const D4& si, uint32_t k, uint32_t l,
@@ -897,6 +1131,17 @@ uint32_t edgeSearch(const Context& C, const Cell& cell, uint32_t i, const D4& si
             bool ec = minClose < 0.f;
             if (q.hyperbola && ec)
                 ec = ec && dotf(q.n1, subf(closePoint, q.pp1)) < 0.f && dotf(q.n2, subf(closePoint, q.pp2)) < 0.f;
+            if (q.cover) {
+                const Box fb = lc ? rb : lb, cb = lc ? lb : rb;
+                const float fr = b.maxr[farChild], cr = b.maxr[closeChild];
+                bool tf = false, tc = false;
+                for (int m = 0; m < q.cover; m++) {
+                    tf = tf || boxDistance(fb, q.ball[m]) <= fr;
+                    tc = tc || boxDistance(cb, q.ball[m]) <= cr;
+                }
+                ef = ef && tf;
+                ec = ec && tc;
+            }
             if (!ef && !ec) break;
             node = ec ? closeChild : farChild;
             if (ec && ef) {
@@ -1123,6 +1368,13 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
                     needSearch = false;
                 }
                 if (!needSearch) continue;
+                if (C.cfg.edge_bound == 2 || (C.cfg.edge_bound == 1 && q.bs.w > kCoverRatio * kr)) {
+                    EdgeCover cov;
+                    if (edgeCover(C, cell, si, k, l, cov)) {
+                        q.cover = cov.count;
+                        for (int m = 0; m < cov.count; m++) q.ball[m] = cov.ball[m];
+                    }
+                }
                 const uint32_t c = edgeSearch(C, cell, i, si, k, l, q, flags);
                 if (c == kNone) cell.E[eid].status = 1;
                 else found[nFound++] = c;
@@ -1401,6 +1653,7 @@ void orc_default_config(orc_config* cfg) {
     cfg->max_restarts = 100;
     cfg->threads = 0;
     cfg->stage_mask = 7;
+    cfg->edge_bound = 0;
 }
 
 orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfgIn) {

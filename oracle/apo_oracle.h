@@ -74,6 +74,9 @@ typedef struct orc_config {
     uint32_t max_restarts;      /* restarts per cell per validation kernel (100) */
     uint32_t threads;           /* OpenMP threads, 0 = default */
     uint32_t stage_mask;        /* bit0 knn+cells, bit1 vertex/edge validation, bit2 bisector validation */
+    uint32_t edge_bound;        /* edge searches pruned by the cover of the edge's empty spheres: 0 never
+                                   (the reference's search), 1 when the search sphere exceeds 2x the known
+                                   radius, 2 always; results are identical in every mode */
 } orc_config;
 
 void orc_default_config(orc_config* cfg);

@@ -16,7 +16,12 @@ CASES = {
     "uniform_small_257": lambda: datasets.uniform_sites(257, seed=11, r_min=0.01, r_max=0.05),
     "uniform_10k": lambda: datasets.uniform_sites(10_000, seed=1),
     "loguniform_20k": lambda: datasets.loguniform_sites(20_000, seed=3),
+    "loguniform_100k": lambda: datasets.loguniform_sites(100_000, seed=3),
 }
+
+# Above this size the oracle runs its edge searches with the edge radius
+# bound, which tests/test_oracle_edge_bound.py pins to the reference search.
+BOUND_ABOVE = 30_000
 
 
 def _say(msg):
@@ -29,7 +34,7 @@ def built(request):
     sites = CASES[name]()
     _say(f"[case] {name} n={len(sites)} full pipeline")
     gpu = AlgorithmGPU(sites).build()
-    orc = oracle.Diagram(sites)
+    orc = oracle.Diagram(sites, edge_bound=1 if len(sites) > BOUND_ABOVE else 0)
     yield name, sites, gpu, orc
     gpu.close()
 

@@ -33,6 +33,10 @@ class Stats(ctypes.Structure):
                 ("sites", ctypes.c_uint64), ("vertices", ctypes.c_uint64), ("uncertified", ctypes.c_uint64),
                 ("buried", ctypes.c_uint64), ("flagged_cells", ctypes.c_uint64),
                 ("vertex_records", ctypes.c_uint64), ("edge_records", ctypes.c_uint64),
+                ("ms_vertex_validation_kernel", ctypes.c_double), ("ms_edge_validation_kernel", ctypes.c_double),
+                ("validation_launches", ctypes.c_uint64), ("validation_tasks", ctypes.c_uint64),
+                ("bytes_cells", ctypes.c_uint64), ("bytes_vertex_validation", ctypes.c_uint64),
+                ("bytes_edge_validation", ctypes.c_uint64), ("bytes_bisector_validation", ctypes.c_uint64),
                 ("reserved", ctypes.c_uint64 * 2)]
 
 

@@ -54,6 +54,8 @@ struct apo_ctx {
     double* boxPartials = nullptr;
     uint32_t* boxTicket = nullptr;
     uint32_t* sortScratch = nullptr;
+    uint32_t* range = nullptr;
+    uint32_t* crossList = nullptr;
     uint64_t* codesA = nullptr;
     uint64_t* codesB = nullptr;
     uint32_t* idxA = nullptr;
@@ -85,7 +87,9 @@ struct apo_ctx {
     uint64_t outCap = 0;
     uint64_t vertexCount = 0;
 
+    unsigned long long* bytes = nullptr;  // per-kernel algorithmic byte counters
     hipEvent_t ev[8] = {};
+    ValidationTiming vt{};
     apo_stats stats{};
 };
 
@@ -125,6 +129,7 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n) {
     if (n <= ctx->cap && K <= ctx->capK) return APO_OK;
     const uint32_t cap = n;
     const size_t nodes = 2 * size_t(cap) - 1;
+    APO_HIP(alloc(ctx->bytes, size_t(kByteFamilies) * kByteSlots));
     APO_HIP(alloc(ctx->sitesIn, cap));
     APO_HIP(alloc(ctx->scene, 1));
     APO_HIP(alloc(ctx->boxPartials, size_t(kBoxBlocks) * 8));
@@ -132,7 +137,10 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n) {
     // the ticket is self-resetting; clear it once, ordered on the build stream
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->boxTicket, 1u, 0u);
     APO_HIP(hipGetLastError());
-    APO_HIP(alloc(ctx->sortScratch, size_t(kRadixPasses) * 256 + kRadixPasses + size_t(kRadixPasses) * tilesFor(cap) * 256));
+    APO_HIP(alloc(ctx->sortScratch,
+                  size_t(kRadixPasses) * 256 + kRadixPasses + size_t(kRadixPasses) * tilesFor(cap) * 256 + 1));
+    APO_HIP(alloc(ctx->range, 2 * size_t(cap)));
+    APO_HIP(alloc(ctx->crossList, cap));
     APO_HIP(alloc(ctx->codesA, cap));
     APO_HIP(alloc(ctx->codesB, cap));
     APO_HIP(alloc(ctx->idxA, cap));
@@ -179,6 +187,7 @@ CellCtx cellCtx(apo_ctx* ctx) {
     c.edg = ctx->edg;
     c.enb = ctx->enb;
     c.err = ctx->err;
+    c.bytes = ctx->bytes;
     return c;
 }
 
@@ -227,12 +236,17 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     ctx->n = n;
     LbvhBuffers lb{ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA, ctx->codesB,
                    ctx->idxA,  ctx->idxB,        ctx->sorted,    ctx->nodes,       ctx->parent, ctx->sizes,
-                   ctx->refitCounter, ctx->rootBox, ctx->err};
+                   ctx->refitCounter, ctx->rootBox, ctx->err, ctx->range, ctx->crossList};
     const CellCtx cc = cellCtx(ctx);
     const uint32_t K = ctx->cfg.k;
     const uint32_t waves = persistentWaves(ctx);
 
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
+    constexpr uint32_t byteWords = 2u * kByteFamilies * kByteSlots;
+    hipLaunchKernelGGL(k_fill_u32, dim3((byteWords + 63u) / 64u), dim3(64), 0, st,
+                       reinterpret_cast<uint32_t*>(ctx->bytes), byteWords, 0u);
+    ctx->vt.rounds = 0;
+    ctx->vt.totalTasks = 0;
     APO_HIP(hipEventRecord(ctx->ev[0], st));
     APO_HIP(lbvhBuild(lb, sitesDev, n, st));
     APO_STAGE("lbvh");
@@ -253,10 +267,11 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     APO_STAGE("cells");
     APO_HIP(hipEventRecord(ctx->ev[3], st));
     ValidationBuffers vb{ctx->tasks, ctx->counters + 1};
-    if (ctx->cfg.stage_mask & 2u) APO_HIP(validationLaunch(cc, vb, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st));
+    if (ctx->cfg.stage_mask & 2u)
+        APO_HIP(validationLaunch(cc, vb, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st, &ctx->vt));
     APO_STAGE("vertex/edge validation");
     APO_HIP(hipEventRecord(ctx->ev[4], st));
-    if (ctx->cfg.stage_mask & 4u) APO_HIP(validationLaunch(cc, vb, 1, ctx->cfg.max_restarts, true, st));
+    if (ctx->cfg.stage_mask & 4u) APO_HIP(validationLaunch(cc, vb, 1, ctx->cfg.max_restarts, true, st, nullptr));
     APO_STAGE("bisector validation");
     APO_HIP(hipEventRecord(ctx->ev[5], st));
     OutputBuffers ob{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
@@ -294,6 +309,24 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     S.ms_output = ms[5];
     S.sites = n;
     S.vertices = total;
+    for (uint32_t r = 0; r < ctx->vt.rounds; r++) {
+        float a = 0.f, b = 0.f;
+        APO_HIP(hipEventElapsedTime(&a, ctx->vt.ev[r][0], ctx->vt.ev[r][1]));
+        APO_HIP(hipEventElapsedTime(&b, ctx->vt.ev[r][1], ctx->vt.ev[r][2]));
+        S.ms_vertex_validation_kernel += a;
+        S.ms_edge_validation_kernel += b;
+    }
+    S.validation_launches = ctx->vt.rounds;
+    S.validation_tasks = ctx->vt.totalTasks;
+    unsigned long long counts[kByteFamilies * kByteSlots] = {};
+    APO_HIP(hipMemcpy(counts, ctx->bytes, sizeof(counts), hipMemcpyDeviceToHost));
+    uint64_t fam[kByteFamilies] = {};
+    for (uint32_t f = 0; f < kByteFamilies; f++)
+        for (uint32_t q = 0; q < kByteSlots; q++) fam[f] += counts[f * kByteSlots + q];
+    S.bytes_cells = fam[kBytesCells];
+    S.bytes_vertex_validation = fam[kBytesVertexVal];
+    S.bytes_edge_validation = fam[kBytesEdgeVal];
+    S.bytes_bisector_validation = fam[kBytesBisectorVal];
     ctx->built = true;
     return APO_OK;
 }
@@ -328,6 +361,8 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     for (int s = 0; s < 8 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
+    for (uint32_t r = 0; r < kTimedRounds && e == hipSuccess; r++)
+        for (int q = 0; q < 3 && e == hipSuccess; q++) e = hipEventCreate(&ctx->vt.ev[r][q]);
     if (e != hipSuccess) {
         apo_destroy(ctx);
         return e == hipErrorOutOfMemory ? APO_ERR_OUT_OF_MEMORY : APO_ERR_HIP;
@@ -344,11 +379,14 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->codesB, ctx->idxA, ctx->idxB, ctx->sorted, ctx->nodes, ctx->parent, ctx->sizes,
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
-                    ctx->total, ctx->outXyzw, ctx->outIds, ctx->err};
+                    ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->crossList};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& round : ctx->vt.ev)
+        for (auto& e : round)
+            if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

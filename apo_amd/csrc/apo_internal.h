@@ -52,7 +52,12 @@ struct LbvhBuffers {
     uint32_t* refitCounter;// n-1
     float* rootBox;        // 7 floats: min, max, max radius
     uint32_t* err;         // device guard bits
+    uint32_t* range;       // 2(n-1): first and last leaf of each internal node
+    uint32_t* crossList;   // n-1: internal nodes whose leaves span two refit blocks
 };
+
+// Leaves per block of the first refit phase (lbvh.hip).
+constexpr uint32_t kRefitLeaves = 2048;
 
 hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, hipStream_t st);
 
@@ -70,7 +75,16 @@ struct CellCtx {
     uint4* edg;             // n * kEdgeCap records (j, k, vertexNb, status)
     uint32_t* enb;
     uint32_t* err;          // device guard bits
+    unsigned long long* bytes;  // algorithmic HBM bytes per kernel family, kByteSlots counters each
 };
+
+// Byte counters: family * kByteSlots + (block & (kByteSlots - 1)), summed on the host.
+constexpr uint32_t kByteSlots = 32;
+enum ByteFamily : uint32_t { kBytesCells = 0, kBytesVertexVal = 1, kBytesEdgeVal = 2, kBytesBisectorVal = 3,
+                             kByteFamilies = 4 };
+__device__ __forceinline__ void countBytes(const CellCtx& c, uint32_t family, uint64_t b) {
+    atomicAdd(&c.bytes[family * kByteSlots + (blockIdx.x & (kByteSlots - 1u))], static_cast<unsigned long long>(b));
+}
 
 hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st);
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t waves,
@@ -81,8 +95,17 @@ struct ValidationBuffers {
     uint32_t* tasks;       // n
     uint32_t* counters;    // [0] task count, [1..] queue heads
 };
+// Events around each vertex / edge validation launch (rounds beyond
+// kTimedRounds run untimed) and the task count of every round.
+constexpr uint32_t kTimedRounds = 16;
+struct ValidationTiming {
+    hipEvent_t ev[kTimedRounds][3];  // before vertex, between, after edge
+    uint32_t tasks[kTimedRounds];
+    uint32_t rounds;                 // rounds that launched kernels
+    uint64_t totalTasks;
+};
 hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t maxRestarts,
-                            bool bisector, hipStream_t st);
+                            bool bisector, hipStream_t st, ValidationTiming* tm);
 
 // Output (algorithm.inl:1235-1307): count, scan, write.
 struct OutputBuffers {

@@ -47,6 +47,7 @@ __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restr
     const bool active = !(st == kStatusCertified || st == kStatusBuried || K == 0);
     float kr = -FLT_MAX;
     bool certified = false;
+    uint64_t readBytes = 1;  // status
     if (active) {
         const D4 si = loadSite(c, i);
         uint32_t myJ = kNone;
@@ -75,11 +76,16 @@ __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restr
         const double maxR = cellRadius(c, L, w, si);
         certified = gtD(knownRadius, 2.0 * maxR);
         kr = float(knownRadius);
+        // own site, K neighbour ids and their status, the sites used
+        const uint64_t lim = limit < 64u ? ((uint64_t(1) << limit) - 1u) : ~uint64_t(0);
+        readBytes += 16u + 5u * uint64_t(K) + 16u * uint64_t(popc64(lim & ~buriedMask));
     }
     storeCell(c, L, w, i);
     if (lane == 0) {
         if (certified) c.status[i] = kStatusCertified;
         c.knownRadii[i] = kr;
+        // records, vertex and edge counts, known radius, status
+        countBytes(c, kBytesCells, readBytes + 16u * uint64_t(w.V + w.E) + 13u);
     }
 }
 

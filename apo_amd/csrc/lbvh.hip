@@ -64,7 +64,6 @@ __global__ __launch_bounds__(256) void k_scene_box(const F4* __restrict__ sites,
             sred[wave][4 + a] = mx[a];
         }
     __syncthreads();
-    __shared__ bool last;
     if (threadIdx.x == 0) {
         double r[8];
         for (int a = 0; a < 8; a++) r[a] = sred[0][a];
@@ -74,22 +73,33 @@ __global__ __launch_bounds__(256) void k_scene_box(const F4* __restrict__ sites,
                 r[4 + a] = __builtin_fmax(r[4 + a], sred[w][4 + a]);
             }
         for (int a = 0; a < 8; a++) partials[blockIdx.x * 8 + a] = r[a];
-        __threadfence();
-        const uint32_t t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    __threadfence();
-    double r[8];
-    for (int a = 0; a < 8; a++) r[a] = __hip_atomic_load(&partials[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t b = 1; b < gridDim.x; b++)
-        for (int a = 0; a < 4; a++) {
-            r[a] = __builtin_fmin(r[a], __hip_atomic_load(&partials[b * 8 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            r[4 + a] = __builtin_fmax(r[4 + a],
-                                      __hip_atomic_load(&partials[b * 8 + 4 + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Reduction of the block partials (one block of kBoxBlocks threads) and the
+// scene record: FP32 Morton frame and the 6 artificial sites.
+__global__ __launch_bounds__(kBoxBlocks) void k_scene_final(const double* __restrict__ partials, Scene* scene) {
+    double v[8];
+    for (int a = 0; a < 8; a++) v[a] = partials[threadIdx.x * 8 + a];
+    for (int a = 0; a < 4; a++) {
+        for (int o = 32; o >= 1; o >>= 1) {
+            v[a] = __builtin_fmin(v[a], __shfl_xor(v[a], o, 64));
+            v[4 + a] = __builtin_fmax(v[4 + a], __shfl_xor(v[4 + a], o, 64));
         }
-    *ticket = 0u;  // ready for the next build
+    }
+    __shared__ double sred[kBoxBlocks / 64][8];
+    const uint32_t wave = threadIdx.x / 64, lane = laneId();
+    if (lane == 0)
+        for (int a = 0; a < 8; a++) sred[wave][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double r[8];
+    for (int a = 0; a < 8; a++) r[a] = sred[0][a];
+    for (uint32_t w = 1; w < kBoxBlocks / 64; w++)
+        for (int a = 0; a < 4; a++) {
+            r[a] = __builtin_fmin(r[a], sred[w][a]);
+            r[4 + a] = __builtin_fmax(r[4 + a], sred[w][4 + a]);
+        }
     Scene sc;
     for (int a = 0; a < 3; a++) {
         sc.mn[a] = r[a];
@@ -323,7 +333,9 @@ __device__ __forceinline__ int deltaK(const uint64_t* __restrict__ c, int i, int
 __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ codes, const uint32_t* __restrict__ indices,
                                                 const F4* __restrict__ sitesIn, uint32_t n, F4* __restrict__ sorted,
                                                 Node* __restrict__ nodes, int32_t* __restrict__ parent,
-                                                uint32_t* __restrict__ refitCounter, uint32_t* err) {
+                                                uint32_t* __restrict__ refitCounter, uint32_t* __restrict__ sizes,
+                                                uint32_t* __restrict__ range, uint32_t* __restrict__ crossList,
+                                                uint32_t* crossCount, uint32_t* err) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
     const int ni = int(n);
     if (i >= ni) return;
@@ -334,6 +346,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ cod
         sorted[i] = F4{0.f, 0.f, 0.f, 0.f};
         atomicOr(err, kErrGatherIndex);
     }
+    sizes[ni - 1 + i] = 1u;
     if (i >= ni - 1) return;
     refitCounter[i] = 0u;
     const int dd = deltaK(codes, i, i + 1, ni) - deltaK(codes, i, i - 1, ni);
@@ -361,6 +374,12 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ cod
     parent[lc] = i;
     parent[rc] = i;
     if (i == 0) parent[0] = -1;
+    // leaf range (a node's range starts or ends at its own index)
+    const uint32_t first = uint32_t(i < j ? i : j), last = uint32_t(i > j ? i : j);
+    range[2 * i] = first;
+    range[2 * i + 1] = last;
+    sizes[i] = last - first + 1u;
+    if (first / kRefitLeaves != last / kRefitLeaves) crossList[atomicAdd(crossCount, 1u)] = uint32_t(i);
 }
 
 // ---------------------------------------------------------------------------
@@ -381,59 +400,134 @@ __device__ __forceinline__ void writeSlot(Node* nd, bool left, const float* mn, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_refit(const F4* __restrict__ sorted, uint32_t n, Node* nodes,
-                                               const int32_t* __restrict__ parent, uint32_t* sizes,
-                                               uint32_t* refitCounter, float* rootBox, uint32_t* err) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const F4 s = sorted[i];
+__device__ __forceinline__ void leafBox(const F4& s, float* mn, float* mx) {
     const double x = s.x, y = s.y, z = s.z, r = s.w;
-    float mn[3] = {float(x - r), float(y - r), float(z - r)};
-    float mx[3] = {float(x + r), float(y + r), float(z + r)};
-    float maxr = s.w;
-    int node = int(n - 1 + i);
-    sizes[node] = 1u;
+    mn[0] = float(x - r);
+    mn[1] = float(y - r);
+    mn[2] = float(z - r);
+    mx[0] = float(x + r);
+    mx[1] = float(y + r);
+    mx[2] = float(z + r);
+}
+
+// Union of both child slots of a node record.
+__device__ __forceinline__ void nodeUnion(const Node* P, float* mn, float* mx, float& maxr) {
+    const float* f = reinterpret_cast<const float*>(P);
+    for (int c = 0; c < 3; c++) {
+        mn[c] = __builtin_fminf(f[c], f[6 + c]);
+        mx[c] = __builtin_fmaxf(f[3 + c], f[9 + c]);
+    }
+    maxr = __builtin_fmaxf(f[14], f[15]);
+}
+
+__device__ __forceinline__ bool crossesBlocks(const uint32_t* range, uint32_t node) {
+    return range[2 * node] / kRefitLeaves != range[2 * node + 1] / kRefitLeaves;
+}
+
+// Phase 1: one block per kRefitLeaves consecutive leaves climbs through the
+// nodes whose leaves all lie in its range.  The first child to arrive at a
+// node writes its slot and stops; the second unions both slots and climbs
+// on.  Arrivals are counted in LDS; slots travel through the node records
+// with workgroup-scope ordering only (no device-wide fence).  A node whose
+// leaves span two blocks only receives its child's slot here.
+__global__ __launch_bounds__(512) void k_refit_local(const F4* __restrict__ sorted, uint32_t n, Node* nodes,
+                                                    const int32_t* __restrict__ parent,
+                                                    const uint32_t* __restrict__ range, float* rootBox, uint32_t* err) {
+    __shared__ uint32_t arrivals[kRefitLeaves];
+    const uint32_t lo = blockIdx.x * kRefitLeaves;
+    const uint32_t hi = (n - lo) < kRefitLeaves ? n : lo + kRefitLeaves;
+    for (uint32_t t = threadIdx.x; t < kRefitLeaves; t += blockDim.x) arrivals[t] = 0u;
+    __syncthreads();
     const int internal = int(n) - 1;
-    for (uint32_t climb = 0; climb < n; climb++) {
-        const int p = parent[node];
-        if (p < 0 || p >= internal) {
-            if (!(p == -1 && node == 0)) atomicOr(err, kErrRefitParent);
-            return;
-        }
-        Node* P = nodes + p;
-        const bool isLeft = P->d.x == node;
-        writeSlot(P, isLeft, mn, mx, maxr);
-        __threadfence();
-        const uint32_t old = atomicAdd(&refitCounter[p], 1u);
-        if (old == 0u) return;
-        __threadfence();
-        const float* f = reinterpret_cast<const float*>(P);
-        float v[16];
-        for (int c = 0; c < 16; c++) v[c] = __hip_atomic_load(f + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        for (int c = 0; c < 3; c++) {
-            mn[c] = __builtin_fminf(v[c], v[6 + c]);
-            mx[c] = __builtin_fmaxf(v[3 + c], v[9 + c]);
-        }
-        maxr = __builtin_fmaxf(v[14], v[15]);
-        const int lc = __builtin_bit_cast(int, v[12]), rc = __builtin_bit_cast(int, v[13]);
-        if (lc < 0 || rc < 0 || lc >= 2 * int(n) - 1 || rc >= 2 * int(n) - 1) {
-            atomicOr(err, kErrRefitChild);
-            return;
-        }
-        const uint32_t sl = __hip_atomic_load(&sizes[lc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t sr = __hip_atomic_load(&sizes[rc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        sizes[p] = sl + sr;
-        node = p;
-        if (p == 0) {
-            for (int c = 0; c < 3; c++) {
-                rootBox[c] = mn[c];
-                rootBox[3 + c] = mx[c];
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        float mn[3], mx[3];
+        leafBox(sorted[i], mn, mx);
+        float maxr = sorted[i].w;
+        int node = int(n - 1 + i);
+        for (uint32_t climb = 0; climb < n; climb++) {
+            const int p = parent[node];
+            if (p < 0) {
+                if (node != 0) {
+                    atomicOr(err, kErrRefitParent);
+                } else {
+                    for (int c = 0; c < 3; c++) {
+                        rootBox[c] = mn[c];
+                        rootBox[3 + c] = mx[c];
+                    }
+                    rootBox[6] = maxr;
+                }
+                break;
             }
-            rootBox[6] = maxr;
-            return;
+            if (p >= internal) {
+                atomicOr(err, kErrRefitParent);
+                break;
+            }
+            Node* P = nodes + p;
+            writeSlot(P, P->d.x == node, mn, mx, maxr);
+            const uint32_t first = range[2 * p], last = range[2 * p + 1];
+            if (first < lo || last >= hi) break;  // finished by the second phase
+            if (uint32_t(p) - lo >= kRefitLeaves) {  // a node's index is its first or last leaf
+                atomicOr(err, kErrRefitParent);
+                break;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            const uint32_t old = atomicAdd(&arrivals[uint32_t(p) - lo], 1u);
+            if (old == 0u) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            nodeUnion(P, mn, mx, maxr);
+            node = p;
         }
     }
-    atomicOr(err, kErrRefitParent);
+}
+
+// Phase 2: one block finishes the nodes whose leaves span two phase-1
+// blocks.  Climbs start at those with no such child; a node continues once
+// all its spanning children have arrived.  One block, so workgroup-scope
+// ordering suffices.
+__global__ __launch_bounds__(1024) void k_refit_top(uint32_t n, Node* nodes, const int32_t* __restrict__ parent,
+                                                   const uint32_t* __restrict__ range,
+                                                   const uint32_t* __restrict__ crossList,
+                                                   const uint32_t* __restrict__ crossCount, uint32_t* refitCounter,
+                                                   float* rootBox, uint32_t* err) {
+    const uint32_t internal = n - 1;
+    const uint32_t count = *crossCount < internal ? *crossCount : internal;
+    auto spanningChildren = [&](uint32_t q) {
+        const uint32_t l = uint32_t(nodes[q].d.x), r = uint32_t(nodes[q].d.y);
+        return uint32_t(l < internal && crossesBlocks(range, l)) + uint32_t(r < internal && crossesBlocks(range, r));
+    };
+    for (uint32_t u = threadIdx.x; u < count; u += blockDim.x) {
+        uint32_t node = crossList[u];
+        if (node >= internal) {
+            atomicOr(err, kErrRefitParent);
+            continue;
+        }
+        if (spanningChildren(node) != 0u) continue;
+        for (uint32_t climb = 0; climb < n; climb++) {
+            float mn[3], mx[3], maxr;
+            nodeUnion(nodes + node, mn, mx, maxr);
+            const int p = parent[node];
+            if (p < 0) {
+                if (node != 0u) atomicOr(err, kErrRefitParent);
+                for (int c = 0; c < 3; c++) {
+                    rootBox[c] = mn[c];
+                    rootBox[3 + c] = mx[c];
+                }
+                rootBox[6] = maxr;
+                break;
+            }
+            if (uint32_t(p) >= internal) {
+                atomicOr(err, kErrRefitParent);
+                break;
+            }
+            Node* P = nodes + p;
+            writeSlot(P, P->d.x == int(node), mn, mx, maxr);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            const uint32_t old = atomicAdd(&refitCounter[p], 1u);
+            if (old + 1u < spanningChildren(uint32_t(p))) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            node = uint32_t(p);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -441,9 +535,11 @@ __global__ __launch_bounds__(256) void k_refit(const F4* __restrict__ sorted, ui
 // ---------------------------------------------------------------------------
 hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, hipStream_t st) {
     const uint32_t tiles = (n + kTileKeys - 1) / kTileKeys;
-    const uint32_t zeroWords = kRadixPasses * 256u + kRadixPasses + kRadixPasses * tiles * 256u;
+    // sort counters and look-back, then the spanning-node counter
+    const uint32_t zeroWords = kRadixPasses * 256u + kRadixPasses + kRadixPasses * tiles * 256u + 1u;
     hipLaunchKernelGGL(k_scene_box, dim3(kBoxBlocks), dim3(256), 0, st, sitesIn, n, b.scene, b.boxPartials, b.boxTicket,
                        b.sortScratch, zeroWords);
+    hipLaunchKernelGGL(k_scene_final, dim3(1), dim3(kBoxBlocks), 0, st, b.boxPartials, b.scene);
     uint32_t* hist = b.sortScratch;
     uint32_t* tileCounters = hist + kRadixPasses * 256u;
     uint32_t* lookback = tileCounters + kRadixPasses;
@@ -461,10 +557,14 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, hipStr
     }
     // after an even number of passes the sorted data is back in A
     const uint32_t blocks = (n + 255u) / 256u;
+    uint32_t* crossCount = lookback + size_t(kRadixPasses) * tiles * 256u;
     hipLaunchKernelGGL(k_karras, dim3(blocks), dim3(256), 0, st, b.codesA, b.idxA, sitesIn, n, b.sorted, b.nodes,
-                       b.parent, b.refitCounter, b.err);
-    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, st, b.sorted, n, b.nodes, b.parent, b.sizes,
-                       b.refitCounter, b.rootBox, b.err);
+                       b.parent, b.refitCounter, b.sizes, b.range, b.crossList, crossCount, b.err);
+    const uint32_t refitBlocks = (n + kRefitLeaves - 1u) / kRefitLeaves;
+    hipLaunchKernelGGL(k_refit_local, dim3(refitBlocks), dim3(512), 0, st, b.sorted, n, b.nodes, b.parent, b.range,
+                       b.rootBox, b.err);
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, n, b.nodes, b.parent, b.range, b.crossList,
+                       crossCount, b.refitCounter, b.rootBox, b.err);
     return hipGetLastError();
 }
 

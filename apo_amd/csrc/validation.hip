@@ -139,6 +139,7 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
     if (st == kStatusCertified || st == kStatusBuried) return;
     WaveCell w;
     loadCell(c, L, w, i, true);
+    const uint32_t loaded = w.V + w.E;
     const float kr = c.knownRadii[i];
     const D4 si = loadSite(c, i);
     uint32_t vs = 0, restarts = 0;
@@ -179,6 +180,8 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
     for (int o = 32; o >= 1; o >>= 1) allFlags |= __shfl_xor(allFlags, o, 64);
     w.flags |= allFlags;
     storeCell(c, L, w, i);
+    // status, known radius, counts and site read; records read and written
+    if (lane == 0) countBytes(c, kBytesVertexVal, 29u + 16u * uint64_t(loaded + w.V + w.E) + 8u);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,7 +291,7 @@ __device__ __forceinline__ uint32_t waveOr(uint32_t v) {
 // Store the edges and append the vertices created during this kernel after
 // the cell's stored vertices (algorithm.inl:840-855, 1216-1231).
 __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L, WaveCell& w, uint32_t i,
-                                              bool didChange) {
+                                              bool didChange, uint32_t family, uint32_t edgesLoaded) {
     const uint32_t lane = laneId();
     const uint32_t baseV = c.vnb[i];
     uint32_t total = baseV + w.V;
@@ -306,6 +309,9 @@ __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L
         c.enb[i] = w.E;
         c.vnb[i] = total;
         if (w.flags) c.flags[i] |= w.flags;
+        // status, known radius, counts and site read, edges read; edges,
+        // appended vertices, status and counts written
+        countBytes(c, family, 29u + 16u * uint64_t(edgesLoaded + w.E + (total - baseV)) + 9u);
     }
 }
 
@@ -319,6 +325,7 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
     if (st == kStatusCertified || st == kStatusBuried) return;
     WaveCell w;
     loadCell(c, L, w, i, false);
+    const uint32_t edgesLoaded = w.E;
     for (uint32_t e = lane; e < w.E; e += 64)
         if (L.edg[e].w != 1u) L.edg[e].w = 2u;
     waveSync();
@@ -417,7 +424,7 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
         }
     }
     w.flags |= waveOr(flags);
-    storeAppended(c, L, w, i, didChange);
+    storeAppended(c, L, w, i, didChange, kBytesEdgeVal, edgesLoaded);
 }
 
 // ---------------------------------------------------------------------------
@@ -497,6 +504,7 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t 
     if (st == kStatusBuried) return;
     WaveCell w;
     loadCell(c, L, w, i, false);
+    const uint32_t edgesLoaded = w.E;
     uint32_t flags = 0;
     // sorted distinct neighbours above i (algorithm.inl:903-968)
     buildNeighbours(L, w);
@@ -579,7 +587,7 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t 
         }
     }
     w.flags |= waveOr(flags);
-    storeAppended(c, L, w, i, didChange);
+    storeAppended(c, L, w, i, didChange, kBytesBisectorVal, edgesLoaded);
 }
 
 __global__ void k_compact_tasks(const uint8_t* __restrict__ status, uint32_t n, uint32_t* tasks, uint32_t* count) {
@@ -590,7 +598,7 @@ __global__ void k_compact_tasks(const uint8_t* __restrict__ status, uint32_t n, 
 __global__ void k_zero1(uint32_t* p) { *p = 0u; }
 
 hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t maxRestarts,
-                            bool bisector, hipStream_t st) {
+                            bool bisector, hipStream_t st, ValidationTiming* tm) {
     if (bisector) {
         hipLaunchKernelGGL(k_bisector_validation, dim3(c.n), dim3(64), 0, st, c, maxRestarts);
         return hipGetLastError();
@@ -604,9 +612,18 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
         if (count == 0) break;
+        const bool timed = tm && it < kTimedRounds;
+        if (timed) {
+            tm->tasks[it] = count;
+            tm->rounds = it + 1;
+            e = hipEventRecord(tm->ev[it][0], st);
+        }
+        if (tm) tm->totalTasks += count;
         hipLaunchKernelGGL(k_vertex_validation, dim3(count), dim3(64), 0, st, c, v.tasks, maxRestarts);
+        if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][1], st);
         hipLaunchKernelGGL(k_edge_validation, dim3(count), dim3(64), 0, st, c, v.tasks, maxRestarts);
-        e = hipGetLastError();
+        if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][2], st);
+        if (e == hipSuccess) e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

@@ -39,20 +39,21 @@ def parse():
 
 
 def cpu_baseline(sites: np.ndarray, target: int):
-    """Time the oracle (host C++/OpenMP restatement of the same construction)
-    on a sub-cube of the workload holding about `target` sites: same density
-    and radius distribution, bounded CPU time."""
+    """Time the oracle (host C++/OpenMP restatement of the reference's method,
+    reference edge searches) on a sub-cube of the workload holding about
+    `target` sites: same density and radius distribution, bounded CPU time."""
     import oracle
     edge = (target / len(sites)) ** (1.0 / 3.0)
     sub = sites[np.all(sites[:, :3] < edge, axis=1)]
-    threads = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     t0 = time.perf_counter()
     d = oracle.Diagram(sub, threads=threads)
     dt = time.perf_counter() - t0
     v = d.stats()["vertices"]
     return {"value": len(sub) / dt, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"{len(sub)} sites of the workload inside [0,{edge:.4f}]^3 (same density and radii), "
-                      f"full construction in {dt:.2f} s, {v} vertices"}
+                      f"full construction by the oracle with the reference's edge searches, {dt:.2f} s, "
+                      f"{v} vertices"}
 
 
 def main():
@@ -82,9 +83,14 @@ def main():
     def step():
         _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
 
+    def say(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    say(f"{n} sites resident on {torch.cuda.get_device_name(0)}; {args.warmup} warm-up steps")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    say(f"timing {args.steps} steps")
     stage = {k: 0.0 for k in ("ms_total", "ms_lbvh", "ms_knn", "ms_cells", "ms_validation", "ms_bisector",
                               "ms_output")}
     t0 = time.perf_counter()
@@ -93,6 +99,15 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1000.0 * wall / args.steps
+    say(f"{ms_per_step:.1f} ms per step")
+
+    # Calibration copy for the HBM counters: one elementwise device copy of
+    # exactly 256 MiB read + 256 MiB written (profiled beside the kernels).
+    src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
     s = _lib.Stats()
     _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
@@ -100,20 +115,24 @@ def main():
     for k in stage:
         stage[k] = float(getattr(s, k))
 
-    # stage times are HIP events recorded on the library's stream (last step);
-    # the dominant kernel is the stage with the largest time
-    parts = {"lbvh": s.ms_lbvh, "knn": s.ms_knn, "cells": s.ms_cells, "validation": s.ms_validation,
-             "bisector": s.ms_bisector, "output": s.ms_output}
-    dominant = max(parts, key=parts.get)
-    # algorithmic HBM bytes per cell of each stage
-    cell_bytes = 16 + args.k * (4 + 16) + 16 * ((s.vertex_records + s.edge_records) / max(n, 1)) + 5
-    bytes_per_cell = {"lbvh": 440.0, "knn": 16 + 4 * args.k, "cells": cell_bytes,
-                      "validation": cell_bytes, "bisector": cell_bytes, "output": 48.0 * vertices / n}
-    achieved = bytes_per_cell[dominant] * n / (parts[dominant] * 1e-3) / 1e9
+    # Stage and kernel times are HIP events recorded on the library's stream
+    # around each launch (last timed step).  The dominant kernel is the edge
+    # validation; its algorithmic bytes are counted by the kernel itself
+    # (records, sites and per-cell scalars it must read or write once).
+    kernels = {
+        "k_edge_validation": (s.ms_edge_validation_kernel, s.bytes_edge_validation, s.validation_launches),
+        "k_vertex_validation": (s.ms_vertex_validation_kernel, s.bytes_vertex_validation, s.validation_launches),
+        "k_cells": (s.ms_cells, s.bytes_cells, 1),
+        "k_bisector_validation": (s.ms_bisector, s.bytes_bisector_validation, 1),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k][0])
+    ms_k, bytes_k, launches = kernels[dominant]
+    achieved = bytes_k / (ms_k * 1e-3) / 1e9
     lbvh_gbs = 440.0 * n / (s.ms_lbvh * 1e-3) / 1e9
 
     violations = None
     if not args.no_check:
+        say("FP64 empty-sphere check of every vertex through the LBVH")
         cnt = _lib.c_uint64()
         _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
         violations = int(cnt.value)
@@ -141,9 +160,13 @@ def main():
         "empty_sphere_violations": violations,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launches": int(launches), "avg_launch_ms": ms_k / max(int(launches), 1),
+                     "bytes_per_launch": bytes_k / max(int(launches), 1),
                      "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
+        "kernel_ms": {k: v[0] for k, v in kernels.items()},
     }
     if not args.no_cpu_baseline:
+        say(f"CPU baseline: oracle on a sub-cube of about {args.cpu_sample} sites")
         out["cpu_baseline"] = cpu_baseline(sites, args.cpu_sample)
     L.apo_destroy(ctx)
     print(json.dumps(out), flush=True)

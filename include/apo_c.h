@@ -59,6 +59,17 @@ typedef struct apo_stats {
     uint64_t flagged_cells;     /* cells with an overflow / cap flag */
     uint64_t vertex_records;    /* sum over cells of stored vertex records */
     uint64_t edge_records;      /* sum over cells of stored edge records */
+    /* per-kernel device time (HIP events around each launch, summed) */
+    double ms_vertex_validation_kernel;
+    double ms_edge_validation_kernel;
+    uint64_t validation_launches; /* rounds that launched the vertex + edge validation kernels */
+    uint64_t validation_tasks;    /* cells processed, summed over those rounds */
+    /* algorithmic HBM bytes of each kernel family (records, sites and per-cell
+     * scalars each kernel must read or write once), summed over its launches */
+    uint64_t bytes_cells;
+    uint64_t bytes_vertex_validation;
+    uint64_t bytes_edge_validation;
+    uint64_t bytes_bisector_validation;
     uint64_t reserved[2];
 } apo_stats;
 

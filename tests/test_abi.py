@@ -36,3 +36,15 @@ def test_default_config_matches_reference_defaults():
     _lib.lib().apo_default_config(ctypes.byref(cfg))
     # AlgorithmGPU<16>(sites, knnValidationStep = 1), 10 validation rounds, <100 restarts
     assert (cfg.k, cfg.max_iterations, cfg.max_restarts, cfg.stage_mask) == (16, 10, 100, 7)
+
+
+def test_stats_struct_mirrors_header():
+    text = open(os.path.join(ROOT, "include", "apo_c.h")).read()
+    body = re.search(r"typedef struct apo_stats \{(.*?)\} apo_stats;", text, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"(double|uint64_t)\s+([a-z_0-9]+)(\[\d+\])?;", body)
+    want = [(name, {"double": "c_double", "uint64_t": "c_ulong"}[ty], bool(arr)) for ty, name, arr in fields]
+    got = [(name, (t._type_ if hasattr(t, "_length_") else t).__name__, hasattr(t, "_length_"))
+           for name, t in _lib.Stats._fields_]
+    assert [w[0] for w in want] == [g[0] for g in got]
+    assert ctypes.sizeof(_lib.Stats) == 8 * (len(fields) + 1)

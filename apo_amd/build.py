@@ -51,9 +51,25 @@ def build(verbose: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError("link failed:\n" + res.stderr)
     os.replace(tmp, LIB)
+    _build_example()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+EXAMPLE = os.path.join(LIBDIR, "apo_diagram")
+
+
+def _build_example() -> None:
+    """The C++ caller of include/apo/gpu/algorithm.hpp, linked to the library."""
+    include = os.path.join(os.path.dirname(HERE), "include")
+    src = os.path.join(HERE, "examples", "diagram.cpp")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", include, src, "-o", EXAMPLE + ".tmp",
+           "-L", LIBDIR, "-lapo_hip", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("example build failed:\n" + res.stderr)
+    os.replace(EXAMPLE + ".tmp", EXAMPLE)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,52 @@
+// diagram.cpp -- C++ caller of the library through include/apo/gpu/algorithm.hpp.
+//
+//   diagram <sites.bin> <out.bin>
+//
+// sites.bin: n x (x, y, z, r) float32.  out.bin: uint64 vertex count V, then
+// V x (x, y, z, w) float64, then V x (i, j, k, l) uint32 original site ids.
+// The call sequence is the reference's computeApolloniusDiagram
+// (src/gpu/algorithm.cu:6-15).
+#include <cstdio>
+#include <vector>
+
+#include "apo/gpu/algorithm.hpp"
+
+int main(int argc, char** argv) {
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: %s <sites.bin> <out.bin>\n", argv[0]);
+        return 2;
+    }
+    std::FILE* in = std::fopen(argv[1], "rb");
+    if (!in) {
+        std::perror(argv[1]);
+        return 1;
+    }
+    std::vector<float> sites;
+    float buf[4096];
+    std::size_t got;
+    while ((got = std::fread(buf, sizeof(float), 4096, in)) > 0) sites.insert(sites.end(), buf, buf + got);
+    std::fclose(in);
+    try {
+        apo::gpu::AlgorithmGPU algo({sites.data(), sites.size()});
+        algo.build();
+        const apo::gpu::VertexDiagram d = algo.toHost(true);
+        const uint64_t v = d.vertices.size() / 4;
+        std::FILE* out = std::fopen(argv[2], "wb");
+        if (!out) {
+            std::perror(argv[2]);
+            return 1;
+        }
+        std::fwrite(&v, sizeof(v), 1, out);
+        std::fwrite(d.vertices.data(), sizeof(double), d.vertices.size(), out);
+        std::fwrite(d.verticesId.data(), sizeof(uint32_t), d.verticesId.size(), out);
+        std::fclose(out);
+        std::printf("%zu sites, %llu vertices, %.3f ms (lbvh %.3f, knn %.3f, cells %.3f, validation %.3f)\n",
+                    sites.size() / 4, static_cast<unsigned long long>(v), algo.sample.total,
+                    algo.sample.bvhConstruction, algo.sample.knnSearch, algo.sample.knnConstruction,
+                    algo.sample.vertexEdgeValidation + algo.sample.bisectorValidation);
+    } catch (const apo::gpu::Error& e) {
+        std::fprintf(stderr, "apo error %d: %s\n", int(e.status), e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -91,13 +91,29 @@ def main():
         step()
     torch.cuda.synchronize()
     say(f"timing {args.steps} steps")
-    stage = {k: 0.0 for k in ("ms_total", "ms_lbvh", "ms_knn", "ms_cells", "ms_validation", "ms_bisector",
-                              "ms_output")}
-    t0 = time.perf_counter()
+    # Each step is bracketed by device synchronisations; per-kernel times
+    # and byte counts are read between steps, outside the timed spans, and
+    # summed over all timed steps.
+    kern = {"k_edge_validation": [0.0, 0, 0], "k_vertex_validation": [0.0, 0, 0], "k_cells": [0.0, 0, 0],
+            "k_bisector_validation": [0.0, 0, 0]}
+    s = _lib.Stats()
+    wall = 0.0
     for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         step()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+        for name, ms, b, launches in (
+                ("k_edge_validation", s.ms_edge_validation_kernel, s.bytes_edge_validation, s.validation_launches),
+                ("k_vertex_validation", s.ms_vertex_validation_kernel, s.bytes_vertex_validation,
+                 s.validation_launches),
+                ("k_cells", s.ms_cells, s.bytes_cells, 1),
+                ("k_bisector_validation", s.ms_bisector, s.bytes_bisector_validation, 1)):
+            kern[name][0] += ms
+            kern[name][1] += b
+            kern[name][2] += launches
     ms_per_step = 1000.0 * wall / args.steps
     say(f"{ms_per_step:.1f} ms per step")
 
@@ -109,26 +125,28 @@ def main():
     torch.cuda.synchronize()
     del src, dst
 
-    s = _lib.Stats()
-    _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
     vertices = int(s.vertices)
+    stage = {k: 0.0 for k in ("ms_total", "ms_lbvh", "ms_knn", "ms_cells", "ms_validation", "ms_bisector",
+                              "ms_output")}
     for k in stage:
         stage[k] = float(getattr(s, k))
-
-    # Stage and kernel times are HIP events recorded on the library's stream
-    # around each launch (last timed step).  The dominant kernel is the edge
-    # validation; its algorithmic bytes are counted by the kernel itself
-    # (records, sites and per-cell scalars it must read or write once).
-    kernels = {
-        "k_edge_validation": (s.ms_edge_validation_kernel, s.bytes_edge_validation, s.validation_launches),
-        "k_vertex_validation": (s.ms_vertex_validation_kernel, s.bytes_vertex_validation, s.validation_launches),
-        "k_cells": (s.ms_cells, s.bytes_cells, 1),
-        "k_bisector_validation": (s.ms_bisector, s.bytes_bisector_validation, 1),
-    }
-    dominant = max(kernels, key=lambda k: kernels[k][0])
-    ms_k, bytes_k, launches = kernels[dominant]
+    # Roofline of the dominant kernel: its algorithmic bytes (counted by the
+    # kernel itself: records, sites and per-cell scalars it must read or
+    # write once) over its launch time (HIP events around each launch on the
+    # library's stream), both summed over the timed steps.
+    dominant = max(kern, key=lambda k: kern[k][0])
+    ms_k, bytes_k, launches = kern[dominant]
     achieved = bytes_k / (ms_k * 1e-3) / 1e9
     lbvh_gbs = 440.0 * n / (s.ms_lbvh * 1e-3) / 1e9
+    # HBM bytes per launch of that kernel from the two rocprofv3 counter
+    # passes (FETCH_SIZE, WRITE_SIZE) committed under profiles/, when they
+    # were taken on this workload
+    traffic, traffic_src = None, None
+    tp = os.path.join(ROOT, "profiles", f"pmc_traffic_{dominant}_1M_loguniform.json")
+    if n == 1_000_000 and args.k == 16 and os.path.exists(tp):
+        with open(tp) as f:
+            traffic = float(json.load(f)["traffic_bytes_per_launch"])
+        traffic_src = os.path.relpath(tp, ROOT)
 
     violations = None
     if not args.no_check:
@@ -159,11 +177,11 @@ def main():
         "flagged_cells": int(s.flagged_cells),
         "empty_sphere_violations": violations,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "launches": int(launches), "avg_launch_ms": ms_k / max(int(launches), 1),
                      "bytes_per_launch": bytes_k / max(int(launches), 1),
                      "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
-        "kernel_ms": {k: v[0] for k, v in kernels.items()},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
     }
     if not args.no_cpu_baseline:
         say(f"CPU baseline: oracle on a sub-cube of about {args.cpu_sample} sites")

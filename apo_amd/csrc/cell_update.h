@@ -43,7 +43,7 @@ __device__ __forceinline__ D4 vertexPosition(const CellCtx& c, const D4& si, uin
 }
 
 // Distinct sites over the edges [0, E), in first-occurrence order.
-__device__ inline void buildNeighbours(CellLds& L, WaveCell& w) {
+__device__ inline void buildNeighbours(const CellCtx& c, CellLds& L, WaveCell& w) {
     const uint32_t lane = laneId();
     uint32_t F = 0;
     for (uint32_t b = 0; b < w.E; b += 64) {
@@ -74,6 +74,13 @@ __device__ inline void buildNeighbours(CellLds& L, WaveCell& w) {
     waveSync();
 }
 
+// Site of neighbour slot q (id = L.nb[q]).
+__device__ __forceinline__ D4 nbSite(const CellCtx& c, const CellLds& L, uint32_t q, uint32_t id) {
+    (void)L;
+    (void)q;
+    return loadSite(c, id);
+}
+
 // Candidate sphere is clear of every neighbour except skip1/skip2
 // (topology_update.inl:578-592).  Per lane, serial over the neighbour set.
 __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLds& L, uint32_t F, const D4& cand,
@@ -81,7 +88,7 @@ __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLd
     for (uint32_t q = 0; q < F; q++) {
         const uint32_t id = __builtin_amdgcn_readfirstlane(L.nb[q]);
         if (id == skip1 || id == skip2) continue;
-        if (intersectD(loadSite(c, id), cand)) return false;
+        if (intersectD(nbSite(c, L, q, id), cand)) return false;
     }
     return true;
 }
@@ -130,7 +137,7 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
         }
     }
     const uint32_t baseV = w.V, baseE = w.E;
-    buildNeighbours(L, w);
+    buildNeighbours(c, L, w);
 
     // (1) vertices whose empty sphere intersects s_j
     uint32_t nInv = 0;

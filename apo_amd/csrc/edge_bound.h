@@ -33,7 +33,7 @@ namespace apo_hip {
 
 constexpr int kCoverBalls = 8;       // balls per cover
 constexpr float kCoverRatio = 2.0f;  // cover an edge whose search sphere exceeds this x known radius
-constexpr uint32_t kCoverRoots = 1024;
+constexpr uint32_t kCoverRoots = 512;
 constexpr int kCoverRuns = kCoverBalls + 1;
 
 // Buffers of the cover construction; they overlay the traversal stacks,
@@ -118,7 +118,7 @@ __device__ __forceinline__ bool clearlyHit(const CellCtx& c, const CellLds& L, u
     for (uint32_t q = 0; q < F; q++) {
         const uint32_t s = L.nb[q];
         if (s == k || s == l) continue;
-        if (sphereDistD(loadSite(c, s), x) < lim) return true;
+        if (sphereDistD(nbSite(c, L, q, s), x) < lim) return true;
     }
     return false;
 }
@@ -186,7 +186,7 @@ __device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L,
         if (q < F) {
             const uint32_t s = L.nb[q];
             if (s != k && s != l) {
-                const D4 ss = loadSite(c, s);
+                const D4 ss = nbSite(c, L, q, s);
                 const D3 qs = sub(xyz(ss), p1);
                 const double sigma = dot(t.n, qs);
                 const double L0 = dot(qs, qs) - 2.0 * dot(t.P, qs) + r1 * r1 - ss.w * ss.w;

@@ -9,6 +9,12 @@
 // in LDS ([slot][lane]); the contributors it finds are applied to the cell
 // in ascending lane order with the wave-wide updateCell of cells.hip.  Steps
 // are 64 records wide.
+//
+// Edge and bisector searches pause at each candidate leaf: every lane walks
+// its own traversal (same order as the reference) to its next candidate,
+// then the wave tests all pending candidates together (vertex solve and the
+// empty-sphere test against the cell's neighbours, whose sites sit in LDS).  The expensive test thus runs once per round instead of once per
+// lane that happens to reach a candidate.
 #include "apo_device.h"
 #include "apo_internal.h"
 #include "cell_update.h"
@@ -16,17 +22,31 @@
 
 namespace apo_hip {
 
-struct ValLds {
+// LDS of each kernel (one wave per block)
+struct VertexLds {
+    CellLds cell;
+    uint32_t stNode[kStackDepth][64];
+    float stDist[kStackDepth][64];
+};
+struct EdgeLds {
     CellLds cell;
     union {
-        struct {
-            uint32_t stNode[kStackDepth][64];
-            float stDist[kStackDepth][64];
-        };
+        uint32_t stNode[kStackDepth][64];
         CoverLds cover;  // edge covers are built while no traversal runs
     };
-    uint32_t bis[kBisectorCap];
     CoverOut coverOut;
+};
+struct BisectorLds {
+    CellLds cell;
+    uint32_t stNode[kStackDepth][64];
+    uint32_t bis[kBisectorCap];
+};
+static_assert(sizeof(CoverLds) <= sizeof(uint32_t) * kStackDepth * 64, "cover buffers overlay the stack");
+
+// Traversal state of one lane: next node and stack height.
+struct Trav {
+    uint32_t node, sp;
+    bool live;
 };
 
 __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
@@ -68,70 +88,73 @@ __device__ __forceinline__ bool knownLane(const CellLds& L, uint32_t F, uint32_t
 // ---------------------------------------------------------------------------
 // Nearest contributing site to a vertex sphere (algorithm.inl:305-414)
 // ---------------------------------------------------------------------------
-__device__ uint32_t vertexSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, uint4 vx, F4 cur,
-                                 uint32_t& flags) {
+// One step of a lane's nearest-site search (algorithm.inl:305-414): visit
+// one internal node (descend to the closer child under the best gap so far,
+// stack the other with its gap) or test one leaf; stacked nodes whose gap
+// exceeds the best are skipped when popped.
+__device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, uint32_t (*stNode)[64],
+                                           float (*stDist)[64], uint32_t F, uint32_t i, uint4 vx, F4 cur, Trav& t,
+                                           float& nd, uint32_t& nb, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
-    ValLds& M = const_cast<ValLds&>(S);
-    uint32_t sp = 0, node = 0, nb = kNone;
-    float nd = FLT_MAX;
-    while (true) {
-        while (node < n - 1) {
-            const Node nn = c.nodes[node];
-            Box lb, rb;
-            nodeBoxes(nn, lb, rb);
-            const uint32_t L = uint32_t(nn.d.x), R = uint32_t(nn.d.y);
-            if (L >= 2 * n - 1 || R >= 2 * n - 1) {
-                atomicOr(c.err, kErrKnnChild);
-                node = 2 * n;
-                break;
-            }
+    const uint32_t node = t.node;
+    if (node < n - 1) {
+        const Node nn = c.nodes[node];
+        Box lb, rb;
+        nodeBoxes(nn, lb, rb);
+        const uint32_t L0 = uint32_t(nn.d.x), R0 = uint32_t(nn.d.y);
+        bool ef = false, ec = false;
+        uint32_t farChild = 0, closeChild = 0;
+        float minFar = 0.f;
+        if (L0 >= 2 * n - 1 || R0 >= 2 * n - 1) {
+            atomicOr(c.err, kErrKnnChild);
+        } else {
             const float dL = boxDistance(lb, cur), dR = boxDistance(rb, cur);
             const bool lc = dL < dR;
-            const float minFar = lc ? dR : dL;
-            const uint32_t farChild = lc ? R : L;
+            minFar = lc ? dR : dL;
+            farChild = lc ? R0 : L0;
             const float minClose = lc ? dL : dR;
-            const uint32_t closeChild = lc ? L : R;
-            const bool ef = minFar < nd, ec = minClose < nd;
-            if (!ef && !ec) break;
-            node = ec ? closeChild : farChild;
+            closeChild = lc ? L0 : R0;
+            ef = minFar < nd;
+            ec = minClose < nd;
+        }
+        if (ec || ef) {
+            t.node = ec ? closeChild : farChild;
             if (ec && ef) {
-                if (sp < kStackDepth) {
-                    M.stNode[sp][lane] = farChild;
-                    M.stDist[sp][lane] = minFar;
-                    sp++;
+                if (t.sp < kStackDepth) {
+                    stNode[t.sp][lane] = farChild;
+                    stDist[t.sp][lane] = minFar;
+                    t.sp++;
                 } else {
                     flags |= kFlagStackOverflow;
                 }
             }
+            return;
         }
-        if (node >= n - 1 && node < 2 * n - 1) {
-            const uint32_t j = node - (n - 1);
-            if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried &&
-                !knownLane(S.cell, F, j)) {
-                const float d = sphereDistF(c.sorted[j], cur);
-                if (d < nd) {
-                    nd = d;
-                    nb = j;
-                }
+    } else if (node < 2 * n - 1) {
+        const uint32_t j = node - (n - 1);
+        if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried) {
+            const float d = sphereDistF(c.sorted[j], cur);
+            if (d < nd && !knownLane(L, F, j)) {
+                nd = d;
+                nb = j;
             }
         }
-        bool found = false;
-        while (sp > 0) {
-            sp--;
-            if (M.stDist[sp][lane] > nd) continue;
-            node = M.stNode[sp][lane];
-            found = true;
-            break;
-        }
-        if (!found) break;
     }
-    return nb;
+    bool more = false;
+    while (t.sp > 0) {
+        t.sp--;
+        if (stDist[t.sp][lane] > nd) continue;
+        t.node = stNode[t.sp][lane];
+        more = true;
+        break;
+    }
+    if (!more) t.live = false;
 }
 
 __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint32_t* __restrict__ tasks,
                                                           uint32_t maxRestarts) {
-    __shared__ ValLds S;
+    __shared__ VertexLds S;
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
     const uint32_t i = tasks[blockIdx.x];
@@ -147,7 +170,7 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
     while (vs < w.V / 64 + 1 && restarts < maxRestarts) {
         const uint32_t v = vs * 64 + lane;
         vs++;
-        buildNeighbours(L, w);
+        buildNeighbours(c, L, w);
         bool need = v < w.V && ((L.vtx[v].w >> 8) & 0xffu) == 0u;
         D4 cur{0, 0, 0, 0};
         uint32_t nb = kNone;
@@ -155,8 +178,17 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
             const uint4 r = L.vtx[v];
             cur = vertexPosition(c, si, r);
             if (2.0 * cur.w < double(kr)) need = false;
-            if (need) nb = vertexSearch(c, S, w.F, i, r, toF(cur), flags);
             L.vtx[v].w = (r.w & 0xffu) | (1u << 8);
+        }
+        {
+            // nearest non-neighbour site to the vertex sphere, all lanes' searches in step
+            Trav tr{0u, 0u, need};
+            float nd = FLT_MAX;
+            const uint4 r = v < w.V ? L.vtx[v] : uint4{0, 0, 0, 0};
+            const F4 fcur = toF(cur);
+            while (ballot(tr.live) != 0) {
+                if (tr.live) vertexStep(c, L, S.stNode, S.stDist, w.F, i, r, fcur, tr, nd, nb, flags);
+            }
         }
         bool invalid = false;
         if (nb != kNone) invalid = intersectD(cur, loadSite(c, nb));
@@ -195,37 +227,51 @@ struct EdgeQuery {
     F4 ball[kCoverBalls];     // every contributor j has |p_j - c| - r_j <= r for one ball (edge_bound.h)
 };
 
-__device__ __forceinline__ uint32_t edgeSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, const D4& si, uint32_t k,
-                               uint32_t l, const EdgeQuery& q, uint32_t& flags) {
+// Pop the next node of a lane's depth-first traversal, or end it.
+__device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
+    if (t.sp == 0) {
+        t.live = false;
+    } else {
+        t.sp--;
+        t.node = stNode[t.sp][laneId()];
+    }
+}
+
+// One step of a lane's contributor traversal (algorithm.inl:616-793): visit
+// one internal node (descend to the closer accepted child, stack the other)
+// or test one leaf.  Returns the leaf's site when it is a candidate: other
+// than i, k, l, not buried, touching the search sphere and, for a
+// hyperbolic edge, on the right side of both tangent planes.  A lane walks
+// its traversal in the reference's order; the wave never waits for lanes
+// to reach leaves together.
+__device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, uint32_t (*stNode)[64], uint32_t i, uint32_t k,
+                                             uint32_t l, const EdgeQuery& q, Trav& t, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
-    ValLds& M = const_cast<ValLds&>(S);
-    uint32_t sp = 0, node = 0;
-    const D4 sk = loadSite(c, k), sl = loadSite(c, l);
-    while (true) {
-        while (node < n - 1) {
-            const Node nn = c.nodes[node];
-            Box lb, rb;
-            nodeBoxes(nn, lb, rb);
-            const uint32_t L = uint32_t(nn.d.x), R = uint32_t(nn.d.y);
-            if (L >= 2 * n - 1 || R >= 2 * n - 1) {
-                atomicOr(c.err, kErrKnnChild);
-                node = 2 * n;
-                break;
-            }
+    const uint32_t node = t.node;
+    if (node < n - 1) {
+        const Node nn = c.nodes[node];
+        Box lb, rb;
+        nodeBoxes(nn, lb, rb);
+        const uint32_t L0 = uint32_t(nn.d.x), R0 = uint32_t(nn.d.y);
+        bool ef = false, ec = false;
+        uint32_t farChild = 0, closeChild = 0;
+        if (L0 >= 2 * n - 1 || R0 >= 2 * n - 1) {
+            atomicOr(c.err, kErrKnnChild);
+        } else {
             const F3 cpL = closestPoint(lb, q.bs), cpR = closestPoint(rb, q.bs);
             const float dL = boxDistance(lb, q.bs), dR = boxDistance(rb, q.bs);
             const bool lc = dL < dR;
             const float minFar = lc ? dR : dL;
             const F3 farPoint = selF3(lc, cpR, cpL);
-            const uint32_t farChild = lc ? R : L;
+            farChild = lc ? R0 : L0;
             const float minClose = lc ? dL : dR;
             const F3 closePoint = selF3(lc, cpL, cpR);
-            const uint32_t closeChild = lc ? L : R;
-            bool ef = minFar < 0.f;
+            closeChild = lc ? L0 : R0;
+            ef = minFar < 0.f;
             if (q.hyperbola && ef)
                 ef = dotf(q.n1, subf(farPoint, q.pp1)) < 0.f && dotf(q.n2, subf(farPoint, q.pp2)) < 0.f;
-            bool ec = minClose < 0.f;
+            ec = minClose < 0.f;
             if (q.hyperbola && ec)
                 ec = dotf(q.n1, subf(closePoint, q.pp1)) < 0.f && dotf(q.n2, subf(closePoint, q.pp2)) < 0.f;
             if (q.cover) {
@@ -243,44 +289,38 @@ __device__ __forceinline__ uint32_t edgeSearch(const CellCtx& c, const ValLds& S
                 ef = ef && tf;
                 ec = ec && tc;
             }
-            if (!ef && !ec) break;
-            node = ec ? closeChild : farChild;
+        }
+        if (ec || ef) {
+            t.node = ec ? closeChild : farChild;
             if (ec && ef) {
-                if (sp < kStackDepth) {
-                    M.stNode[sp][lane] = farChild;
-                    sp++;
+                if (t.sp < kStackDepth) {
+                    stNode[t.sp][lane] = farChild;
+                    t.sp++;
                 } else {
                     flags |= kFlagStackOverflow;
                 }
             }
+        } else {
+            popNode(stNode, t);
         }
-        if (node >= n - 1 && node < 2 * n - 1) {
-            const uint32_t j = node - (n - 1);
-            bool cc = j != i && j != k && j != l && c.status[j] != kStatusBuried;
-            if (cc) {
-                const F4 fsj = c.sorted[j];
-                cc = intersectF(fsj, q.bs);
-                if (q.hyperbola) {
-                    const F3 cj = xyzf(fsj);
-                    cc = cc && dotf(q.n1, subf(cj, q.pp1)) < -fsj.w && dotf(q.n2, subf(cj, q.pp2)) < -fsj.w;
-                }
-            }
-            if (cc && knownLane(S.cell, F, j)) cc = false;
-            if (cc) {
-                D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
-                uint32_t mask = quadrisector(si, loadSite(c, j), sk, sl, x0, x1);
-                if (mask != 3) mask = 0;
-                uint32_t valid = mask;
-                if ((mask & 1u) && !clearOfNeighbours(c, S.cell, F, x0, k, l)) valid &= ~1u;
-                if ((mask & 2u) && !clearOfNeighbours(c, S.cell, F, x1, k, l)) valid &= ~2u;
-                if (valid) return j;
-            }
-        }
-        if (sp == 0) break;
-        sp--;
-        node = M.stNode[sp][lane];
+        return kNone;
     }
-    return kNone;
+    uint32_t cand = kNone;
+    if (node < 2 * n - 1) {
+        const uint32_t j = node - (n - 1);
+        bool cc = j != i && j != k && j != l && c.status[j] != kStatusBuried;
+        if (cc) {
+            const F4 fsj = c.sorted[j];
+            cc = intersectF(fsj, q.bs);
+            if (q.hyperbola) {
+                const F3 cj = xyzf(fsj);
+                cc = cc && dotf(q.n1, subf(cj, q.pp1)) < -fsj.w && dotf(q.n2, subf(cj, q.pp2)) < -fsj.w;
+            }
+        }
+        if (cc) cand = j;
+    }
+    popNode(stNode, t);
+    return cand;
 }
 
 __device__ __forceinline__ uint32_t waveOr(uint32_t v) {
@@ -317,7 +357,7 @@ __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L
 
 __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_t* __restrict__ tasks,
                                                         uint32_t maxRestarts) {
-    __shared__ ValLds S;
+    __shared__ EdgeLds S;
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
     const uint32_t i = tasks[blockIdx.x];
@@ -337,7 +377,7 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
     const uint32_t stepNb = w.E / 64 + 1;
     uint32_t es = 0, restarts = 0;
     while (es < stepNb) {
-        buildNeighbours(L, w);
+        buildNeighbours(c, L, w);
         const uint32_t e = es * 64 + lane;
         const bool inRange = e < w.E;
         bool need = inRange && L.edg[e].w == 2u;
@@ -390,17 +430,53 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
         while (bm) {
             const uint32_t t = uint32_t(__builtin_ctzll(bm));
             bm &= bm - 1;
-            const int cnt = waveEdgeCover(c, L, S.cover, S.coverOut, w.F, si, readLane(k, t), readLane(l, t));
+            const int cnt =
+                waveEdgeCover(c, L, S.cover, S.coverOut, w.F, si, readLane(k, t), readLane(l, t));
             const bool mine = cnt > 0 && lane == t;
             q.cover = mine ? cnt : q.cover;
 #pragma unroll
             for (int m = 0; m < kCoverBalls; m++) q.ball[m] = selF4(mine, S.coverOut.ball[m], q.ball[m]);
             waveSync();
         }
+        // the first contributor of each lane's traversal: a candidate with two
+        // vertex solutions of which one is clear of the cell's neighbours
         uint32_t found = kNone;
-        if (need) {
-            found = edgeSearch(c, S, w.F, i, si, k, l, q, flags);
-            if (found == kNone) L.edg[e].w = 1u;
+        {
+            Trav tr{0u, 0u, need};
+            D4 sk{0, 0, 0, 0}, sl{0, 0, 0, 0};
+            if (need) {
+                sk = loadSite(c, k);
+                sl = loadSite(c, l);
+            }
+            uint32_t cand = kNone;  // candidate waiting for the wave's test
+            while (true) {
+                const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
+                if (stepping == 0 && waiting == 0) break;
+                if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
+                    // test the waiting candidates together: not yet a neighbour,
+                    // two vertex solutions, one clear of the neighbours
+                    if (cand != kNone) {
+                        bool ok = !knownLane(L, w.F, cand);
+                        if (ok) {
+                            D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
+                            uint32_t mask = quadrisector(si, loadSite(c, cand), sk, sl, x0, x1);
+                            if (mask != 3u) mask = 0u;
+                            uint32_t valid = mask;
+                            if ((mask & 1u) && !clearOfNeighbours(c, L, w.F, x0, k, l)) valid &= ~1u;
+                            if ((mask & 2u) && !clearOfNeighbours(c, L, w.F, x1, k, l)) valid &= ~2u;
+                            ok = valid != 0u;
+                        }
+                        if (ok) {
+                            found = cand;
+                            tr.live = false;
+                        }
+                        cand = kNone;
+                    }
+                    continue;
+                }
+                if (tr.live && cand == kNone) cand = edgeStep(c, S.stNode, i, k, l, q, tr, flags);
+            }
+            if (need && found == kNone) L.edg[e].w = 1u;
         }
         const uint64_t m = ballot(found != kNone);
         waveSync();
@@ -430,73 +506,71 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
 // ---------------------------------------------------------------------------
 // Contributor search along a bisector (algorithm.inl:859-1233)
 // ---------------------------------------------------------------------------
-__device__ uint32_t bisectorSearch(const CellCtx& c, const ValLds& S, uint32_t F, uint32_t i, const D4& si,
-                                   uint32_t bk, F3 fci, F3 fck, F4 bs, float cylR, uint32_t& flags) {
+// One step of a lane's bisector traversal (algorithm.inl:1019-1176).
+// Returns the leaf's site when it is a candidate: other than i, bk, inside
+// the search sphere and the cylinder around the bisector axis.
+__device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*stNode)[64], uint32_t i, uint32_t bk,
+                                                 F3 fci, F3 fck, F4 bs, float cylR, Trav& t, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
-    ValLds& M = const_cast<ValLds&>(S);
-    const D4 sk = loadSite(c, bk);
-    uint32_t sp = 0, node = 0;
-    while (true) {
-        while (node < n - 1) {
-            const Node nn = c.nodes[node];
-            Box lb, rb;
-            nodeBoxes(nn, lb, rb);
-            const uint32_t L = uint32_t(nn.d.x), R = uint32_t(nn.d.y);
-            if (L >= 2 * n - 1 || R >= 2 * n - 1) {
-                atomicOr(c.err, kErrKnnChild);
-                node = 2 * n;
-                break;
-            }
+    const uint32_t node = t.node;
+    if (node < n - 1) {
+        const Node nn = c.nodes[node];
+        Box lb, rb;
+        nodeBoxes(nn, lb, rb);
+        const uint32_t L0 = uint32_t(nn.d.x), R0 = uint32_t(nn.d.y);
+        bool ef = false, ec = false;
+        uint32_t farChild = 0, closeChild = 0;
+        if (L0 >= 2 * n - 1 || R0 >= 2 * n - 1) {
+            atomicOr(c.err, kErrKnnChild);
+        } else {
             const float dL = boxDistance(lb, bs), dR = boxDistance(rb, bs);
             const bool lc = dL < dR;
             const float minFar = lc ? dR : dL;
-            const uint32_t farChild = lc ? R : L;
+            farChild = lc ? R0 : L0;
             const float minClose = lc ? dL : dR;
-            const uint32_t closeChild = lc ? L : R;
-            const bool ef = minFar < 0.f, ec = minClose < 0.f;
-            if (!ef && !ec) break;
-            node = ec ? closeChild : farChild;
+            closeChild = lc ? L0 : R0;
+            ef = minFar < 0.f;
+            ec = minClose < 0.f;
+        }
+        if (ec || ef) {
+            t.node = ec ? closeChild : farChild;
             if (ec && ef) {
-                if (sp < kStackDepth) {
-                    M.stNode[sp][lane] = farChild;
-                    sp++;
+                if (t.sp < kStackDepth) {
+                    stNode[t.sp][lane] = farChild;
+                    t.sp++;
                 } else {
                     flags |= kFlagStackOverflow;
                 }
             }
+        } else {
+            popNode(stNode, t);
         }
-        if (node >= n - 1 && node < 2 * n - 1) {
-            const uint32_t j = node - (n - 1);
-            bool cc = j != i && j != bk;
-            if (cc) {
-                const F4 fsj = c.sorted[j];
-                const F3 fcj = xyzf(fsj);
-                cc = bs.w >= fsj.w && (lenf(subf(xyzf(bs), fcj)) <= (bs.w - fsj.w));
-                if (cc) {
-                    const F3 itoj = subf(fcj, fci);
-                    const float v = dotf(itoj, normalizef(subf(fck, fci)));
-                    const float d = __builtin_sqrtf(dotf(itoj, itoj) - v * v);
-                    cc = cc && (d + fsj.w) < cylR;
-                }
-            }
-            if (cc && knownLane(S.cell, F, j)) cc = false;
-            if (cc) {
-                D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
-                const uint32_t cnt = trisector(si, loadSite(c, j), sk, x0, x1);
-                cc = cnt == 2 && clearOfNeighbours(c, S.cell, F, x1, bk, bk);
-                if (cc) return j;
-            }
-        }
-        if (sp == 0) break;
-        sp--;
-        node = M.stNode[sp][lane];
+        return kNone;
     }
-    return kNone;
+    uint32_t cand = kNone;
+    if (node < 2 * n - 1) {
+        const uint32_t j = node - (n - 1);
+        bool cc = j != i && j != bk;
+        if (cc) {
+            const F4 fsj = c.sorted[j];
+            const F3 fcj = xyzf(fsj);
+            cc = bs.w >= fsj.w && (lenf(subf(xyzf(bs), fcj)) <= (bs.w - fsj.w));
+            if (cc) {
+                const F3 itoj = subf(fcj, fci);
+                const float v = dotf(itoj, normalizef(subf(fck, fci)));
+                const float d = __builtin_sqrtf(dotf(itoj, itoj) - v * v);
+                cc = cc && (d + fsj.w) < cylR;
+            }
+        }
+        if (cc) cand = j;
+    }
+    popNode(stNode, t);
+    return cand;
 }
 
 __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t maxRestarts) {
-    __shared__ ValLds S;
+    __shared__ BisectorLds S;
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
     const uint32_t i = blockIdx.x;
@@ -507,7 +581,7 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t 
     const uint32_t edgesLoaded = w.E;
     uint32_t flags = 0;
     // sorted distinct neighbours above i (algorithm.inl:903-968)
-    buildNeighbours(L, w);
+    buildNeighbours(c, L, w);
     const uint32_t F = w.F;
     uint32_t B = 0;
     for (uint32_t b = 0; b < F; b += 64) {
@@ -542,29 +616,55 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t 
     const uint32_t stepNb = B / 64 + 1;
     uint32_t bsi = 0, restarts = 0;
     while (bsi < stepNb) {
-        buildNeighbours(L, w);
+        buildNeighbours(c, L, w);
         const uint32_t bid = bsi * 64 + lane;
         bool need = bid < B && S.bis[bid] != kNone;
-        uint32_t found = kNone;
+        uint32_t found = kNone, bk = kNone;
+        D4 sk{0, 0, 0, 0};
+        F3 fck{0, 0, 0};
+        F4 bsph{0, 0, 0, 0};
+        float cylR = 0.f;
         if (need) {
-            const uint32_t bk = S.bis[bid];
-            const D4 sk = loadSite(c, bk);
-            const F3 fck{float(sk.x), float(sk.y), float(sk.z)};
+            bk = S.bis[bid];
+            sk = loadSite(c, bk);
+            fck = F3{float(sk.x), float(sk.y), float(sk.z)};
             const float frk = float(sk.w);
             const float dist = lenf(subf(fci, fck)) + fri + frk;
             if (ltF(dist, kr)) {
                 S.bis[bid] = kNone;
                 need = false;
             }
-            if (need) {
-                const F3 center = mulf(addf(fci, fck), .5f);
-                const float radius = lenf(subf(center, fci)) + __builtin_fmaxf(fri, frk);
-                const F4 bsph{center.x, center.y, center.z, radius};
-                const float cylR = __builtin_fmaxf(fri, frk);
-                found = bisectorSearch(c, S, w.F, i, si, bk, fci, fck, bsph, cylR, flags);
-                if (found == kNone) S.bis[bid] = kNone;
-            }
+            const F3 center = mulf(addf(fci, fck), .5f);
+            const float radius = lenf(subf(center, fci)) + __builtin_fmaxf(fri, frk);
+            bsph = F4{center.x, center.y, center.z, radius};
+            cylR = __builtin_fmaxf(fri, frk);
         }
+        // the first site of each lane's traversal whose trisector with i, bk
+        // has two points, the second clear of the cell's neighbours
+        Trav tr{0u, 0u, need};
+        uint32_t cand = kNone;
+        while (true) {
+            const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
+            if (stepping == 0 && waiting == 0) break;
+            if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
+                if (cand != kNone) {
+                    bool ok = !knownLane(L, w.F, cand);
+                    if (ok) {
+                        D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
+                        const uint32_t cnt = trisector(si, loadSite(c, cand), sk, x0, x1);
+                        ok = cnt == 2u && clearOfNeighbours(c, L, w.F, x1, bk, bk);
+                    }
+                    if (ok) {
+                        found = cand;
+                        tr.live = false;
+                    }
+                    cand = kNone;
+                }
+                continue;
+            }
+            if (tr.live && cand == kNone) cand = bisectorStep(c, S.stNode, i, bk, fci, fck, bsph, cylR, tr, flags);
+        }
+        if (need && found == kNone) S.bis[bid] = kNone;
         const uint64_t m = ballot(found != kNone);
         waveSync();
         bool changed = false;

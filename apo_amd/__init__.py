@@ -142,6 +142,37 @@ class AlgorithmGPU:
         return c, V, E
 
 
+    # -- AlgorithmGPU::toTopology ---------------------------------------------
+    def topology(self) -> dict:
+        """Bisectors (m, 2), trisectors (m, 3), quadrisectors (m, 4) in original
+        ids, in the order of AlgorithmGPU::toTopology (algorithm.inl:1732-1821)."""
+        L = _lib.lib()
+        counts = np.zeros(3, np.uint64)
+        cp = counts.ctypes.data_as(_lib._u64p)
+        _lib.check(self._ctx, L.apo_get_topology_sets(self._ctx, None, 0, None, 0, None, 0, cp))
+        b = np.zeros((int(counts[0]), 2), np.uint32)
+        t = np.zeros((int(counts[1]), 3), np.uint32)
+        q = np.zeros((int(counts[2]), 4), np.uint32)
+        _lib.check(self._ctx, L.apo_get_topology_sets(self._ctx, _lib.uptr(b), len(b), _lib.uptr(t), len(t),
+                                                      _lib.uptr(q), len(q), cp))
+        return {"bisectors": b, "trisectors": t, "quadrisectors": q}
+
+    def cell_topology(self):
+        """Per-site cell records in original order and ids: (vertex_offsets,
+        vertices (m, 4), edge_offsets, edges (m, 4))."""
+        L = _lib.lib()
+        n = len(self.sites)
+        vo = np.zeros(n + 1, np.uint64)
+        eo = np.zeros(n + 1, np.uint64)
+        _lib.check(self._ctx, L.apo_get_cell_topology(self._ctx, vo.ctypes.data_as(_lib._u64p), None, 0,
+                                                      eo.ctypes.data_as(_lib._u64p), None, 0))
+        v = np.zeros((int(vo[-1]), 4), np.uint32)
+        e = np.zeros((int(eo[-1]), 4), np.uint32)
+        _lib.check(self._ctx, L.apo_get_cell_topology(self._ctx, vo.ctypes.data_as(_lib._u64p), _lib.uptr(v), len(v),
+                                                      eo.ctypes.data_as(_lib._u64p), _lib.uptr(e), len(e)))
+        return vo, v, eo, e
+
+
 def compute_apollonius_diagram(sites, k: int = 16) -> VertexDiagram:
     """apo::gpu::computeApolloniusDiagram (apo/src/gpu/algorithm.cu:6-15)."""
     alg = AlgorithmGPU(sites, k=k)

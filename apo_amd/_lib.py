@@ -68,6 +68,9 @@ SIGNATURES = {
     "apo_get_knns": (ctypes.c_int, [_vp, _u32p]),
     "apo_get_cell_counts": (ctypes.c_int, [_vp, _u32p, _u32p, _u8p, _f32p, _u32p]),
     "apo_get_all_cells": (ctypes.c_int, [_vp, _u32p, _u32p]),
+    "apo_get_topology_sets": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint64, _u32p, ctypes.c_uint64, _u32p,
+                                             ctypes.c_uint64, _u64p]),
+    "apo_get_cell_topology": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p, ctypes.c_uint64]),
 }
 
 

@@ -122,6 +122,16 @@ hipError_t alloc(T*& p, size_t count) {
     return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
 }
 
+// Scoped device buffer of the introspection calls (freed on every return path).
+template <typename T>
+struct DeviceTemp {
+    T* p = nullptr;
+    hipError_t alloc(size_t count) { return hipMalloc(reinterpret_cast<void**>(&p), (count ? count : 1) * sizeof(T)); }
+    ~DeviceTemp() {
+        if (p) (void)hipFree(p);
+    }
+};
+
 uint32_t tilesFor(uint32_t n) { return (n + 4095u) / 4096u; }
 
 apo_status ensureCapacity(apo_ctx* ctx, uint32_t n) {
@@ -610,6 +620,96 @@ apo_status apo_get_all_cells(apo_ctx* ctx, uint32_t* vertices, uint32_t* edges) 
             eo += en[i];
         }
     }
+    return APO_OK;
+}
+
+apo_status apo_get_topology_sets(apo_ctx* ctx, uint32_t* bisectors, uint64_t bisector_cap, uint32_t* trisectors,
+                                 uint64_t trisector_cap, uint32_t* quadrisectors, uint64_t quadrisector_cap,
+                                 uint64_t* counts3) {
+    if (!ctx || !counts3) return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t n = ctx->n;
+    const CellCtx cc = cellCtx(ctx);
+    DeviceTemp<uint32_t> cnt;
+    APO_HIP(cnt.alloc(3 * n));
+    APO_HIP(topologyCountLaunch(cc, cnt.p, st));
+    std::vector<uint32_t> hc(3 * n);
+    APO_HIP(hipMemcpyAsync(hc.data(), cnt.p, 3 * n * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> off(3 * n);
+    uint64_t tot[3] = {0, 0, 0};
+    for (size_t i = 0; i < n; i++)
+        for (int t = 0; t < 3; t++) {
+            off[3 * i + t] = tot[t];
+            tot[t] += hc[3 * i + t];
+        }
+    for (int t = 0; t < 3; t++) counts3[t] = tot[t];
+    if (!bisectors && !trisectors && !quadrisectors) return APO_OK;
+    if (!bisectors || !trisectors || !quadrisectors) return fail(ctx, APO_ERR_INVALID, "pass all three arrays or none");
+    if (bisector_cap < tot[0] || trisector_cap < tot[1] || quadrisector_cap < tot[2])
+        return fail(ctx, APO_ERR_CAPACITY, "topology buffers too small");
+    DeviceTemp<uint64_t> doff;
+    DeviceTemp<uint32_t> b, t, q;
+    APO_HIP(doff.alloc(3 * n));
+    APO_HIP(b.alloc(2 * tot[0]));
+    APO_HIP(t.alloc(3 * tot[1]));
+    APO_HIP(q.alloc(4 * tot[2]));
+    APO_HIP(hipMemcpyAsync(doff.p, off.data(), 3 * n * 8, hipMemcpyHostToDevice, st));
+    APO_HIP(topologyWriteLaunch(cc, doff.p, ctx->idxA, b.p, t.p, q.p, st));
+    APO_HIP(hipMemcpyAsync(bisectors, b.p, 2 * tot[0] * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(trisectors, t.p, 3 * tot[1] * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(quadrisectors, q.p, 4 * tot[2] * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+
+apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_t* vertices, uint64_t vertex_cap,
+                                 uint64_t* edge_offsets, uint32_t* edges, uint64_t edge_cap) {
+    if (!ctx) return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    if ((vertices || edges) && !(vertices && edges)) return fail(ctx, APO_ERR_INVALID, "pass both record arrays or none");
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t n = ctx->n;
+    std::vector<uint32_t> vn(n), en(n), idx(n);
+    std::vector<uint8_t> status(n);
+    APO_HIP(hipMemcpyAsync(vn.data(), ctx->vnb, n * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(en.data(), ctx->enb, n * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(idx.data(), ctx->idxA, n * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(status.data(), ctx->status, n, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> vc(n + 1, 0), ec(n + 1, 0);
+    for (size_t i = 0; i < n; i++) {
+        if (status[i] == kStatusBuried) continue;
+        vc[idx[i]] = vn[i] < kVertexCap ? vn[i] : kVertexCap;
+        ec[idx[i]] = en[i] < kEdgeCap ? en[i] : kEdgeCap;
+    }
+    uint64_t vt = 0, et = 0;
+    for (size_t o = 0; o <= n; o++) {
+        const uint64_t a = vc[o], b = ec[o];
+        vc[o] = vt;
+        ec[o] = et;
+        vt += a;
+        et += b;
+    }
+    if (vertex_offsets) std::memcpy(vertex_offsets, vc.data(), (n + 1) * 8);
+    if (edge_offsets) std::memcpy(edge_offsets, ec.data(), (n + 1) * 8);
+    if (!vertices) return APO_OK;
+    if (vertex_cap < vt || edge_cap < et) return fail(ctx, APO_ERR_CAPACITY, "record buffers too small");
+    DeviceTemp<uint64_t> dvo, deo;
+    DeviceTemp<uint4> dv, de;
+    APO_HIP(dvo.alloc(n + 1));
+    APO_HIP(deo.alloc(n + 1));
+    APO_HIP(dv.alloc(vt));
+    APO_HIP(de.alloc(et));
+    APO_HIP(hipMemcpyAsync(dvo.p, vc.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(deo.p, ec.data(), (n + 1) * 8, hipMemcpyHostToDevice, st));
+    APO_HIP(cellRecordsLaunch(cellCtx(ctx), ctx->idxA, dvo.p, deo.p, dv.p, de.p, st));
+    APO_HIP(hipMemcpyAsync(vertices, dv.p, vt * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(edges, de.p, et * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
     return APO_OK;
 }
 

@@ -119,6 +119,13 @@ struct OutputBuffers {
 hipError_t outputCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st);
 hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices, hipStream_t st);
 
+// Topology (topology.hip)
+hipError_t topologyCountLaunch(const CellCtx& c, uint32_t* counts, hipStream_t st);
+hipError_t topologyWriteLaunch(const CellCtx& c, const uint64_t* offsets, const uint32_t* indices, uint32_t* bis,
+                               uint32_t* tri, uint32_t* quad, hipStream_t st);
+hipError_t cellRecordsLaunch(const CellCtx& c, const uint32_t* indices, const uint64_t* vOff, const uint64_t* eOff,
+                             uint4* vOut, uint4* eOut, hipStream_t st);
+
 hipError_t checkEmptyLaunch(const F4* sorted, const uint32_t* indices, uint32_t* rank, const Node* nodes, uint32_t n,
                             const D4* xyzw, const uint4* ids, uint64_t count, double tol,
                             unsigned long long* violations, uint32_t* err, hipStream_t st);

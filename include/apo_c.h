@@ -107,6 +107,25 @@ apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats);
 apo_status apo_check_empty_spheres(apo_ctx* ctx, const double* xyzw, const uint32_t* ids, uint64_t count, double tol,
                                    uint64_t* violations);
 
+/* Topology sets of AlgorithmGPU::toTopology (algorithm.inl:1732-1821): for
+ * every cell in sorted order, its distinct bisectors {i, j}, trisectors
+ * {i, j, k} and quadrisectors {i, j, k, l} with i below the others and no
+ * artificial site, ascending within the cell, as original site ids (2, 3
+ * and 4 u32 per entry).  counts3 receives the three entry counts; pass NULL
+ * arrays to query them (two-call pattern). */
+apo_status apo_get_topology_sets(apo_ctx* ctx, uint32_t* bisectors, uint64_t bisector_cap, uint32_t* trisectors,
+                                 uint64_t trisector_cap, uint32_t* quadrisectors, uint64_t quadrisector_cap,
+                                 uint64_t* counts3);
+
+/* Per-site cell topology (the cell records of AlgorithmGPU's dVertices /
+ * dEdges, algorithm.cuh:38-43) as CSR in original site order with original
+ * ids: vertex records (j, k, l, type) and edge records (j, k, vertexNb, 0);
+ * artificial sites keep ids n..n+5, vertexNb 0xffffffff marks a closed
+ * edge, buried sites have no records.  vertex_offsets / edge_offsets: n+1
+ * entries (may be NULL); NULL record arrays only fill the offsets. */
+apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_t* vertices, uint64_t vertex_cap,
+                                 uint64_t* edge_offsets, uint32_t* edges, uint64_t edge_cap);
+
 /* ---- introspection of the device state (the public members dSites,
  * bvh, dStatus, dVertices, dEdges of AlgorithmGPU, algorithm.cuh:147-155) ---- */
 

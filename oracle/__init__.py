@@ -86,6 +86,8 @@ def lib():
         L.orc_get_cell.argtypes = [ctypes.c_void_p, ctypes.c_uint32, u32p, u32p]
         L.orc_get_all_cells.restype = None
         L.orc_get_all_cells.argtypes = [ctypes.c_void_p, u32p, u32p]
+        L.orc_topology.restype = None
+        L.orc_topology.argtypes = [ctypes.c_void_p, u64p, u32p, u32p, u32p]
         L.orc_get_knns.restype = None
         L.orc_get_knns.argtypes = [ctypes.c_void_p, u32p]
         L.orc_get_stats.restype = None
@@ -258,6 +260,16 @@ class Diagram:
         E = np.zeros((int(c["edge_nb"].sum()), 4), np.uint32)
         lib().orc_get_all_cells(self._h, _p(V, u32p), _p(E, u32p))
         return c, V, E
+
+    def topology(self) -> dict:
+        """toTopology sets (algorithm.inl:1732-1821), original ids."""
+        c = np.zeros(3, np.uint64)
+        lib().orc_topology(self._h, _p(c, u64p), None, None, None)
+        b = np.zeros((int(c[0]), 2), np.uint32)
+        t = np.zeros((int(c[1]), 3), np.uint32)
+        q = np.zeros((int(c[2]), 4), np.uint32)
+        lib().orc_topology(self._h, _p(c, u64p), _p(b, u32p), _p(t, u32p), _p(q, u32p))
+        return {"bisectors": b, "trisectors": t, "quadrisectors": q}
 
     def knns(self):
         out = np.zeros((self.n, self.cfg.k), np.uint32)

@@ -31,6 +31,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <set>
 #include <vector>
 
 #ifdef _OPENMP
@@ -1829,6 +1830,50 @@ void orc_get_all_cells(const orc_diagram* d, uint32_t* vertices, uint32_t* edges
         vo += c.V.size();
         eo += c.E.size();
     }
+}
+
+// toTopology (algorithm.inl:1732-1821): per cell in sorted order, the std::set
+// of bisectors {i, j}, trisectors {i, j, k}, quadrisectors {i, j, k, l} with i
+// below the others and no artificial site, emitted in set order as original
+// ids.  With null outputs only the counts are produced.
+void orc_topology(const orc_diagram* d, uint64_t* counts3, uint32_t* bis, uint32_t* tri, uint32_t* quad) {
+    const Context& C = d->C;
+    const uint32_t n = C.n;
+    const std::vector<uint32_t>& idx = d->bvh.indices;
+    uint64_t nb = 0, nt = 0, nq = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const Cell& cell = C.cells[i];
+        std::set<std::array<uint32_t, 2>> bs;
+        std::set<std::array<uint32_t, 3>> ts;
+        std::set<std::array<uint32_t, 4>> qs;
+        for (const Edg& e : cell.E) {
+            if (i < e.j && e.j < n) bs.insert({i, e.j});
+            if (i < e.k && e.k < n) bs.insert({i, e.k});
+            if (i < e.j && i < e.k && e.j < n && e.k < n) ts.insert({i, e.j, e.k});
+        }
+        for (const Vtx& v : cell.V)
+            if (i < v.j && i < v.k && i < v.l && v.j < n && v.k < n && v.l < n) qs.insert({i, v.j, v.k, v.l});
+        for (const auto& b : bs) {
+            if (bis) {
+                bis[2 * nb] = idx[b[0]];
+                bis[2 * nb + 1] = idx[b[1]];
+            }
+            nb++;
+        }
+        for (const auto& t : ts) {
+            if (tri)
+                for (int a = 0; a < 3; a++) tri[3 * nt + a] = idx[t[a]];
+            nt++;
+        }
+        for (const auto& q : qs) {
+            if (quad)
+                for (int a = 0; a < 4; a++) quad[4 * nq + a] = idx[q[a]];
+            nq++;
+        }
+    }
+    counts3[0] = nb;
+    counts3[1] = nt;
+    counts3[2] = nq;
 }
 
 void orc_get_knns(const orc_diagram* d, uint32_t* knn) { std::memcpy(knn, d->knn.data(), d->knn.size() * 4); }

@@ -98,6 +98,10 @@ void orc_get_cell_counts(const orc_diagram* d, uint32_t* vertex_nb, uint32_t* ed
 void orc_get_cell(const orc_diagram* d, uint32_t i, uint32_t* vertices, uint32_t* edges);
 /* Concatenated records of all cells in cell order (counts from orc_get_cell_counts). */
 void orc_get_all_cells(const orc_diagram* d, uint32_t* vertices, uint32_t* edges);
+/* toTopology (algorithm.inl:1732-1821): counts3 = bisectors, trisectors,
+ * quadrisectors; the arrays (2, 3, 4 u32 per entry, original ids) may be NULL. */
+void orc_topology(const orc_diagram* d, uint64_t* counts3, uint32_t* bisectors, uint32_t* trisectors,
+                  uint32_t* quadrisectors);
 void orc_get_knns(const orc_diagram* d, uint32_t* knn);
 /* Stage statistics: [0] uncertified after cells, [1] tasks per validation
  * iteration summed, [2] iterations run, [3] overflowed cells, [4] vertices

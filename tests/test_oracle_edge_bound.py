@@ -31,6 +31,7 @@ CASES = {
     "loguniform_6k": lambda: datasets.loguniform_sites(6000, seed=3),
     "loguniform_6k_seed9": lambda: datasets.loguniform_sites(6000, seed=9),
     "wide_radii_2k": lambda: datasets.uniform_sites(2000, seed=5, r_min=0.0, r_max=0.05),
+    "protein_4k": lambda: datasets.protein_like_sites(4000, seed=2),
 }
 
 

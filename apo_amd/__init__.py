@@ -35,7 +35,8 @@ class AlgorithmGPU:
     """One construction context on one GPU (AlgorithmGPU<K>, algorithm.cuh:12-46)."""
 
     def __init__(self, sites, k: int = 16, max_iterations: int = 10, max_restarts: int = 100,
-                 stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0):
+                 stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0, proposal=None,
+                 proposal_cost: float = 0.0):
         self.sites = np.ascontiguousarray(np.asarray(sites, dtype=np.float32).reshape(-1, 4))
         self.k = int(k)
         cfg = _lib.Config()
@@ -46,8 +47,12 @@ class AlgorithmGPU:
         cfg.stage_mask = stage_mask
         cfg.waves_per_cu = waves_per_cu
         cfg.device = device
+        cfg.proposal_cost = proposal_cost
         self._ctx = _lib.create(cfg)
         self.built = False
+        if proposal is not None:
+            self._proposal = np.ascontiguousarray(proposal, np.float32)
+            _lib.check(self._ctx, _lib.lib().apo_proposal_set(self._ctx, _lib.fptr(self._proposal)))
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -141,6 +146,12 @@ class AlgorithmGPU:
         _lib.check(self._ctx, _lib.lib().apo_get_all_cells(self._ctx, _lib.uptr(V), _lib.uptr(E)))
         return c, V, E
 
+
+    def proposal_classes(self) -> np.ndarray:
+        """Classes the learned proposal gave each site in the last build."""
+        out = np.zeros(len(self.sites), np.uint8)
+        _lib.check(self._ctx, _lib.lib().apo_proposal_classes(self._ctx, _lib.u8ptr(out)))
+        return out
 
     # -- AlgorithmGPU::toTopology ---------------------------------------------
     def topology(self) -> dict:

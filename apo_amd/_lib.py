@@ -23,7 +23,8 @@ class ApoError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint32), ("max_iterations", ctypes.c_uint32), ("max_restarts", ctypes.c_uint32),
-                ("stage_mask", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("device", ctypes.c_uint32)]
+                ("stage_mask", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("device", ctypes.c_uint32),
+                ("proposal_cost", ctypes.c_float)]
 
 
 class Stats(ctypes.Structure):
@@ -37,6 +38,7 @@ class Stats(ctypes.Structure):
                 ("validation_launches", ctypes.c_uint64), ("validation_tasks", ctypes.c_uint64),
                 ("bytes_cells", ctypes.c_uint64), ("bytes_vertex_validation", ctypes.c_uint64),
                 ("bytes_edge_validation", ctypes.c_uint64), ("bytes_bisector_validation", ctypes.c_uint64),
+                ("ms_proposal", ctypes.c_double), ("proposal", ctypes.c_uint64), ("cells_certified", ctypes.c_uint64),
                 ("reserved", ctypes.c_uint64 * 2)]
 
 
@@ -71,6 +73,12 @@ SIGNATURES = {
     "apo_get_topology_sets": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint64, _u32p, ctypes.c_uint64, _u32p,
                                              ctypes.c_uint64, _u64p]),
     "apo_get_cell_topology": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p, ctypes.c_uint64]),
+    "apo_proposal_labels": (ctypes.c_int, [_vp, _f32p, _u32, _f32p, _u8p]),
+    "apo_proposal_train": (ctypes.c_int, [_vp, _f32p, _u8p, ctypes.c_uint64, _u32, _u32, ctypes.c_float, _u32,
+                                          _f32p, _f32p, _f32p]),
+    "apo_proposal_set": (ctypes.c_int, [_vp, _f32p]),
+    "apo_proposal_infer": (ctypes.c_int, [_vp, _f32p, _u32, _u8p, _f32p]),
+    "apo_proposal_classes": (ctypes.c_int, [_vp, _u8p]),
 }
 
 

@@ -12,6 +12,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <cmath>
+#include <random>
 #include <vector>
 
 #include "../../include/apo_c.h"
@@ -88,6 +91,14 @@ struct apo_ctx {
     uint64_t vertexCount = 0;
 
     unsigned long long* bytes = nullptr;  // per-kernel algorithmic byte counters
+    // learned neighbour-count proposal
+    bool proposalOn = false;
+    ProposalWeights pw{};       // inference layout (BF16 weights)
+    float* params = nullptr;    // FP32 parameters
+    float* feat = nullptr;      // sites x kFeatures
+    uint8_t* cls = nullptr;     // predicted class per sorted site
+    uint8_t* labels = nullptr;  // label pass output per sorted site
+    uint32_t capP = 0;          // site capacity of feat / cls / labels
     hipEvent_t ev[8] = {};
     ValidationTiming vt{};
     apo_stats stats{};
@@ -134,8 +145,7 @@ struct DeviceTemp {
 
 uint32_t tilesFor(uint32_t n) { return (n + 4095u) / 4096u; }
 
-apo_status ensureCapacity(apo_ctx* ctx, uint32_t n) {
-    const uint32_t K = ctx->cfg.k;
+apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     if (n <= ctx->cap && K <= ctx->capK) return APO_OK;
     const uint32_t cap = n;
     const size_t nodes = 2 * size_t(cap) - 1;
@@ -182,6 +192,16 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n) {
     ctx->capK = K;
     return APO_OK;
 }
+
+apo_status ensureProposalBuffers(apo_ctx* ctx, uint32_t n) {
+    if (n <= ctx->capP) return APO_OK;
+    APO_HIP(alloc(ctx->feat, size_t(n) * kFeatures));
+    APO_HIP(alloc(ctx->cls, n));
+    APO_HIP(alloc(ctx->labels, n));
+    ctx->capP = n;
+    return APO_OK;
+}
+
 
 CellCtx cellCtx(apo_ctx* ctx) {
     CellCtx c;
@@ -241,7 +261,9 @@ apo_status stageCheck(apo_ctx* ctx, const char* stage) {
         if (s_ != APO_OK) return s_;                 \
     } while (0)
 
-apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
+// labelPass: LBVH, kNN with 64 neighbours, features and the label cell pass
+// (the test after every update), then stop.
+apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool labelPass = false) {
     hipStream_t st = ctx->stream;
     ctx->n = n;
     LbvhBuffers lb{ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA, ctx->codesB,
@@ -261,9 +283,22 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     APO_HIP(lbvhBuild(lb, sitesDev, n, st));
     APO_STAGE("lbvh");
     APO_HIP(hipEventRecord(ctx->ev[1], st));
-    APO_HIP(knnLaunch(cc, K, ctx->knn, st));
+    const bool learned = ctx->proposalOn && !labelPass;
+    const uint32_t Kk = (learned || labelPass) ? 64u : K;
+    APO_HIP(knnLaunch(cc, Kk, ctx->knn, st));
     APO_STAGE("knn");
     APO_HIP(hipEventRecord(ctx->ev[2], st));
+    if (labelPass) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->counters, 64u, 0u);
+        APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st));
+        APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, nullptr, ctx->labels, kLabels, st));
+        uint32_t guard = 0;
+        APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        APO_HIP(hipStreamSynchronize(st));
+        if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped: " + guardText(guard));
+        ctx->built = false;  // cells stopped at certification: not a diagram
+        return APO_OK;
+    }
     if (ctx->cfg.stage_mask & 8u) {
         // stop after the kNN pass (introspection of the LBVH and kNN state)
         APO_HIP(hipStreamSynchronize(st));
@@ -272,8 +307,18 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
         return APO_OK;
     }
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->counters, 64u, 0u);
-    if (ctx->cfg.stage_mask & 1u) APO_HIP(cellsLaunch(cc, ctx->knn, K, ctx->counters, waves, st));
-    else APO_HIP(cellsLaunch(cc, ctx->knn, 0, ctx->counters, waves, st));
+    if (learned && (ctx->cfg.stage_mask & 1u)) {
+        // features of the 16 nearest, proposal classes, then each cell with its
+        // proposed neighbour count and the plain certification test
+        APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st));
+        APO_HIP(mlpInferLaunch(ctx->feat, n, ctx->pw, ctx->cls, nullptr, ctx->cfg.proposal_cost, st));
+        APO_HIP(hipEventRecord(ctx->ev[7], st));
+        APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, ctx->cls, nullptr, kProposal, st));
+    } else {
+        APO_HIP(hipEventRecord(ctx->ev[7], st));
+        if (ctx->cfg.stage_mask & 1u) APO_HIP(cellsLaunch(cc, ctx->knn, K, ctx->counters, waves, st));
+        else APO_HIP(cellsLaunch(cc, ctx->knn, 0, ctx->counters, waves, st));
+    }
     APO_STAGE("cells");
     APO_HIP(hipEventRecord(ctx->ev[3], st));
     ValidationBuffers vb{ctx->tasks, ctx->counters + 1};
@@ -310,10 +355,15 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     APO_HIP(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[6]));
     apo_stats& S = ctx->stats;
     std::memset(&S, 0, sizeof(S));
+    float msProposal = 0.f, msCellsOnly = 0.f;
+    APO_HIP(hipEventElapsedTime(&msProposal, ctx->ev[2], ctx->ev[7]));
+    APO_HIP(hipEventElapsedTime(&msCellsOnly, ctx->ev[7], ctx->ev[3]));
     S.ms_total = all;
     S.ms_lbvh = ms[0];
     S.ms_knn = ms[1];
-    S.ms_cells = ms[2];
+    S.ms_cells = msCellsOnly;
+    S.ms_proposal = msProposal;
+    S.proposal = learned ? 1u : 0u;
     S.ms_validation = ms[3];
     S.ms_bisector = ms[4];
     S.ms_output = ms[5];
@@ -353,6 +403,7 @@ void apo_default_config(apo_config* cfg) {
     cfg->stage_mask = 7;
     cfg->waves_per_cu = 0;
     cfg->device = 0;
+    cfg->proposal_cost = 0.f;
 }
 
 apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
@@ -389,7 +440,9 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->codesB, ctx->idxA, ctx->idxB, ctx->sorted, ctx->nodes, ctx->parent, ctx->sizes,
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
-                    ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->crossList};
+                    ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->crossList,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto& e : ctx->ev)
@@ -409,8 +462,12 @@ apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n) 
     if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
     APO_HIP(hipSetDevice(ctx->device));
     ctx->built = false;
-    apo_status s = ensureCapacity(ctx, n);
+    apo_status s = ensureCapacity(ctx, n, ctx->proposalOn ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
     if (s != APO_OK) return s;
+    if (ctx->proposalOn) {
+        s = ensureProposalBuffers(ctx, n);
+        if (s != APO_OK) return s;
+    }
     return runPipeline(ctx, reinterpret_cast<const F4*>(xyzr_device), n);
 }
 
@@ -420,8 +477,12 @@ apo_status apo_build(apo_ctx* ctx, const float* xyzr, uint32_t n) {
     if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
     APO_HIP(hipSetDevice(ctx->device));
     ctx->built = false;
-    apo_status s = ensureCapacity(ctx, n);
+    apo_status s = ensureCapacity(ctx, n, ctx->proposalOn ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
     if (s != APO_OK) return s;
+    if (ctx->proposalOn) {
+        s = ensureProposalBuffers(ctx, n);
+        if (s != APO_OK) return s;
+    }
     APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, ctx->stream));
     return runPipeline(ctx, ctx->sitesIn, n);
 }
@@ -475,6 +536,8 @@ apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats) {
     ctx->stats.flagged_cells = flagged;
     ctx->stats.vertex_records = vr;
     ctx->stats.edge_records = er;
+    const uint64_t firstRound = ctx->vt.rounds ? ctx->vt.tasks[0] : 0u;
+    ctx->stats.cells_certified = n >= buried + firstRound ? n - buried - firstRound : 0u;
     *stats = ctx->stats;
     return APO_OK;
 }
@@ -710,6 +773,181 @@ apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_
     APO_HIP(hipMemcpyAsync(vertices, dv.p, vt * sizeof(uint4), hipMemcpyDeviceToHost, st));
     APO_HIP(hipMemcpyAsync(edges, de.p, et * sizeof(uint4), hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+
+apo_status apo_proposal_labels(apo_ctx* ctx, const float* xyzr, uint32_t n, float* features, uint8_t* labels) {
+    if (!ctx || !xyzr || !labels) return APO_ERR_INVALID;
+    if (n < 17) return fail(ctx, APO_ERR_INVALID, "the label pass needs at least 17 sites");
+    if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
+    APO_HIP(hipSetDevice(ctx->device));
+    ctx->built = false;
+    apo_status s = ensureCapacity(ctx, n, ctx->cfg.k > 64u ? ctx->cfg.k : 64u);
+    if (s != APO_OK) return s;
+    s = ensureProposalBuffers(ctx, n);
+    if (s != APO_OK) return s;
+    hipStream_t st = ctx->stream;
+    APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, st));
+    s = runPipeline(ctx, ctx->sitesIn, n, true);
+    if (s != APO_OK) return s;
+    std::vector<uint32_t> idx(n);
+    std::vector<uint8_t> lab(n);
+    APO_HIP(hipMemcpy(idx.data(), ctx->idxA, size_t(n) * 4, hipMemcpyDeviceToHost));
+    APO_HIP(hipMemcpy(lab.data(), ctx->labels, n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) labels[idx[i]] = lab[i];
+    if (features) {
+        std::vector<float> f(size_t(n) * kFeatures);
+        APO_HIP(hipMemcpy(f.data(), ctx->feat, f.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) std::memcpy(features + size_t(idx[i]) * kFeatures, &f[size_t(i) * kFeatures], kFeatures * 4);
+    }
+    return APO_OK;
+}
+
+apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t* labels, uint64_t count,
+                              uint32_t epochs, uint32_t batch, float lr, uint32_t seed, const float* init,
+                              float* params, float* epoch_loss) {
+    if (!ctx || !features || !labels || !params || count == 0 || batch == 0) return APO_ERR_INVALID;
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // buried sites carry no label; the rest become classes
+    std::vector<uint32_t> keep;
+    keep.reserve(count);
+    for (uint64_t q = 0; q < count; q++)
+        if (labels[q] != kLabelBuried) keep.push_back(uint32_t(q));
+    if (keep.empty()) return fail(ctx, APO_ERR_INVALID, "no labelled site");
+    const uint32_t m = uint32_t(keep.size());
+    std::vector<float> f(size_t(m) * kFeatures);
+    std::vector<uint8_t> y(m);
+    for (uint32_t q = 0; q < m; q++) {
+        std::memcpy(&f[size_t(q) * kFeatures], features + size_t(keep[q]) * kFeatures, kFeatures * 4);
+        y[q] = uint8_t(proposalClass(labels[keep[q]]));
+    }
+    // parameters: given, or uniform(+-sqrt(6 / fan_in)) from the seed
+    std::vector<float> w(kProposalParams, 0.f);
+    if (init) {
+        std::memcpy(w.data(), init, kProposalParams * 4);
+    } else {
+        std::mt19937 gen(seed);
+        auto fill = [&](size_t off, uint32_t fanIn, uint32_t cnt) {
+            std::uniform_real_distribution<float> u(-std::sqrt(6.f / float(fanIn)), std::sqrt(6.f / float(fanIn)));
+            for (uint32_t q = 0; q < cnt; q++) w[off + q] = u(gen);
+        };
+        size_t off = 0;
+        fill(off, kFeatures, kFeatures * kHidden);
+        off += kFeatures * kHidden + kHidden;
+        fill(off, kHidden, kHidden * kHidden);
+        off += kHidden * kHidden + kHidden;
+        fill(off, kHidden, kHidden * kProposalClasses);
+    }
+    std::vector<float> wq(kProposalParams);
+    for (uint32_t q = 0; q < kProposalParams; q++) {
+        uint32_t b;
+        std::memcpy(&b, &w[q], 4);
+        b = ((b + 0x7fffu + ((b >> 16) & 1u)) >> 16) << 16;
+        std::memcpy(&wq[q], &b, 4);
+    }
+    DeviceTemp<float> dF, dW, dWq, dG, dM, dV, dLoss, dX, dH1, dH2, dZ, dDz, dD2, dD1;
+    DeviceTemp<uint8_t> dY, dYb;
+    DeviceTemp<uint32_t> dOrder;
+    APO_HIP(dF.alloc(f.size()));
+    APO_HIP(dY.alloc(m));
+    APO_HIP(dOrder.alloc(m));
+    APO_HIP(dW.alloc(kProposalParams));
+    APO_HIP(dWq.alloc(kProposalParams));
+    APO_HIP(dG.alloc(kProposalParams));
+    APO_HIP(dM.alloc(kProposalParams));
+    APO_HIP(dV.alloc(kProposalParams));
+    APO_HIP(dLoss.alloc(1));
+    APO_HIP(dX.alloc(size_t(batch) * kFeatures));
+    APO_HIP(dYb.alloc(batch));
+    APO_HIP(dH1.alloc(size_t(batch) * kHidden));
+    APO_HIP(dH2.alloc(size_t(batch) * kHidden));
+    APO_HIP(dZ.alloc(size_t(batch) * kProposalClasses));
+    APO_HIP(dDz.alloc(size_t(batch) * kProposalClasses));
+    APO_HIP(dD2.alloc(size_t(batch) * kHidden));
+    APO_HIP(dD1.alloc(size_t(batch) * kHidden));
+    APO_HIP(hipMemcpyAsync(dF.p, f.data(), f.size() * 4, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dY.p, y.data(), m, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dW.p, w.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dWq.p, wq.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
+    std::vector<float> zeros(kProposalParams, 0.f);
+    APO_HIP(hipMemcpyAsync(dM.p, zeros.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dV.p, zeros.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
+    TrainBuffers tb{dX.p, dYb.p, dH1.p, dH2.p, dZ.p, dDz.p, dD2.p, dD1.p, dW.p, dWq.p, dG.p, dM.p, dV.p, dLoss.p};
+    std::vector<uint32_t> order(m);
+    for (uint32_t q = 0; q < m; q++) order[q] = q;
+    std::mt19937 shuffler(seed ^ 0x9e3779b9u);
+    uint32_t step = 0;
+    for (uint32_t ep = 0; ep < epochs; ep++) {
+        std::shuffle(order.begin(), order.end(), shuffler);
+        APO_HIP(hipMemcpyAsync(dOrder.p, order.data(), size_t(m) * 4, hipMemcpyHostToDevice, st));
+        const float zero = 0.f;
+        APO_HIP(hipMemcpyAsync(dLoss.p, &zero, 4, hipMemcpyHostToDevice, st));
+        for (uint32_t b0 = 0; b0 < m; b0 += batch) {
+            const uint32_t rows = (m - b0) < batch ? (m - b0) : batch;
+            APO_HIP(gatherLaunch(dF.p, dY.p, dOrder.p + b0, rows, dX.p, dYb.p, st));
+            APO_HIP(mlpTrainStep(tb, rows, lr, step++, st));
+        }
+        float loss = 0.f;
+        APO_HIP(hipMemcpyAsync(&loss, dLoss.p, 4, hipMemcpyDeviceToHost, st));
+        APO_HIP(hipStreamSynchronize(st));
+        if (epoch_loss) epoch_loss[ep] = loss / float(m);
+    }
+    APO_HIP(hipMemcpyAsync(params, dW.p, kProposalParams * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+
+apo_status apo_proposal_set(apo_ctx* ctx, const float* params) {
+    if (!ctx) return APO_ERR_INVALID;
+    APO_HIP(hipSetDevice(ctx->device));
+    if (!params) {
+        ctx->proposalOn = false;
+        return APO_OK;
+    }
+    if (!ctx->params) {
+        APO_HIP(alloc(ctx->params, kProposalParams));
+        APO_HIP(alloc(ctx->pw.w1, size_t(kHidden) * kHidden));
+        APO_HIP(alloc(ctx->pw.w2, size_t(kHidden) * kHidden));
+        APO_HIP(alloc(ctx->pw.w3, size_t(16) * kHidden));
+        APO_HIP(alloc(ctx->pw.b1, kHidden));
+        APO_HIP(alloc(ctx->pw.b2, kHidden));
+        APO_HIP(alloc(ctx->pw.b3, 16));
+    }
+    APO_HIP(hipMemcpyAsync(ctx->params, params, kProposalParams * 4, hipMemcpyHostToDevice, ctx->stream));
+    APO_HIP(packWeightsLaunch(ctx->params, ctx->pw, ctx->stream));
+    APO_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->proposalOn = true;
+    return APO_OK;
+}
+
+apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t count, uint8_t* classes, float* logits) {
+    if (!ctx || !features || !classes) return APO_ERR_INVALID;
+    if (!ctx->proposalOn) return fail(ctx, APO_ERR_STATE, "no proposal weights set");
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DeviceTemp<float> dF, dL;
+    DeviceTemp<uint8_t> dC;
+    APO_HIP(dF.alloc(size_t(count) * kFeatures));
+    APO_HIP(dC.alloc(count));
+    if (logits) APO_HIP(dL.alloc(size_t(count) * kProposalClasses));
+    APO_HIP(hipMemcpyAsync(dF.p, features, size_t(count) * kFeatures * 4, hipMemcpyHostToDevice, st));
+    APO_HIP(mlpInferLaunch(dF.p, count, ctx->pw, dC.p, logits ? dL.p : nullptr, ctx->cfg.proposal_cost, st));
+    APO_HIP(hipMemcpyAsync(classes, dC.p, count, hipMemcpyDeviceToHost, st));
+    if (logits) APO_HIP(hipMemcpyAsync(logits, dL.p, size_t(count) * kProposalClasses * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+
+apo_status apo_proposal_classes(apo_ctx* ctx, uint8_t* classes) {
+    if (!ctx || !classes) return APO_ERR_INVALID;
+    if (!ctx->built || !ctx->stats.proposal) return fail(ctx, APO_ERR_STATE, "no build with the proposal");
+    const size_t n = ctx->n;
+    std::vector<uint32_t> idx(n);
+    std::vector<uint8_t> c(n);
+    APO_HIP(hipMemcpy(idx.data(), ctx->idxA, n * 4, hipMemcpyDeviceToHost));
+    APO_HIP(hipMemcpy(c.data(), ctx->cls, n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) classes[idx[i]] = c[i];
     return APO_OK;
 }
 

@@ -119,6 +119,44 @@ struct OutputBuffers {
 hipError_t outputCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st);
 hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices, hipStream_t st);
 
+// Learned neighbour-count proposal (learned.hip)
+constexpr uint32_t kFeatures = 52, kHidden = 64, kProposalClasses = 8;
+constexpr uint32_t kProposalParams = kFeatures * kHidden + kHidden + kHidden * kHidden + kHidden +
+                                     kHidden * kProposalClasses + kProposalClasses;
+constexpr uint32_t kLabelBuried = 255;
+enum CellsMode : uint32_t { kProposal = 0, kLabels = 1 };
+// Neighbours a class asks for; the last class (not certified within 64)
+// runs the plain 16 and goes to validation.
+__host__ __device__ constexpr uint32_t proposalK(uint32_t c) {
+    return c == 0 ? 20u : c == 1 ? 24u : c == 2 ? 28u : c == 3 ? 32u : c == 4 ? 40u : c == 5 ? 48u : c == 6 ? 64u : 16u;
+}
+// Class of a label (neighbours used at certification, 0 = never within 64).
+__host__ __device__ constexpr uint32_t proposalClass(uint32_t label) {
+    if (label == 0) return kProposalClasses - 1;
+    for (uint32_t c = 0; c + 1 < kProposalClasses; c++)
+        if (label <= proposalK(c)) return c;
+    return kProposalClasses - 1;
+}
+struct ProposalWeights {
+    uint16_t *w1, *w2, *w3;  // BF16, transposed [out][64]
+    float *b1, *b2, *b3;
+};
+struct TrainBuffers {
+    float* x;
+    uint8_t* y;
+    float *h1, *h2, *z, *dz, *d2, *d1;
+    float *w, *wq, *grad, *m, *v, *loss;
+};
+hipError_t cellsKLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, const uint8_t* cls, uint8_t* labels,
+                        uint32_t mode, hipStream_t st);
+hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st);
+hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
+                          float cost, hipStream_t st);
+hipError_t packWeightsLaunch(const float* params, const ProposalWeights& w, hipStream_t st);
+hipError_t mlpTrainStep(const TrainBuffers& t, uint32_t rows, float lr, uint32_t step, hipStream_t st);
+hipError_t gatherLaunch(const float* feat, const uint8_t* labels, const uint32_t* order, uint32_t rows, float* x,
+                        uint8_t* y, hipStream_t st);
+
 // Topology (topology.hip)
 hipError_t topologyCountLaunch(const CellCtx& c, uint32_t* counts, hipStream_t st);
 hipError_t topologyWriteLaunch(const CellCtx& c, const uint64_t* offsets, const uint32_t* indices, uint32_t* bis,

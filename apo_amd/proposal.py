@@ -1,0 +1,103 @@
+"""Learned neighbour-count proposal: labels, training and inference.
+
+An MLP (52 -> 64 -> 64 -> 8) predicts for each site how many nearest
+neighbours its cell needs before the plain security-radius test certifies
+it; builds with the proposal run that many updates per cell and send the
+cells the test does not certify through the plain validation.  Labels are
+the reference's certified k (app/benchmarks/benchmark_contributing_nearest.
+cu:75-113), computed by the library's label pass on the GPU; training runs
+in HIP kernels (apo_proposal_train).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from . import _lib
+
+FEATURES = 52
+CLASSES = 8
+PARAMS = 8072
+# neighbours asked for by each class; the last (not certified within 64) runs 16
+CLASS_K = (20, 24, 28, 32, 40, 48, 64, 16)
+LABEL_BURIED = 255
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(_HERE, "models", "proposal_c5.f32")
+
+
+def label_class(labels: np.ndarray) -> np.ndarray:
+    """Class of each label (neighbours used at certification, 0 = never)."""
+    labels = np.asarray(labels)
+    out = np.full(labels.shape, CLASSES - 1, np.uint8)
+    for c in range(CLASSES - 2, -1, -1):
+        out[(labels >= 1) & (labels <= CLASS_K[c])] = c
+    return out
+
+
+def _ctx(k: int = 16):
+    cfg = _lib.Config()
+    _lib.lib().apo_default_config(_lib.byref(cfg))
+    cfg.k = k
+    return _lib.create(cfg)
+
+
+def labels(sites: np.ndarray):
+    """(features (n, 52), labels (n,)) of a site set, in input order."""
+    s = np.ascontiguousarray(np.asarray(sites, np.float32).reshape(-1, 4))
+    n = len(s)
+    feat = np.zeros((n, FEATURES), np.float32)
+    lab = np.zeros(n, np.uint8)
+    ctx = _ctx()
+    try:
+        _lib.check(ctx, _lib.lib().apo_proposal_labels(ctx, _lib.fptr(s), n, _lib.fptr(feat), _lib.u8ptr(lab)))
+    finally:
+        _lib.lib().apo_destroy(ctx)
+    return feat, lab
+
+
+def train(features: np.ndarray, labels_: np.ndarray, epochs: int = 8, batch: int = 2048, lr: float = 3e-3,
+          seed: int = 0, init: np.ndarray | None = None):
+    """Train on the GPU; returns (params (8072,), mean loss per epoch)."""
+    f = np.ascontiguousarray(features, np.float32)
+    y = np.ascontiguousarray(labels_, np.uint8)
+    params = np.zeros(PARAMS, np.float32)
+    loss = np.zeros(epochs, np.float32)
+    ini = None if init is None else np.ascontiguousarray(init, np.float32)
+    ctx = _ctx()
+    try:
+        _lib.check(ctx, _lib.lib().apo_proposal_train(ctx, _lib.fptr(f), _lib.u8ptr(y), len(y), epochs, batch, lr,
+                                                       seed, None if ini is None else _lib.fptr(ini),
+                                                       _lib.fptr(params), _lib.fptr(loss)))
+    finally:
+        _lib.lib().apo_destroy(ctx)
+    return params, loss
+
+
+def infer(params: np.ndarray, features: np.ndarray):
+    """Classes and logits of feature rows through the matrix-core inference."""
+    p = np.ascontiguousarray(params, np.float32)
+    f = np.ascontiguousarray(features, np.float32)
+    cls = np.zeros(len(f), np.uint8)
+    logits = np.zeros((len(f), CLASSES), np.float32)
+    ctx = _ctx()
+    try:
+        L = _lib.lib()
+        _lib.check(ctx, L.apo_proposal_set(ctx, _lib.fptr(p)))
+        _lib.check(ctx, L.apo_proposal_infer(ctx, _lib.fptr(f), len(f), _lib.u8ptr(cls), _lib.fptr(logits)))
+    finally:
+        _lib.lib().apo_destroy(ctx)
+    return cls, logits
+
+
+def load(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
+    p = np.fromfile(path, dtype="<f4")
+    if p.size != PARAMS:
+        raise ValueError(f"{path}: expected {PARAMS} float32 parameters, found {p.size}")
+    return p
+
+
+def save(params: np.ndarray, path: str = DEFAULT_WEIGHTS) -> None:
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.ascontiguousarray(params, "<f4").tofile(path)

@@ -35,7 +35,69 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=60_000, help="sites in the CPU-baseline sub-cube")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
+    ap.add_argument("--proposal-cost", type=float, default=400.0)
     return ap.parse_args()
+
+
+def learned_comparison(sites, dev, n, k, steps, warmup, cost, plain_ids, say):
+    """Train the neighbour-count proposal on C5-style labels (two 1M
+    log-uniform sets, seeds 501/502, untimed), then time the same workload
+    with it.  Cells the plain test does not certify after the proposed count
+    go through the plain validation."""
+    import torch
+    from apo_amd import _lib, datasets, proposal
+    t0 = time.perf_counter()
+    fa, la = proposal.labels(datasets.loguniform_sites(1_000_000, seed=501))
+    fb, lb = proposal.labels(datasets.loguniform_sites(1_000_000, seed=502))
+    t_labels = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    params, loss = proposal.train(np.concatenate([fa, fb]), np.concatenate([la, lb]), epochs=8, batch=4096,
+                                  lr=2e-3, seed=7)
+    t_train = time.perf_counter() - t0
+    say(f"proposal trained on 2M labelled sites: labels {t_labels:.1f} s, training {t_train:.1f} s, "
+        f"loss {loss[0]:.3f} -> {loss[-1]:.3f}")
+    cfg = _lib.Config()
+    _lib.lib().apo_default_config(_lib.byref(cfg))
+    cfg.k = k
+    cfg.proposal_cost = cost
+    ctx = _lib.create(cfg)
+    L = _lib.lib()
+    p32 = np.ascontiguousarray(params, np.float32)
+    _lib.check(ctx, L.apo_proposal_set(ctx, _lib.fptr(p32)))
+    ptr = _lib.ctypes.c_void_p(dev.data_ptr())
+    for _ in range(warmup):
+        _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+    wall = 0.0
+    s = _lib.Stats()
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+    _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+    v = int(s.vertices)
+    xyzw = np.zeros((v, 4), np.float64)
+    ids = np.zeros((v, 4), np.uint32)
+    _lib.check(ctx, L.apo_get_vertices(ctx, _lib.dptr(xyzw), _lib.uptr(ids), v))
+    cnt = _lib.c_uint64()
+    _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
+    learned_ids = {tuple(r) for r in np.sort(ids, axis=1)}
+    L.apo_destroy(ctx)
+    ms = 1000.0 * wall / steps
+    non_buried = n - int(s.buried)
+    return {"cells_per_s": n / (ms / 1000.0), "ms_per_step": ms, "proposal_cost": cost,
+            "cells_certified_by_cell_pass": int(s.cells_certified),
+            "fallback_rate": 1.0 - int(s.cells_certified) / max(non_buried, 1),
+            "stage_ms": {"knn": s.ms_knn, "proposal": s.ms_proposal, "cells": s.ms_cells,
+                         "validation": s.ms_validation, "bisector": s.ms_bisector},
+            "vertices": v, "vertices_only_in_plain": len(plain_ids - learned_ids),
+            "vertices_only_in_learned": len(learned_ids - plain_ids),
+            "empty_sphere_violations": int(cnt.value),
+            "training": {"labelled_sites": int(len(la) + len(lb)), "epochs": 8,
+                         "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
+                         "label_s": t_labels, "train_s": t_train}}
 
 
 def cpu_baseline(sites: np.ndarray, target: int):
@@ -169,7 +231,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{n} sites, centres U[0,1]^3, radii log-uniform [1e-4,1e-2], seed 3, plain k={args.k}",
-                   "path": "plain", "sites": n, "k": args.k},
+                   "path": "plain (the learned-k proposal is reported under learned)", "sites": n, "k": args.k},
         "vertices_per_s": vertices / (ms_per_step / 1000.0),
         "vertices": vertices,
         "stage_ms": stage,
@@ -183,6 +245,15 @@ def main():
                      "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
     }
+    if not args.no_learned:
+        xyzw_p = np.zeros((vertices, 4), np.float64)
+        ids_p = np.zeros((vertices, 4), np.uint32)
+        _lib.check(ctx, L.apo_get_vertices(ctx, _lib.dptr(xyzw_p), _lib.uptr(ids_p), vertices))
+        plain_ids = {tuple(r) for r in np.sort(ids_p, axis=1)}
+        say("learned neighbour-count proposal: training, then the same workload")
+        lc = learned_comparison(sites, dev, n, args.k, args.steps, args.warmup, args.proposal_cost, plain_ids, say)
+        lc["speedup_vs_plain"] = lc["cells_per_s"] / out["value"]
+        out["learned"] = lc
     if not args.no_cpu_baseline:
         say(f"CPU baseline: oracle on a sub-cube of about {args.cpu_sample} sites")
         out["cpu_baseline"] = cpu_baseline(sites, args.cpu_sample)

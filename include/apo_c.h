@@ -40,6 +40,9 @@ typedef struct apo_config {
                                  bit3 stop after the kNN pass */
     uint32_t waves_per_cu;    /* persistent-grid occupancy of the cell kernels, 0 = auto */
     uint32_t device;          /* HIP device ordinal */
+    float proposal_cost;      /* learned proposal: cost of validating one cell, in neighbour updates;
+                                 a site gets the class c minimising k_c + cost * P(not certified at k_c).
+                                 0 = the most probable class */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,
@@ -70,6 +73,11 @@ typedef struct apo_stats {
     uint64_t bytes_vertex_validation;
     uint64_t bytes_edge_validation;
     uint64_t bytes_bisector_validation;
+    /* learned proposal: features + MLP time, whether the build used it, and
+     * the cells certified by the cell pass (before any validation) */
+    double ms_proposal;
+    uint64_t proposal;
+    uint64_t cells_certified;
     uint64_t reserved[2];
 } apo_stats;
 
@@ -125,6 +133,37 @@ apo_status apo_get_topology_sets(apo_ctx* ctx, uint32_t* bisectors, uint64_t bis
  * entries (may be NULL); NULL record arrays only fill the offsets. */
 apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_t* vertices, uint64_t vertex_cap,
                                  uint64_t* edge_offsets, uint32_t* edges, uint64_t edge_cap);
+
+/* ---- learned neighbour-count proposal.  No counterpart in the reference's
+ * construction: an MLP (52 -> 64 -> 64 -> 8, BF16 weights, FP32
+ * accumulation on the matrix cores) predicts per site how many nearest
+ * neighbours the cell pass runs before the plain certification test
+ * (algorithm.inl:197-201); cells the test does not certify go through the
+ * plain validation.  Classes ask for 20, 24, 28, 32, 40, 48, 64
+ * neighbours; class 7 (not certified within 64) runs 16. ---- */
+#define APO_PROPOSAL_FEATURES 52
+#define APO_PROPOSAL_CLASSES 8
+#define APO_PROPOSAL_PARAMS 8072 /* W1[52][64] b1[64] W2[64][64] b2[64] W3[64][8] b3[8], weights [in][out] */
+
+/* Label pass (app/benchmarks/benchmark_contributing_nearest.cu:75-113): for
+ * every site, the number of neighbours in kNN order after which the cell is
+ * first certified, testing after every update (1..64; 0: not within 64;
+ * 255: buried), plus its 52 features; both in input order. */
+apo_status apo_proposal_labels(apo_ctx* ctx, const float* xyzr, uint32_t n, float* features, uint8_t* labels);
+/* Train the MLP on (features, labels) from apo_proposal_labels with HIP
+ * kernels (FP32, BF16-rounded forward weights, Adam, softmax cross-entropy
+ * over the 8 classes).  init: starting parameters or NULL (seeded uniform);
+ * params receives APO_PROPOSAL_PARAMS floats, epoch_loss (may be NULL) the
+ * mean loss of each epoch. */
+apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t* labels, uint64_t count,
+                              uint32_t epochs, uint32_t batch, float lr, uint32_t seed, const float* init,
+                              float* params, float* epoch_loss);
+/* Use these parameters in the following builds (NULL: the plain path). */
+apo_status apo_proposal_set(apo_ctx* ctx, const float* params);
+/* Classes (and optionally the 8 logits) the set MLP gives to count feature rows. */
+apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t count, uint8_t* classes, float* logits);
+/* Classes of the last build's sites, in input order. */
+apo_status apo_proposal_classes(apo_ctx* ctx, uint8_t* classes);
 
 /* ---- introspection of the device state (the public members dSites,
  * bvh, dStatus, dVertices, dEdges of AlgorithmGPU, algorithm.cuh:147-155) ---- */

@@ -72,6 +72,8 @@ def lib():
         L.orc_default_config.argtypes = [ctypes.POINTER(OrcConfig)]
         L.orc_build.restype = ctypes.c_void_p
         L.orc_build.argtypes = [f32p, ctypes.c_uint32, ctypes.POINTER(OrcConfig)]
+        L.orc_build_proposal.restype = ctypes.c_void_p
+        L.orc_build_proposal.argtypes = [f32p, ctypes.c_uint32, ctypes.POINTER(OrcConfig), u8p]
         L.orc_free.restype = None
         L.orc_free.argtypes = [ctypes.c_void_p]
         L.orc_vertex_count.restype = ctypes.c_uint64
@@ -216,11 +218,18 @@ def default_config(**kw) -> OrcConfig:
 class Diagram:
     """Result of the oracle's full construction."""
 
-    def __init__(self, xyzr, **cfg):
+    def __init__(self, xyzr, classes=None, **cfg):
+        """classes: per-site proposal classes (input order) for the learned path."""
         self.sites = _sites(xyzr)
         self.n = len(self.sites)
         self.cfg = default_config(**cfg)
-        self._h = lib().orc_build(_p(self.sites, f32p), self.n, ctypes.byref(self.cfg))
+        if classes is None:
+            self._h = lib().orc_build(_p(self.sites, f32p), self.n, ctypes.byref(self.cfg))
+        else:
+            self._classes = np.ascontiguousarray(classes, np.uint8)
+            assert len(self._classes) == self.n
+            self._h = lib().orc_build_proposal(_p(self.sites, f32p), self.n, ctypes.byref(self.cfg),
+                                               _p(self._classes, u8p))
         if not self._h:
             raise ValueError("oracle build needs n >= 2")
 

@@ -629,7 +629,15 @@ struct Context {
     std::vector<float> knownRadii;
     std::vector<Cell> cells;
     std::vector<uint32_t> flags;
+    std::vector<uint32_t> siteK;  // per-site neighbour count of the proposal path (empty: all K)
 };
+
+// Neighbours asked for by each class of the learned proposal (mirrors the
+// product's mapping; class 7, not certified within 64, runs 16).
+inline uint32_t proposalK(uint32_t c) {
+    static const uint32_t k[8] = {20, 24, 28, 32, 40, 48, 64, 16};
+    return c < 8 ? k[c] : 64;
+}
 
 inline D4 vertexPosition(const Context& C, const D4& si, const Vtx& v) {
     D4 x[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -1249,7 +1257,8 @@ void stageCells(Context& C, const std::vector<uint32_t>& knn, uint32_t K) {
         const D4 si = C.S[i];
         double knownRadius = -DBL_MAX;
         uint32_t flags = 0;
-        for (uint32_t nId = 0; nId < K; nId++) {
+        const uint32_t use = C.siteK.empty() ? K : std::min(K, C.siteK[i]);
+        for (uint32_t nId = 0; nId < use; nId++) {
             const uint32_t j = knn[size_t(i) * K + nId];
             if (j == kNone) break;
             if (C.buried[j]) continue;
@@ -1657,7 +1666,7 @@ void orc_default_config(orc_config* cfg) {
     cfg->edge_bound = 0;
 }
 
-orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfgIn) {
+static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {
     if (n < 2) return nullptr;
     orc_config cfg;
     if (cfgIn) cfg = *cfgIn;
@@ -1689,6 +1698,12 @@ orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfgIn) {
     C.knownRadii.assign(n, -FLT_MAX);
     C.cells.resize(n);
     C.flags.assign(n, 0);
+    if (classes) {
+        C.siteK.resize(n);
+        for (uint32_t i = 0; i < n; i++) C.siteK[i] = proposalK(classes[d->bvh.indices[i]]);
+        cfg.k = 64;
+        C.cfg.k = 64;
+    }
     const uint32_t K = cfg.k;
 
     // getKnns, iteration 0 (algorithm.inl:32-99)
@@ -1769,6 +1784,17 @@ orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfgIn) {
     }
     d->stats[4] = d->outIds.size() / 4;
     return d;
+}
+
+orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfgIn) {
+    return buildDiagram(xyzr, n, cfgIn, nullptr);
+}
+
+// The learned-proposal path with given classes (input order): kNN with 64
+// neighbours, each cell runs its class's neighbour count, then the plain
+// certification and validation.
+orc_diagram* orc_build_proposal(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {
+    return buildDiagram(xyzr, n, cfgIn, classes);
 }
 
 void orc_free(orc_diagram* d) { delete d; }

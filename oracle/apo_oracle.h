@@ -84,6 +84,10 @@ void orc_default_config(orc_config* cfg);
 typedef struct orc_diagram orc_diagram;
 
 orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfg);
+/* The learned-proposal path given per-site classes (input order): 64-nearest
+ * lists, each cell runs the neighbour count of its class (20, 24, 28, 32,
+ * 40, 48, 64; class 7: 16) before certification and validation. */
+orc_diagram* orc_build_proposal(const float* xyzr, uint32_t n, const orc_config* cfg, const uint8_t* classes);
 void         orc_free(orc_diagram* d);
 
 /* Emitted Apollonius vertices (apo toHost semantics). */

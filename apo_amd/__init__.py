@@ -6,6 +6,7 @@ This mirrors the construction API of PlathC/apo
     AlgorithmGPU(sites, k=16).build()           -> AlgorithmGPU::build
     AlgorithmGPU.to_host()                      -> AlgorithmGPU::toHost -> VertexDiagram
     compute_apollonius_diagram(sites)           -> apo::gpu::computeApolloniusDiagram
+    AlgorithmGPU(sites, offsets=o).build()      -> one build of many independent molecules
 
 Everything runs in the HIP library apo_amd/lib/libapo_hip.so through its C
 ABI (include/apo_c.h).  There is no CPU fallback: if the library is missing
@@ -32,12 +33,18 @@ class VertexDiagram:
 
 
 class AlgorithmGPU:
-    """One construction context on one GPU (AlgorithmGPU<K>, algorithm.cuh:12-46)."""
+    """One construction context on one GPU (AlgorithmGPU<K>, algorithm.cuh:12-46).
+
+    offsets: for a batch, the molecule boundaries (m + 1 values, molecule i is
+    sites[offsets[i]:offsets[i+1]]), built in one pass of offset-indexed
+    LBVHs and persistent work-queue kernels.  graph: capture the device build
+    in a HIP graph and replay it on later builds of the same shape."""
 
     def __init__(self, sites, k: int = 16, max_iterations: int = 10, max_restarts: int = 100,
                  stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0, proposal=None,
-                 proposal_cost: float = 0.0):
+                 proposal_cost: float = 0.0, offsets=None, graph: bool = False):
         self.sites = np.ascontiguousarray(np.asarray(sites, dtype=np.float32).reshape(-1, 4))
+        self.offsets = None if offsets is None else np.ascontiguousarray(offsets, np.uint32)
         self.k = int(k)
         cfg = _lib.Config()
         _lib.lib().apo_default_config(_lib.byref(cfg))
@@ -48,6 +55,7 @@ class AlgorithmGPU:
         cfg.waves_per_cu = waves_per_cu
         cfg.device = device
         cfg.proposal_cost = proposal_cost
+        cfg.graph = 1 if graph else 0
         self._ctx = _lib.create(cfg)
         self.built = False
         if proposal is not None:
@@ -64,9 +72,26 @@ class AlgorithmGPU:
 
     # -- AlgorithmGPU::build ------------------------------------------------
     def build(self):
-        _lib.check(self._ctx, _lib.lib().apo_build(self._ctx, _lib.fptr(self.sites), len(self.sites)))
+        if self.offsets is None:
+            st = _lib.lib().apo_build(self._ctx, _lib.fptr(self.sites), len(self.sites))
+        else:
+            if len(self.offsets) < 2 or int(self.offsets[-1]) != len(self.sites):
+                raise ValueError("offsets must end at the number of sites")
+            st = _lib.lib().apo_build_batched(self._ctx, _lib.fptr(self.sites), _lib.uptr(self.offsets),
+                                              len(self.offsets) - 1)
+        _lib.check(self._ctx, st)
         self.built = True
         return self
+
+    @property
+    def molecules(self) -> int:
+        return 1 if self.offsets is None else len(self.offsets) - 1
+
+    def batch_vertex_offsets(self) -> np.ndarray:
+        """First output vertex of each molecule (molecules + 1 values)."""
+        out = np.zeros(self.molecules + 1, np.uint64)
+        _lib.check(self._ctx, _lib.lib().apo_batch_vertex_offsets(self._ctx, _lib.u64ptr(out)))
+        return out
 
     # -- AlgorithmGPU::toHost -----------------------------------------------
     def to_host(self) -> VertexDiagram:
@@ -108,7 +133,7 @@ class AlgorithmGPU:
 
     def lbvh(self) -> dict:
         n = len(self.sites)
-        nodes = 2 * n - 1
+        nodes = 2 * n - self.molecules
         out = dict(codes=np.zeros(n, np.uint64), parent=np.zeros(nodes, np.int32),
                    left=np.zeros(nodes, np.int32), right=np.zeros(nodes, np.int32),
                    aabb=np.zeros((nodes, 6), np.float32), sizes=np.zeros(nodes, np.uint32),

@@ -51,6 +51,7 @@ def build(verbose: bool = True) -> str:
     if res.returncode != 0:
         raise RuntimeError("link failed:\n" + res.stderr)
     os.replace(tmp, LIB)
+    build_samples()
     _build_example()
     if verbose:
         print(f"built {LIB}")
@@ -58,6 +59,19 @@ def build(verbose: bool = True) -> str:
 
 
 EXAMPLE = os.path.join(LIBDIR, "apo_diagram")
+SAMPLES = os.path.join(LIBDIR, "libapo_samples.so")
+
+
+def build_samples() -> str:
+    """The host library of the seeded molecule generator (csrc/samples.cpp)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-fPIC", "-shared",
+           os.path.join(CSRC, "samples.cpp"), "-o", SAMPLES + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("samples build failed:\n" + res.stderr)
+    os.replace(SAMPLES + ".tmp", SAMPLES)
+    return SAMPLES
 
 
 def _build_example() -> None:

@@ -25,15 +25,18 @@ using namespace apo_hip;
 
 namespace {
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t count, uint32_t value) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = value;
-}
-
-template <typename T>
-struct DevBuf {
-    T* ptr = nullptr;
-    size_t count = 0;
+// What a captured build graph depends on besides buffer addresses (a change
+// of any buffer drops the graph).
+struct GraphKey {
+    const void* sites = nullptr;
+    uint32_t n = 0, molecules = 0, k = 0, stageMask = 0, maxIterations = 0, maxRestarts = 0, wavesPerCu = 0;
+    bool learned = false;
+    uint64_t outCap = 0;
+    bool operator==(const GraphKey& o) const {
+        return sites == o.sites && n == o.n && molecules == o.molecules && k == o.k && stageMask == o.stageMask &&
+               maxIterations == o.maxIterations && maxRestarts == o.maxRestarts && wavesPerCu == o.wavesPerCu &&
+               learned == o.learned && outCap == o.outCap;
+    }
 };
 
 }  // namespace
@@ -48,7 +51,21 @@ struct apo_ctx {
     uint32_t cap = 0;      // site capacity of the buffers below
     uint32_t capK = 0;
     uint32_t n = 0;        // sites of the last build
+    uint32_t molecules = 1;  // molecules of the last build
     bool built = false;
+
+    // batches: site offsets of the molecules and the vertex offsets of each
+    // molecule's output (device), molecule capacity of these and of `scene`
+    uint32_t capM = 0;
+    uint32_t* offsets = nullptr;
+    uint64_t* moleculeVertices = nullptr;
+    // blocks per CU of the persistent kernels at full occupancy
+    uint32_t occCells = 1, occVertex = 1, occEdge = 1, occBisector = 1;
+    // the captured build (config.graph) and what it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    GraphKey graphKey{};
+    std::vector<uint32_t> roundTasks;  // task count of each validation round of the last build
 
     // input staging
     F4* sitesIn = nullptr;
@@ -143,22 +160,36 @@ struct DeviceTemp {
     }
 };
 
-uint32_t tilesFor(uint32_t n) { return (n + 4095u) / 4096u; }
+void dropGraph(apo_ctx* ctx) {
+    if (ctx->graphExec) (void)hipGraphExecDestroy(ctx->graphExec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graphExec = nullptr;
+    ctx->graph = nullptr;
+}
+
+apo_status ensureMolecules(apo_ctx* ctx, uint32_t molecules) {
+    if (molecules <= ctx->capM) return APO_OK;
+    dropGraph(ctx);
+    APO_HIP(alloc(ctx->scene, molecules));
+    APO_HIP(alloc(ctx->offsets, size_t(molecules) + 1));
+    APO_HIP(alloc(ctx->moleculeVertices, size_t(molecules) + 1));
+    ctx->capM = molecules;
+    return APO_OK;
+}
 
 apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     if (n <= ctx->cap && K <= ctx->capK) return APO_OK;
+    dropGraph(ctx);
     const uint32_t cap = n;
     const size_t nodes = 2 * size_t(cap) - 1;
     APO_HIP(alloc(ctx->bytes, size_t(kByteFamilies) * kByteSlots));
     APO_HIP(alloc(ctx->sitesIn, cap));
-    APO_HIP(alloc(ctx->scene, 1));
     APO_HIP(alloc(ctx->boxPartials, size_t(kBoxBlocks) * 8));
     APO_HIP(alloc(ctx->boxTicket, 1));
     // the ticket is self-resetting; clear it once, ordered on the build stream
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->boxTicket, 1u, 0u);
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, ctx->stream, ctx->boxTicket, 1u, 0u);
     APO_HIP(hipGetLastError());
-    APO_HIP(alloc(ctx->sortScratch,
-                  size_t(kRadixPasses) * 256 + kRadixPasses + size_t(kRadixPasses) * tilesFor(cap) * 256 + 1));
+    APO_HIP(alloc(ctx->sortScratch, sortScratchWords(cap)));
     APO_HIP(alloc(ctx->range, 2 * size_t(cap)));
     APO_HIP(alloc(ctx->crossList, cap));
     APO_HIP(alloc(ctx->codesA, cap));
@@ -179,7 +210,7 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     APO_HIP(alloc(ctx->edg, size_t(cap) * kEdgeCap));
     APO_HIP(alloc(ctx->enb, cap));
     APO_HIP(alloc(ctx->knn, size_t(cap) * K));
-    APO_HIP(alloc(ctx->counters, 64));
+    APO_HIP(alloc(ctx->counters, kCounterWords));
     APO_HIP(alloc(ctx->err, 1));
     APO_HIP(alloc(ctx->tasks, cap));
     APO_HIP(alloc(ctx->cellCount, cap));
@@ -206,6 +237,9 @@ apo_status ensureProposalBuffers(apo_ctx* ctx, uint32_t n) {
 CellCtx cellCtx(apo_ctx* ctx) {
     CellCtx c;
     c.n = ctx->n;
+    c.internal = ctx->n - ctx->molecules;
+    c.molecules = ctx->molecules;
+    c.offsets = ctx->molecules > 1u ? ctx->offsets : nullptr;
     c.sorted = ctx->sorted;
     c.scene = ctx->scene;
     c.nodes = ctx->nodes;
@@ -221,8 +255,10 @@ CellCtx cellCtx(apo_ctx* ctx) {
     return c;
 }
 
-uint32_t persistentWaves(apo_ctx* ctx) {
-    const uint32_t perCu = ctx->cfg.waves_per_cu ? ctx->cfg.waves_per_cu : 12u;
+// Blocks of a persistent launch: the CUs times the waves each holds at full
+// occupancy, or times config.waves_per_cu when that is set.
+uint32_t persistentBlocks(apo_ctx* ctx, uint32_t occupancy) {
+    const uint32_t perCu = ctx->cfg.waves_per_cu ? ctx->cfg.waves_per_cu : occupancy;
     return uint32_t(ctx->cus) * perCu;
 }
 
@@ -255,129 +291,153 @@ apo_status stageCheck(apo_ctx* ctx, const char* stage) {
     return APO_OK;
 }
 
-#define APO_STAGE(name)                              \
-    do {                                             \
-        apo_status s_ = stageCheck(ctx, name);       \
-        if (s_ != APO_OK) return s_;                 \
-    } while (0)
+LbvhBuffers lbvhBuffers(apo_ctx* ctx) {
+    return LbvhBuffers{ctx->scene,  ctx->boxPartials, ctx->boxTicket,    ctx->sortScratch, ctx->codesA,
+                       ctx->codesB, ctx->idxA,        ctx->idxB,         ctx->sorted,      ctx->nodes,
+                       ctx->parent, ctx->sizes,       ctx->refitCounter, ctx->rootBox,     ctx->err,
+                       ctx->range,  ctx->crossList};
+}
 
-// labelPass: LBVH, kNN with 64 neighbours, features and the label cell pass
-// (the test after every update), then stop.
-apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool labelPass = false) {
+Segments segments(apo_ctx* ctx) { return Segments{ctx->molecules, ctx->molecules > 1u ? ctx->offsets : nullptr}; }
+
+// Clears the guard word, the byte counters and the build counters.
+void resetCounters(apo_ctx* ctx, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
+    constexpr uint32_t byteWords = 2u * kByteFamilies * kByteSlots;
+    hipLaunchKernelGGL(k_fill_words, dim3((byteWords + 63u) / 64u), dim3(64), 0, st,
+                       reinterpret_cast<uint32_t*>(ctx->bytes), byteWords, 0u);
+    hipLaunchKernelGGL(k_fill_words, dim3((kCounterWords + 63u) / 64u), dim3(64), 0, st, ctx->counters,
+                       kCounterWords, 0u);
+}
+
+// Every device step of a build in stream order, with no host round trip:
+// what config.graph captures.  With `timed`, events mark the stages and the
+// validation rounds, and APO_SYNC_STAGES checks each stage.
+apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learned, bool timed) {
     hipStream_t st = ctx->stream;
-    ctx->n = n;
-    LbvhBuffers lb{ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA, ctx->codesB,
-                   ctx->idxA,  ctx->idxB,        ctx->sorted,    ctx->nodes,       ctx->parent, ctx->sizes,
-                   ctx->refitCounter, ctx->rootBox, ctx->err, ctx->range, ctx->crossList};
     const CellCtx cc = cellCtx(ctx);
     const uint32_t K = ctx->cfg.k;
-    const uint32_t waves = persistentWaves(ctx);
-
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
-    constexpr uint32_t byteWords = 2u * kByteFamilies * kByteSlots;
-    hipLaunchKernelGGL(k_fill_u32, dim3((byteWords + 63u) / 64u), dim3(64), 0, st,
-                       reinterpret_cast<uint32_t*>(ctx->bytes), byteWords, 0u);
-    ctx->vt.rounds = 0;
-    ctx->vt.totalTasks = 0;
-    APO_HIP(hipEventRecord(ctx->ev[0], st));
-    APO_HIP(lbvhBuild(lb, sitesDev, n, st));
-    APO_STAGE("lbvh");
-    APO_HIP(hipEventRecord(ctx->ev[1], st));
-    const bool learned = ctx->proposalOn && !labelPass;
-    const uint32_t Kk = (learned || labelPass) ? 64u : K;
+    auto mark = [&](int e) { return timed ? hipEventRecord(ctx->ev[e], st) : hipSuccess; };
+#define APO_TIMED_STAGE(name)                            \
+    do {                                                 \
+        if (timed) {                                     \
+            apo_status s_ = stageCheck(ctx, name);       \
+            if (s_ != APO_OK) return s_;                 \
+        }                                                \
+    } while (0)
+    resetCounters(ctx, st);
+    APO_HIP(mark(0));
+    APO_HIP(lbvhBuild(lbvhBuffers(ctx), sitesDev, n, segments(ctx), st));
+    APO_TIMED_STAGE("lbvh");
+    APO_HIP(mark(1));
+    const uint32_t Kk = learned ? 64u : K;
     APO_HIP(knnLaunch(cc, Kk, ctx->knn, st));
-    APO_STAGE("knn");
-    APO_HIP(hipEventRecord(ctx->ev[2], st));
-    if (labelPass) {
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->counters, 64u, 0u);
-        APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st));
-        APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, nullptr, ctx->labels, kLabels, st));
-        uint32_t guard = 0;
-        APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        APO_HIP(hipStreamSynchronize(st));
-        if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped: " + guardText(guard));
-        ctx->built = false;  // cells stopped at certification: not a diagram
-        return APO_OK;
-    }
-    if (ctx->cfg.stage_mask & 8u) {
-        // stop after the kNN pass (introspection of the LBVH and kNN state)
-        APO_HIP(hipStreamSynchronize(st));
-        ctx->vertexCount = 0;
-        ctx->built = true;
-        return APO_OK;
-    }
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->counters, 64u, 0u);
+    APO_TIMED_STAGE("knn");
+    APO_HIP(mark(2));
+    if (ctx->cfg.stage_mask & 8u) return APO_OK;  // stop after the kNN pass
     if (learned && (ctx->cfg.stage_mask & 1u)) {
         // features of the 16 nearest, proposal classes, then each cell with its
         // proposed neighbour count and the plain certification test
         APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st));
         APO_HIP(mlpInferLaunch(ctx->feat, n, ctx->pw, ctx->cls, nullptr, ctx->cfg.proposal_cost, st));
-        APO_HIP(hipEventRecord(ctx->ev[7], st));
+        APO_HIP(mark(7));
         APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, ctx->cls, nullptr, kProposal, st));
     } else {
-        APO_HIP(hipEventRecord(ctx->ev[7], st));
-        if (ctx->cfg.stage_mask & 1u) APO_HIP(cellsLaunch(cc, ctx->knn, K, ctx->counters, waves, st));
-        else APO_HIP(cellsLaunch(cc, ctx->knn, 0, ctx->counters, waves, st));
+        APO_HIP(mark(7));
+        APO_HIP(cellsLaunch(cc, ctx->knn, (ctx->cfg.stage_mask & 1u) ? K : 0u, ctx->counters + kCellQueue,
+                            persistentBlocks(ctx, ctx->occCells), st));
     }
-    APO_STAGE("cells");
-    APO_HIP(hipEventRecord(ctx->ev[3], st));
-    ValidationBuffers vb{ctx->tasks, ctx->counters + 1};
+    APO_TIMED_STAGE("cells");
+    APO_HIP(mark(3));
+    const ValidationBuffers vb{ctx->tasks, ctx->counters};
+    const ValidationGrid vg{persistentBlocks(ctx, ctx->occVertex), persistentBlocks(ctx, ctx->occEdge),
+                            persistentBlocks(ctx, ctx->occBisector)};
     if (ctx->cfg.stage_mask & 2u)
-        APO_HIP(validationLaunch(cc, vb, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st, &ctx->vt));
-    APO_STAGE("vertex/edge validation");
-    APO_HIP(hipEventRecord(ctx->ev[4], st));
-    if (ctx->cfg.stage_mask & 4u) APO_HIP(validationLaunch(cc, vb, 1, ctx->cfg.max_restarts, true, st, nullptr));
-    APO_STAGE("bisector validation");
-    APO_HIP(hipEventRecord(ctx->ev[5], st));
-    OutputBuffers ob{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
+        APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st,
+                                 timed ? &ctx->vt : nullptr));
+    APO_TIMED_STAGE("vertex/edge validation");
+    APO_HIP(mark(4));
+    if (ctx->cfg.stage_mask & 4u)
+        APO_HIP(validationLaunch(cc, vb, vg, 1, ctx->cfg.max_restarts, true, st, nullptr));
+    APO_TIMED_STAGE("bisector validation");
+    APO_HIP(mark(5));
+    const OutputBuffers ob{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
     APO_HIP(outputCountLaunch(cc, ob, st));
     APO_HIP(outputWriteLaunch(cc, ob, ctx->idxA, st));
-    APO_STAGE("output");
-    APO_HIP(hipEventRecord(ctx->ev[6], st));
+    if (ctx->molecules > 1u) APO_HIP(moleculeVerticesLaunch(cc, ob, ctx->moleculeVertices, st));
+    APO_TIMED_STAGE("output");
+    APO_HIP(mark(6));
+#undef APO_TIMED_STAGE
+    return APO_OK;
+}
+
+// After the device work: vertex count, guard word, output growth and the
+// statistics.  `timed` builds carry per-stage times; a graph replay only
+// its total (events 0 and 6 around the launch).
+apo_status finishBuild(apo_ctx* ctx, uint32_t n, bool learned, bool timed) {
+    hipStream_t st = ctx->stream;
     uint32_t total = 0, guard = 0;
+    std::vector<uint32_t> counters(kCounterWords);
     APO_HIP(hipMemcpyAsync(&total, ctx->total, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(counters.data(), ctx->counters, kCounterWords * 4, hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
     if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped: " + guardText(guard));
+    if (ctx->cfg.stage_mask & 8u) {
+        ctx->vertexCount = 0;
+        ctx->built = true;
+        return APO_OK;
+    }
     if (total > ctx->outCap) {
         // more vertices than the estimate: grow the output and write again
+        dropGraph(ctx);
         ctx->outCap = uint64_t(total) + (uint64_t(total) >> 3) + 1024;
         APO_HIP(alloc(ctx->outXyzw, ctx->outCap));
         APO_HIP(alloc(ctx->outIds, ctx->outCap));
-        OutputBuffers ob2{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
-        APO_HIP(outputWriteLaunch(cc, ob2, ctx->idxA, st));
+        const OutputBuffers ob2{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
+        APO_HIP(outputWriteLaunch(cellCtx(ctx), ob2, ctx->idxA, st));
         APO_HIP(hipStreamSynchronize(st));
     }
     ctx->vertexCount = total;
-    float ms[7] = {};
-    for (int s = 0; s < 6; s++) APO_HIP(hipEventElapsedTime(&ms[s], ctx->ev[s], ctx->ev[s + 1]));
-    float all = 0;
-    APO_HIP(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[6]));
     apo_stats& S = ctx->stats;
     std::memset(&S, 0, sizeof(S));
-    float msProposal = 0.f, msCellsOnly = 0.f;
-    APO_HIP(hipEventElapsedTime(&msProposal, ctx->ev[2], ctx->ev[7]));
-    APO_HIP(hipEventElapsedTime(&msCellsOnly, ctx->ev[7], ctx->ev[3]));
+    float all = 0;
+    APO_HIP(hipEventElapsedTime(&all, ctx->ev[0], ctx->ev[6]));
     S.ms_total = all;
-    S.ms_lbvh = ms[0];
-    S.ms_knn = ms[1];
-    S.ms_cells = msCellsOnly;
-    S.ms_proposal = msProposal;
+    if (timed) {
+        float ms[6] = {};
+        for (int q = 0; q < 6; q++) APO_HIP(hipEventElapsedTime(&ms[q], ctx->ev[q], ctx->ev[q + 1]));
+        float msProposal = 0.f, msCellsOnly = 0.f;
+        APO_HIP(hipEventElapsedTime(&msProposal, ctx->ev[2], ctx->ev[7]));
+        APO_HIP(hipEventElapsedTime(&msCellsOnly, ctx->ev[7], ctx->ev[3]));
+        S.ms_lbvh = ms[0];
+        S.ms_knn = ms[1];
+        S.ms_cells = msCellsOnly;
+        S.ms_proposal = msProposal;
+        S.ms_validation = ms[3];
+        S.ms_bisector = ms[4];
+        S.ms_output = ms[5];
+        for (uint32_t r = 0; r < ctx->vt.rounds; r++) {
+            float a = 0.f, b = 0.f;
+            APO_HIP(hipEventElapsedTime(&a, ctx->vt.ev[r][0], ctx->vt.ev[r][1]));
+            APO_HIP(hipEventElapsedTime(&b, ctx->vt.ev[r][1], ctx->vt.ev[r][2]));
+            S.ms_vertex_validation_kernel += a;
+            S.ms_edge_validation_kernel += b;
+        }
+    }
     S.proposal = learned ? 1u : 0u;
-    S.ms_validation = ms[3];
-    S.ms_bisector = ms[4];
-    S.ms_output = ms[5];
     S.sites = n;
     S.vertices = total;
-    for (uint32_t r = 0; r < ctx->vt.rounds; r++) {
-        float a = 0.f, b = 0.f;
-        APO_HIP(hipEventElapsedTime(&a, ctx->vt.ev[r][0], ctx->vt.ev[r][1]));
-        APO_HIP(hipEventElapsedTime(&b, ctx->vt.ev[r][1], ctx->vt.ev[r][2]));
-        S.ms_vertex_validation_kernel += a;
-        S.ms_edge_validation_kernel += b;
+    // rounds that had task cells
+    ctx->roundTasks.clear();
+    const uint32_t rounds = (ctx->cfg.stage_mask & 2u) ? ctx->cfg.max_iterations : 0u;
+    for (uint32_t r = 0; r < rounds; r++) {
+        const uint32_t t = counters[kRoundTasks + r];
+        if (t == 0) break;
+        ctx->roundTasks.push_back(t);
+        S.validation_tasks += t;
     }
-    S.validation_launches = ctx->vt.rounds;
-    S.validation_tasks = ctx->vt.totalTasks;
+    S.validation_launches = uint32_t(ctx->roundTasks.size());
     unsigned long long counts[kByteFamilies * kByteSlots] = {};
     APO_HIP(hipMemcpy(counts, ctx->bytes, sizeof(counts), hipMemcpyDeviceToHost));
     uint64_t fam[kByteFamilies] = {};
@@ -388,6 +448,69 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool labelP
     S.bytes_edge_validation = fam[kBytesEdgeVal];
     S.bytes_bisector_validation = fam[kBytesBisectorVal];
     ctx->built = true;
+    return APO_OK;
+}
+
+// One build of the sites at sitesDev (molecule offsets already on the
+// device): eager with stage events, or as a replay of the captured graph.
+apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
+    hipStream_t st = ctx->stream;
+    ctx->n = n;
+    const bool learned = ctx->proposalOn;
+    const bool graph = ctx->cfg.graph != 0u && !(ctx->cfg.stage_mask & 8u);
+    if (!graph) {
+        apo_status s = enqueueBuild(ctx, sitesDev, n, learned, true);
+        if (s != APO_OK) return s;
+        return finishBuild(ctx, n, learned, true);
+    }
+    GraphKey key;
+    key.sites = sitesDev;
+    key.n = n;
+    key.molecules = ctx->molecules;
+    key.k = ctx->cfg.k;
+    key.stageMask = ctx->cfg.stage_mask;
+    key.maxIterations = ctx->cfg.max_iterations;
+    key.maxRestarts = ctx->cfg.max_restarts;
+    key.wavesPerCu = ctx->cfg.waves_per_cu;
+    key.learned = learned;
+    key.outCap = ctx->outCap;
+    if (!ctx->graphExec || !(key == ctx->graphKey)) {
+        dropGraph(ctx);
+        APO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const apo_status s = enqueueBuild(ctx, sitesDev, n, learned, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (s != APO_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return s;
+        }
+        APO_HIP(e);
+        ctx->graph = g;
+        APO_HIP(hipGraphInstantiate(&ctx->graphExec, g, nullptr, nullptr, 0));
+        ctx->graphKey = key;
+    }
+    APO_HIP(hipEventRecord(ctx->ev[0], st));
+    APO_HIP(hipGraphLaunch(ctx->graphExec, st));
+    APO_HIP(hipEventRecord(ctx->ev[6], st));
+    return finishBuild(ctx, n, learned, false);
+}
+
+// Label pass of the learned proposal: LBVH, kNN with 64 neighbours,
+// features and the cell pass that tests after every update, then stop.
+apo_status runLabelPass(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
+    hipStream_t st = ctx->stream;
+    ctx->n = n;
+    const CellCtx cc = cellCtx(ctx);
+    resetCounters(ctx, st);
+    APO_HIP(lbvhBuild(lbvhBuffers(ctx), sitesDev, n, segments(ctx), st));
+    APO_HIP(knnLaunch(cc, 64u, ctx->knn, st));
+    APO_HIP(featuresLaunch(cc, ctx->knn, 64u, ctx->feat, st));
+    APO_HIP(cellsKLaunch(cc, ctx->knn, 64u, nullptr, ctx->labels, kLabels, st));
+    uint32_t guard = 0;
+    APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped: " + guardText(guard));
+    ctx->built = false;  // cells stopped at certification: not a diagram
     return APO_OK;
 }
 
@@ -404,6 +527,7 @@ void apo_default_config(apo_config* cfg) {
     cfg->waves_per_cu = 0;
     cfg->device = 0;
     cfg->proposal_cost = 0.f;
+    cfg->graph = 0;
 }
 
 apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
@@ -413,7 +537,7 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     if (cfg) ctx->cfg = *cfg;
     else apo_default_config(&ctx->cfg);
     const uint32_t k = ctx->cfg.k;
-    if (!(k == 16 || k == 24 || k == 32 || k == 48 || k == 64)) {
+    if (!(k == 16 || k == 24 || k == 32 || k == 48 || k == 64) || ctx->cfg.max_iterations > kMaxRounds) {
         delete ctx;
         return APO_ERR_INVALID;
     }
@@ -421,6 +545,8 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    if (e == hipSuccess) e = cellsOccupancy(&ctx->occCells);
+    if (e == hipSuccess) e = validationOccupancy(&ctx->occVertex, &ctx->occEdge, &ctx->occBisector);
     for (int s = 0; s < 8 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
     for (uint32_t r = 0; r < kTimedRounds && e == hipSuccess; r++)
         for (int q = 0; q < 3 && e == hipSuccess; q++) e = hipEventCreate(&ctx->vt.ev[r][q]);
@@ -436,7 +562,8 @@ void apo_destroy(apo_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {ctx->sitesIn, ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA,
+    dropGraph(ctx);
+    void* ptrs[] = {ctx->offsets, ctx->moleculeVertices, ctx->sitesIn, ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA,
                     ctx->codesB, ctx->idxA, ctx->idxB, ctx->sorted, ctx->nodes, ctx->parent, ctx->sizes,
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
@@ -456,8 +583,27 @@ void apo_destroy(apo_ctx* ctx) {
 
 const char* apo_last_error(const apo_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
 
-apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n) {
-    if (!ctx || !xyzr_device) return APO_ERR_INVALID;
+}  // extern "C"
+
+namespace {
+
+// Checks a batch's site offsets and stores them on the device.
+apo_status setMolecules(apo_ctx* ctx, const uint32_t* offsets, uint32_t molecules, uint32_t n) {
+    if (molecules == 0 || molecules > kMaxMolecules) return fail(ctx, APO_ERR_INVALID, "molecule count out of range");
+    if (offsets[0] != 0u || offsets[molecules] != n) return fail(ctx, APO_ERR_INVALID, "offsets must run from 0 to the site count");
+    for (uint32_t m = 0; m < molecules; m++)
+        if (offsets[m + 1] < offsets[m] || offsets[m + 1] - offsets[m] < 2u)
+            return fail(ctx, APO_ERR_INVALID, "every molecule needs at least 2 sites");
+    apo_status s = ensureMolecules(ctx, molecules);
+    if (s != APO_OK) return s;
+    ctx->molecules = molecules;
+    if (molecules > 1u)
+        APO_HIP(hipMemcpyAsync(ctx->offsets, offsets, (size_t(molecules) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    return APO_OK;
+}
+
+// Shared front of every build entry: argument checks, buffers, molecules.
+apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint32_t molecules) {
     if (n < 2) return fail(ctx, APO_ERR_INVALID, "need at least 2 sites");
     if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
     APO_HIP(hipSetDevice(ctx->device));
@@ -468,23 +614,56 @@ apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n) 
         s = ensureProposalBuffers(ctx, n);
         if (s != APO_OK) return s;
     }
+    const uint32_t one[2] = {0u, n};
+    return setMolecules(ctx, offsets ? offsets : one, offsets ? molecules : 1u, n);
+}
+
+}  // namespace
+
+extern "C" {
+
+apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n) {
+    if (!ctx || !xyzr_device) return APO_ERR_INVALID;
+    apo_status s = prepareBuild(ctx, n, nullptr, 1);
+    if (s != APO_OK) return s;
     return runPipeline(ctx, reinterpret_cast<const F4*>(xyzr_device), n);
 }
 
 apo_status apo_build(apo_ctx* ctx, const float* xyzr, uint32_t n) {
     if (!ctx || !xyzr) return APO_ERR_INVALID;
-    if (n < 2) return fail(ctx, APO_ERR_INVALID, "need at least 2 sites");
-    if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
-    APO_HIP(hipSetDevice(ctx->device));
-    ctx->built = false;
-    apo_status s = ensureCapacity(ctx, n, ctx->proposalOn ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
+    apo_status s = prepareBuild(ctx, n, nullptr, 1);
     if (s != APO_OK) return s;
-    if (ctx->proposalOn) {
-        s = ensureProposalBuffers(ctx, n);
-        if (s != APO_OK) return s;
-    }
     APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, ctx->stream));
     return runPipeline(ctx, ctx->sitesIn, n);
+}
+
+apo_status apo_build_batched_device(apo_ctx* ctx, const float* xyzr_device, const uint32_t* offsets,
+                                    uint32_t molecules) {
+    if (!ctx || !xyzr_device || !offsets) return APO_ERR_INVALID;
+    apo_status s = prepareBuild(ctx, offsets[molecules], offsets, molecules);
+    if (s != APO_OK) return s;
+    return runPipeline(ctx, reinterpret_cast<const F4*>(xyzr_device), offsets[molecules]);
+}
+
+apo_status apo_build_batched(apo_ctx* ctx, const float* xyzr, const uint32_t* offsets, uint32_t molecules) {
+    if (!ctx || !xyzr || !offsets) return APO_ERR_INVALID;
+    const uint32_t n = offsets[molecules];
+    apo_status s = prepareBuild(ctx, n, offsets, molecules);
+    if (s != APO_OK) return s;
+    APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, ctx->stream));
+    return runPipeline(ctx, ctx->sitesIn, n);
+}
+
+apo_status apo_batch_vertex_offsets(apo_ctx* ctx, uint64_t* offsets) {
+    if (!ctx || !offsets) return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    if (ctx->molecules == 1u) {
+        offsets[0] = 0;
+        offsets[1] = ctx->vertexCount;
+        return APO_OK;
+    }
+    APO_HIP(hipMemcpy(offsets, ctx->moleculeVertices, (size_t(ctx->molecules) + 1) * 8, hipMemcpyDeviceToHost));
+    return APO_OK;
 }
 
 apo_status apo_vertex_count(apo_ctx* ctx, uint64_t* count) {
@@ -536,7 +715,7 @@ apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats) {
     ctx->stats.flagged_cells = flagged;
     ctx->stats.vertex_records = vr;
     ctx->stats.edge_records = er;
-    const uint64_t firstRound = ctx->vt.rounds ? ctx->vt.tasks[0] : 0u;
+    const uint64_t firstRound = ctx->roundTasks.empty() ? 0u : ctx->roundTasks[0];
     ctx->stats.cells_certified = n >= buried + firstRound ? n - buried - firstRound : 0u;
     *stats = ctx->stats;
     return APO_OK;
@@ -564,9 +743,8 @@ apo_status apo_check_empty_spheres(apo_ctx* ctx, const double* xyzw, const uint3
     unsigned long long* dv = nullptr;
     APO_HIP(hipMalloc(reinterpret_cast<void**>(&dv), sizeof(unsigned long long)));
     APO_HIP(hipMemsetAsync(dv, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
-    APO_HIP(checkEmptyLaunch(ctx->sorted, ctx->idxA, ctx->cellCount, ctx->nodes, ctx->n, dx, di, count, tol, dv,
-                             ctx->err, st));
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
+    APO_HIP(checkEmptyLaunch(cellCtx(ctx), ctx->idxA, ctx->cellCount, dx, di, count, tol, dv, st));
     unsigned long long bad = 0;
     uint32_t guard = 0;
     APO_HIP(hipMemcpyAsync(&bad, dv, sizeof(bad), hipMemcpyDeviceToHost, st));
@@ -593,20 +771,24 @@ apo_status apo_get_lbvh(apo_ctx* ctx, uint64_t* sorted_codes, int32_t* parent, i
     if (!ctx) return APO_ERR_INVALID;
     if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
     const uint32_t n = ctx->n;
-    const size_t nodes = 2 * size_t(n) - 1;
-    if (sorted_codes) APO_HIP(hipMemcpy(sorted_codes, ctx->codesA, size_t(n) * 8, hipMemcpyDeviceToHost));
-    if (parent) APO_HIP(hipMemcpy(parent, ctx->parent, nodes * 4, hipMemcpyDeviceToHost));
+    const uint32_t internal = n - ctx->molecules;
+    const size_t nodes = size_t(n) + internal;
+    if (sorted_codes) {
+        // the key's bits 60-63 carry the molecule index of a batch
+        APO_HIP(hipMemcpy(sorted_codes, ctx->codesA, size_t(n) * 8, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) sorted_codes[i] &= (uint64_t(1) << 60) - 1u;
+    }
+    std::vector<int32_t> par(nodes);
+    APO_HIP(hipMemcpy(par.data(), ctx->parent, nodes * 4, hipMemcpyDeviceToHost));
+    if (parent) std::memcpy(parent, par.data(), nodes * 4);
     if (sizes) APO_HIP(hipMemcpy(sizes, ctx->sizes, nodes * 4, hipMemcpyDeviceToHost));
-    std::vector<Node> nd(n - 1);
-    APO_HIP(hipMemcpy(nd.data(), ctx->nodes, size_t(n - 1) * sizeof(Node), hipMemcpyDeviceToHost));
-    float root[8];
-    APO_HIP(hipMemcpy(root, ctx->rootBox, sizeof(root), hipMemcpyDeviceToHost));
-    std::vector<F4> srt(n);
-    APO_HIP(hipMemcpy(srt.data(), ctx->sorted, size_t(n) * sizeof(F4), hipMemcpyDeviceToHost));
-    // unpack the child-in-parent records into per-node arrays
+    std::vector<Node> nd(internal);
+    APO_HIP(hipMemcpy(nd.data(), ctx->nodes, size_t(internal) * sizeof(Node), hipMemcpyDeviceToHost));
+    // unpack the child-in-parent records into per-node arrays; a root's box
+    // is the union of its two child slots
     std::vector<float> box(nodes * 6, 0.f), mr(nodes, 0.f);
     std::vector<int32_t> L(nodes, 0), R(nodes, 0);
-    for (uint32_t i = 0; i < n - 1; i++) {
+    for (uint32_t i = 0; i < internal; i++) {
         const float* f = reinterpret_cast<const float*>(&nd[i]);
         const int32_t lc = nd[i].d.x, rc = nd[i].d.y;
         L[i] = lc;
@@ -617,12 +799,17 @@ apo_status apo_get_lbvh(apo_ctx* ctx, uint64_t* sorted_codes, int32_t* parent, i
         }
         mr[lc] = f[14];
         mr[rc] = f[15];
+        if (par[i] < 0) {
+            for (int c = 0; c < 3; c++) {
+                box[size_t(i) * 6 + c] = std::fmin(f[c], f[6 + c]);
+                box[size_t(i) * 6 + 3 + c] = std::fmax(f[3 + c], f[9 + c]);
+            }
+            mr[i] = std::fmax(f[14], f[15]);
+        }
     }
-    for (int c = 0; c < 6; c++) box[c] = root[c];
-    mr[0] = root[6];
     for (uint32_t i = 0; i < n; i++) {
-        L[n - 1 + i] = int32_t(i);
-        R[n - 1 + i] = int32_t(i) + 1;
+        L[internal + i] = int32_t(i);
+        R[internal + i] = int32_t(i) + 1;
     }
     if (left) std::memcpy(left, L.data(), nodes * 4);
     if (right) std::memcpy(right, R.data(), nodes * 4);
@@ -788,7 +975,10 @@ apo_status apo_proposal_labels(apo_ctx* ctx, const float* xyzr, uint32_t n, floa
     if (s != APO_OK) return s;
     hipStream_t st = ctx->stream;
     APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, st));
-    s = runPipeline(ctx, ctx->sitesIn, n, true);
+    s = ensureMolecules(ctx, 1);
+    if (s != APO_OK) return s;
+    ctx->molecules = 1;
+    s = runLabelPass(ctx, ctx->sitesIn, n);
     if (s != APO_OK) return s;
     std::vector<uint32_t> idx(n);
     std::vector<uint8_t> lab(n);

@@ -15,7 +15,12 @@
 
 namespace apo_hip {
 
+// Radix passes of the Morton sort: 8 over the 64-bit key (Morton code in
+// bits 0-59, the molecule's low 4 bits in 60-63), then up to 2 over the
+// molecule index's higher bits for batches of more than 16 molecules.
 constexpr int kRadixPasses = 8;
+constexpr int kMaxRadixPasses = 10;
+constexpr uint32_t kMaxMolecules = 1u << 20;
 constexpr uint32_t kBoxBlocks = 512;
 
 // Per-cell capacities (apo/include/apo/gpu/algorithm.inl:25-30).
@@ -36,11 +41,35 @@ struct Scene {
     D4 art[6];
 };
 
+// Sort passes for a batch of m molecules.
+inline int radixPassesFor(uint32_t m) { return m <= 16u ? 8 : m <= 4096u ? 9 : 10; }
+
+// Molecule of a site index: the largest m with offsets[m] <= i.  Sites of a
+// molecule are contiguous both in input order and in sorted order.
+__host__ __device__ __forceinline__ uint32_t moleculeOf(const uint32_t* offsets, uint32_t molecules, uint32_t i) {
+    uint32_t lo = 0, hi = molecules;  // offsets[lo] <= i < offsets[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Segments of a build: one molecule (offsets == nullptr) or a batch.  The
+// nodes of molecule m are internal [offsets[m] - m, offsets[m+1] - m - 1)
+// (its root first) and leaves internal + [offsets[m], offsets[m+1]), with
+// internal = n - molecules; one molecule is the reference's numbering.
+struct Segments {
+    uint32_t molecules;
+    const uint32_t* offsets;  // molecules + 1 (device), nullptr for one molecule
+};
+
 struct LbvhBuffers {
-    Scene* scene;
+    Scene* scene;             // one per molecule
     double* boxPartials;   // kBoxBlocks * 8
     uint32_t* boxTicket;   // 1
-    uint32_t* sortScratch; // hist [8*256] + tile counters [8] + look-back [8*tiles*256]
+    uint32_t* sortScratch; // hist [P*256] + tile counters [P] + look-back [P*tiles*256] + 1
     uint64_t* codesA;
     uint64_t* codesB;
     uint32_t* idxA;        // sorted position -> original index (final)
@@ -59,13 +88,17 @@ struct LbvhBuffers {
 // Leaves per block of the first refit phase (lbvh.hip).
 constexpr uint32_t kRefitLeaves = 2048;
 
-hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, hipStream_t st);
+size_t sortScratchWords(uint32_t n);
+hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const Segments& seg, hipStream_t st);
 
 // Device view of the construction state.
 struct CellCtx {
-    uint32_t n;
+    uint32_t n;             // real sites of all molecules: ids [0, n); artificial ids n + 6m + [0, 6)
+    uint32_t internal;      // internal nodes (n - molecules); leaf j is node internal + j
+    uint32_t molecules;
+    const uint32_t* offsets;  // molecules + 1 site offsets (nullptr for one molecule)
     const F4* sorted;
-    const Scene* scene;     // artificial sites in scene->art
+    const Scene* scene;     // per molecule; artificial sites in scene[m].art
     const Node* nodes;
     uint8_t* status;
     float* knownRadii;
@@ -78,6 +111,17 @@ struct CellCtx {
     unsigned long long* bytes;  // algorithmic HBM bytes per kernel family, kByteSlots counters each
 };
 
+__device__ __forceinline__ uint32_t nodeCount(const CellCtx& c) { return c.n + c.internal; }
+__device__ __forceinline__ uint32_t cellMolecule(const CellCtx& c, uint32_t i) {
+    return c.molecules == 1u ? 0u : moleculeOf(c.offsets, c.molecules, i);
+}
+// root of molecule m's tree
+__device__ __forceinline__ uint32_t rootNode(const CellCtx& c, uint32_t m) {
+    return m == 0u ? 0u : c.offsets[m] - m;
+}
+// id of molecule m's first artificial site
+__device__ __forceinline__ uint32_t artificialBase(const CellCtx& c, uint32_t m) { return c.n + 6u * m; }
+
 // Byte counters: family * kByteSlots + (block & (kByteSlots - 1)), summed on the host.
 constexpr uint32_t kByteSlots = 32;
 enum ByteFamily : uint32_t { kBytesCells = 0, kBytesVertexVal = 1, kBytesEdgeVal = 2, kBytesBisectorVal = 3,
@@ -87,25 +131,76 @@ __device__ __forceinline__ void countBytes(const CellCtx& c, uint32_t family, ui
 }
 
 hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st);
-hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t waves,
+hipError_t cellsOccupancy(uint32_t* blocksPerCu);
+// Persistent cell pass over all sites; queue = kXcdQueueWords heads.
+hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
                        hipStream_t st);
+
+// Device counters of a build (words of one buffer): work-queue heads of the
+// persistent kernels (8 per queue, one range per XCD), the task count of
+// the current validation round and the task count of every round.
+constexpr uint32_t kXcdQueueWords = 8;
+constexpr uint32_t kCellQueue = 0;
+constexpr uint32_t kTaskCount = 8;
+constexpr uint32_t kVertexQueue = 16;
+constexpr uint32_t kEdgeQueue = 24;
+constexpr uint32_t kRoundWords = kEdgeQueue + kXcdQueueWords - kTaskCount;  // cleared per round
+constexpr uint32_t kBisectorQueue = 32;
+constexpr uint32_t kRoundTasks = 64;
+constexpr uint32_t kMaxRounds = 256;
+constexpr uint32_t kCounterWords = kRoundTasks + kMaxRounds;
+
+static __global__ void k_fill_words(uint32_t* p, uint32_t count, uint32_t value) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+// Work queue over [0, count) for a persistent grid: 8 contiguous ranges,
+// range x served first to the blocks of XCD x (the dispatcher hands blocks
+// to the 8 XCDs round-robin), so each XCD's L2 works through one spatially
+// coherent run of cells; a wave whose range is drained takes work from the
+// following ranges.  Returns the same index to every lane, kNone when all
+// ranges are drained.
+__device__ __forceinline__ uint32_t popWork(uint32_t* heads, uint32_t count) {
+    constexpr uint32_t kXcd = kXcdQueueWords;
+    uint32_t r = kNone;
+    if (laneId() == 0) {
+        const uint32_t home = blockIdx.x % kXcd;
+        const uint32_t base = count / kXcd, extra = count % kXcd;
+        for (uint32_t k = 0; k < kXcd; k++) {
+            const uint32_t x = (home + k) % kXcd;
+            const uint32_t len = base + (x < extra ? 1u : 0u);
+            if (__hip_atomic_load(&heads[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
+            const uint32_t t = atomicAdd(&heads[x], 1u);
+            if (t < len) {
+                r = x * base + (x < extra ? x : extra) + t;
+                break;
+            }
+        }
+    }
+    return __builtin_amdgcn_readfirstlane(r);
+}
 
 // Validation (algorithm.inl:208-1233).  Task lists live on the device.
 struct ValidationBuffers {
     uint32_t* tasks;       // n
-    uint32_t* counters;    // [0] task count, [1..] queue heads
+    uint32_t* counters;    // kCounterWords
+};
+// Persistent grid sizes (blocks of one wave).
+struct ValidationGrid {
+    uint32_t vertex, edge, bisector;
 };
 // Events around each vertex / edge validation launch (rounds beyond
-// kTimedRounds run untimed) and the task count of every round.
+// kTimedRounds run untimed).
 constexpr uint32_t kTimedRounds = 16;
 struct ValidationTiming {
     hipEvent_t ev[kTimedRounds][3];  // before vertex, between, after edge
-    uint32_t tasks[kTimedRounds];
-    uint32_t rounds;                 // rounds that launched kernels
-    uint64_t totalTasks;
+    uint32_t rounds;                 // rounds with events
 };
-hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t maxRestarts,
-                            bool bisector, hipStream_t st, ValidationTiming* tm);
+hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks);
+hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const ValidationGrid& g,
+                            uint32_t iterations, uint32_t maxRestarts, bool bisector, hipStream_t st,
+                            ValidationTiming* tm);
 
 // Output (algorithm.inl:1235-1307): count, scan, write.
 struct OutputBuffers {
@@ -118,6 +213,8 @@ struct OutputBuffers {
 };
 hipError_t outputCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st);
 hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices, hipStream_t st);
+// Vertex offsets of each molecule of a batch (molecules + 1 values).
+hipError_t moleculeVerticesLaunch(const CellCtx& c, const OutputBuffers& o, uint64_t* out, hipStream_t st);
 
 // Learned neighbour-count proposal (learned.hip)
 constexpr uint32_t kFeatures = 52, kHidden = 64, kProposalClasses = 8;
@@ -164,8 +261,8 @@ hipError_t topologyWriteLaunch(const CellCtx& c, const uint64_t* offsets, const 
 hipError_t cellRecordsLaunch(const CellCtx& c, const uint32_t* indices, const uint64_t* vOff, const uint64_t* eOff,
                              uint4* vOut, uint4* eOut, hipStream_t st);
 
-hipError_t checkEmptyLaunch(const F4* sorted, const uint32_t* indices, uint32_t* rank, const Node* nodes, uint32_t n,
-                            const D4* xyzw, const uint4* ids, uint64_t count, double tol,
-                            unsigned long long* violations, uint32_t* err, hipStream_t st);
+hipError_t checkEmptyLaunch(const CellCtx& c, const uint32_t* indices, uint32_t* rank, const D4* xyzw,
+                            const uint4* ids, uint64_t count, double tol, unsigned long long* violations,
+                            hipStream_t st);
 
 }  // namespace apo_hip

@@ -31,7 +31,8 @@ struct WaveCell {
 
 __device__ __forceinline__ D4 loadSite(const CellCtx& c, uint32_t id) {
     if (id < c.n) return toD(c.sorted[id]);
-    if (id - c.n < 6u) return c.scene->art[id - c.n];
+    const uint32_t a = id - c.n;
+    if (a < 6u * c.molecules) return c.scene[a / 6u].art[a % 6u];
     atomicOr(c.err, kErrCellSite);
     return D4{0, 0, 0, 0};
 }
@@ -363,6 +364,7 @@ __device__ inline double cellRadius(const CellCtx& c, const CellLds& L, const Wa
 }
 
 // The initial cell: octahedron of the 6 artificial sites (algorithm.inl:1472-1499)
+// of the cell's molecule, whose ids start at n.
 __device__ __forceinline__ void initCell(CellLds& L, WaveCell& w, uint32_t n) {
     const uint32_t lane = laneId();
     const uint8_t av[8][3] = {{0, 2, 4}, {0, 2, 5}, {0, 3, 4}, {0, 3, 5}, {1, 2, 4}, {1, 2, 5}, {1, 3, 4}, {1, 3, 5}};

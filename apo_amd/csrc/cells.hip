@@ -21,28 +21,14 @@
 
 namespace apo_hip {
 
-// Block b -> cell index so that each of the 8 XCDs (which receive blocks
-// round-robin) works through one contiguous Morton range of cells and its
-// L2 sees spatially coherent sites.
-__device__ __forceinline__ uint32_t xcdCell(uint32_t b, uint32_t n) {
-    constexpr uint32_t kXcd = 8;
-    const uint32_t xcd = b % kXcd, slot = b / kXcd;
-    const uint32_t base = n / kXcd, extra = n % kXcd;
-    const uint32_t start = xcd * base + (xcd < extra ? xcd : extra);
-    return start + slot;
-}
-
-// getCells (algorithm.inl:101-206): K updates in kNN order, then the
-// security-radius certification.  One wave per cell, one exit: the kernel
-// body has no loop around its cross-lane operations that could be split
-// per lane.
-__global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K) {
-    __shared__ CellLds L;
+// getCells (algorithm.inl:101-206) for cell i: K updates in kNN order, then
+// the security-radius certification.
+__device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uint32_t* __restrict__ knn, uint32_t K,
+                                         uint32_t i) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
-    const uint32_t i = xcdCell(blockIdx.x, n);
     WaveCell w;
-    initCell(L, w, n);
+    initCell(L, w, artificialBase(c, cellMolecule(c, i)));
     const uint8_t st = c.status[i];
     const bool active = !(st == kStatusCertified || st == kStatusBuried || K == 0);
     float kr = -FLT_MAX;
@@ -89,12 +75,31 @@ __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restr
     }
 }
 
-hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t waves,
+// Persistent cell pass: each wave takes cells from the XCD-ranged queue
+// (Morton order within each range) until every cell is done.
+__global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
+                                              uint32_t* queue) {
+    __shared__ CellLds L;
+    for (;;) {
+        const uint32_t i = popWork(queue, c.n);
+        if (i == kNone) break;
+        cellTask(c, L, knn, K, i);
+        waveSync();
+    }
+}
+
+hipError_t cellsOccupancy(uint32_t* blocksPerCu) {
+    int b = 0;
+    const hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cells, 64, 0);
+    *blocksPerCu = uint32_t(b > 0 ? b : 1);
+    return r;
+}
+
+hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
                        hipStream_t st) {
-    (void)queue;
-    (void)waves;
     if (k > 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cells, dim3(c.n), dim3(64), 0, st, c, knn, k);
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, queue, kXcdQueueWords, 0u);
+    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue);
     return hipGetLastError();
 }
 
@@ -180,6 +185,31 @@ hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uin
     const uint32_t blocks = (c.n + 255u) / 256u;
     hipLaunchKernelGGL(k_out_write, dim3(blocks), dim3(256), 0, st, c, o.cellCount, o.blockSums, indices, o.xyzw,
                        o.ids, o.capacity);
+    return hipGetLastError();
+}
+
+// Vertex offsets of the molecules of a batch: the exclusive prefix of the
+// per-cell vertex counts at each molecule's first cell (after the scan,
+// blockSums holds the exclusive prefix of each 256-cell block).
+__global__ __launch_bounds__(64) void k_molecule_vertices(CellCtx c, const uint32_t* __restrict__ cellCount,
+                                                          const uint32_t* __restrict__ blockOffsets,
+                                                          const uint32_t* __restrict__ total, uint64_t* out) {
+    const uint32_t m = blockIdx.x;
+    if (m == c.molecules) {
+        if (laneId() == 0) out[m] = *total;
+        return;
+    }
+    const uint32_t first = c.offsets[m];
+    const uint32_t b0 = first & ~255u;
+    uint32_t part = 0;
+    for (uint32_t i = b0 + laneId(); i < first; i += 64) part += cellCount[i];
+    const uint32_t sum = waveSumU(part);
+    if (laneId() == 0) out[m] = uint64_t(blockOffsets[first / 256u]) + sum;
+}
+
+hipError_t moleculeVerticesLaunch(const CellCtx& c, const OutputBuffers& o, uint64_t* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_molecule_vertices, dim3(c.molecules + 1), dim3(64), 0, st, c, o.cellCount, o.blockSums,
+                       o.total, out);
     return hipGetLastError();
 }
 

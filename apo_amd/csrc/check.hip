@@ -16,11 +16,16 @@ __device__ __forceinline__ double weightedDistance(const D4& s, const D3& p) {
     return __builtin_sqrt(dx * dx + dy * dy + dz * dz) - s.w;
 }
 
-__global__ __launch_bounds__(256) void k_check_empty(const F4* __restrict__ sorted, const uint32_t* __restrict__ indices,
-                                                     const uint32_t* __restrict__ rank, const Node* __restrict__ nodes,
-                                                     uint32_t n, const D4* __restrict__ xyzw,
+// Vertices of a batch are checked against the sites of their own molecule
+// (the molecule of their first tangent site), through that molecule's tree.
+__global__ __launch_bounds__(256) void k_check_empty(CellCtx c, const uint32_t* __restrict__ indices,
+                                                     const uint32_t* __restrict__ rank, const D4* __restrict__ xyzw,
                                                      const uint4* __restrict__ ids, uint64_t count, double tol,
-                                                     unsigned long long* violations, uint32_t* err) {
+                                                     unsigned long long* violations) {
+    const F4* __restrict__ sorted = c.sorted;
+    const Node* __restrict__ nodes = c.nodes;
+    const uint32_t n = c.n;
+    uint32_t* err = c.err;
     const uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= count) return;
     const D4 v = xyzw[q];
@@ -44,10 +49,11 @@ __global__ __launch_bounds__(256) void k_check_empty(const F4* __restrict__ sort
     const float fbound = float(bound);
     const F4 pt{float(p.x), float(p.y), float(p.z), 0.f};
     uint32_t stack[64];
-    uint32_t sp = 0, node = 0;
+    uint32_t sp = 0, node = rootNode(c, cellMolecule(c, rank[o[0]]));
+    const uint32_t internal = c.internal, total = nodeCount(c);
     bool bad = false;
     while (!bad) {
-        if (node < n - 1) {
+        if (node < internal) {
             const Node nn = nodes[node];
             Box lb, rb;
             nodeBoxes(nn, lb, rb);
@@ -65,8 +71,8 @@ __global__ __launch_bounds__(256) void k_check_empty(const F4* __restrict__ sort
                 node = goL ? L : R;
                 continue;
             }
-        } else if (node < 2 * n - 1) {
-            const uint32_t j = node - (n - 1);
+        } else if (node < total) {
+            const uint32_t j = node - internal;
             const uint32_t orig = indices[j];
             if (orig != id.x && orig != id.y && orig != id.z && orig != id.w)
                 bad = weightedDistance(toD(sorted[j]), p) < bound;
@@ -82,14 +88,14 @@ __global__ void k_rank(const uint32_t* __restrict__ indices, uint32_t n, uint32_
     if (j < n) rank[indices[j]] = j;
 }
 
-hipError_t checkEmptyLaunch(const F4* sorted, const uint32_t* indices, uint32_t* rank, const Node* nodes, uint32_t n,
-                            const D4* xyzw, const uint4* ids, uint64_t count, double tol,
-                            unsigned long long* violations, uint32_t* err, hipStream_t st) {
-    hipLaunchKernelGGL(k_rank, dim3((n + 255) / 256), dim3(256), 0, st, indices, n, rank);
+hipError_t checkEmptyLaunch(const CellCtx& c, const uint32_t* indices, uint32_t* rank, const D4* xyzw,
+                            const uint4* ids, uint64_t count, double tol, unsigned long long* violations,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(k_rank, dim3((c.n + 255) / 256), dim3(256), 0, st, indices, c.n, rank);
     if (count == 0) return hipGetLastError();
     const uint64_t blocks = (count + 255) / 256;
-    hipLaunchKernelGGL(k_check_empty, dim3(uint32_t(blocks)), dim3(256), 0, st, sorted, indices, rank, nodes, n, xyzw,
-                       ids, count, tol, violations, err);
+    hipLaunchKernelGGL(k_check_empty, dim3(uint32_t(blocks)), dim3(256), 0, st, c, indices, rank, xyzw, ids, count,
+                       tol, violations);
     return hipGetLastError();
 }
 

@@ -38,16 +38,17 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
         const F4 s = c.sorted[i];
         const float startingDistance = -FLT_MAX;
         uint32_t sp = 0;
-        uint32_t node = 0;
+        uint32_t node = rootNode(c, cellMolecule(c, i));
+        const uint32_t internal = c.internal, nodes = nodeCount(c);
         while (true) {
-            while (node < n - 1) {
+            while (node < internal) {
                 const Node nd = c.nodes[node];
                 Box lb, rb;
                 nodeBoxes(nd, lb, rb);
                 const uint32_t L = uint32_t(nd.d.x), R = uint32_t(nd.d.y);
-                if (L >= 2 * n - 1 || R >= 2 * n - 1) {
+                if (L >= nodes || R >= nodes) {
                     atomicOr(c.err, kErrKnnChild);
-                    node = 2 * n;  // neither internal nor leaf: ends this descent
+                    node = nodes;  // neither internal nor leaf: ends this descent
                     break;
                 }
                 const float dL = boxDistance(lb, s), dLmax = boxFarthestDistance(lb, s);
@@ -74,8 +75,8 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
                     }
                 }
             }
-            if (node >= n - 1 && node < 2 * n - 1) {
-                const uint32_t j = node - (n - 1);
+            if (node >= internal && node < nodes) {
+                const uint32_t j = node - internal;
                 const float d = sphereDistF(s, c.sorted[j]);
                 if (j != i && d < hVal[0][lane] && d > startingDistance) {
                     // replace the root and sift down (heap.hpp:11-31)

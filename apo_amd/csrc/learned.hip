@@ -32,7 +32,7 @@ __global__ __launch_bounds__(64) void k_cells_k(CellCtx c, const uint32_t* __res
     const uint32_t n = c.n;
     const uint32_t i = blockIdx.x;
     WaveCell w;
-    initCell(L, w, n);
+    initCell(L, w, artificialBase(c, cellMolecule(c, i)));
     const uint8_t st = c.status[i];
     const bool active = !(st == kStatusCertified || st == kStatusBuried);
     float kr = -FLT_MAX;

@@ -4,7 +4,9 @@
 // 478-857 (globalEdgeValidation), 859-1233 (globalBisectorValidation) and the
 // iteration loop of AlgorithmGPU::build, :1551-1593.
 //
-// One wave per task cell (the grid is the task list, no persistent loop).
+// One wave per task cell: a persistent grid of waves takes the round's
+// task cells (compacted on the device) from a work queue, so a round needs
+// no task count on the host and the whole build can be captured in a graph.
 // Each lane runs its own BVH query (vertex / edge / bisector) with its stack
 // in LDS ([slot][lane]); the contributors it finds are applied to the cell
 // in ascending lane order with the wave-wide updateCell of cells.hip.  Steps
@@ -98,7 +100,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
     const uint32_t node = t.node;
-    if (node < n - 1) {
+    if (node < c.internal) {
         const Node nn = c.nodes[node];
         Box lb, rb;
         nodeBoxes(nn, lb, rb);
@@ -106,7 +108,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
         bool ef = false, ec = false;
         uint32_t farChild = 0, closeChild = 0;
         float minFar = 0.f;
-        if (L0 >= 2 * n - 1 || R0 >= 2 * n - 1) {
+        if (L0 >= nodeCount(c) || R0 >= nodeCount(c)) {
             atomicOr(c.err, kErrKnnChild);
         } else {
             const float dL = boxDistance(lb, cur), dR = boxDistance(rb, cur);
@@ -131,8 +133,8 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
             }
             return;
         }
-    } else if (node < 2 * n - 1) {
-        const uint32_t j = node - (n - 1);
+    } else if (node < nodeCount(c)) {
+        const uint32_t j = node - c.internal;
         if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried) {
             const float d = sphereDistF(c.sorted[j], cur);
             if (d < nd && !knownLane(L, F, j)) {
@@ -152,12 +154,9 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
     if (!more) t.live = false;
 }
 
-__global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint32_t* __restrict__ tasks,
-                                                          uint32_t maxRestarts) {
-    __shared__ VertexLds S;
+__device__ __forceinline__ void vertexTask(const CellCtx& c, VertexLds& S, uint32_t i, uint32_t maxRestarts) {
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
-    const uint32_t i = tasks[blockIdx.x];
     const uint8_t st = c.status[i];
     if (st == kStatusCertified || st == kStatusBuried) return;
     WaveCell w;
@@ -165,6 +164,7 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
     const uint32_t loaded = w.V + w.E;
     const float kr = c.knownRadii[i];
     const D4 si = loadSite(c, i);
+    const uint32_t root = rootNode(c, cellMolecule(c, i));
     uint32_t vs = 0, restarts = 0;
     uint32_t flags = 0;
     while (vs < w.V / 64 + 1 && restarts < maxRestarts) {
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
         }
         {
             // nearest non-neighbour site to the vertex sphere, all lanes' searches in step
-            Trav tr{0u, 0u, need};
+            Trav tr{root, 0u, need};
             float nd = FLT_MAX;
             const uint4 r = v < w.V ? L.vtx[v] : uint4{0, 0, 0, 0};
             const F4 fcur = toF(cur);
@@ -249,14 +249,14 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, uint32_t (*stNode
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
     const uint32_t node = t.node;
-    if (node < n - 1) {
+    if (node < c.internal) {
         const Node nn = c.nodes[node];
         Box lb, rb;
         nodeBoxes(nn, lb, rb);
         const uint32_t L0 = uint32_t(nn.d.x), R0 = uint32_t(nn.d.y);
         bool ef = false, ec = false;
         uint32_t farChild = 0, closeChild = 0;
-        if (L0 >= 2 * n - 1 || R0 >= 2 * n - 1) {
+        if (L0 >= nodeCount(c) || R0 >= nodeCount(c)) {
             atomicOr(c.err, kErrKnnChild);
         } else {
             const F3 cpL = closestPoint(lb, q.bs), cpR = closestPoint(rb, q.bs);
@@ -306,8 +306,8 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, uint32_t (*stNode
         return kNone;
     }
     uint32_t cand = kNone;
-    if (node < 2 * n - 1) {
-        const uint32_t j = node - (n - 1);
+    if (node < nodeCount(c)) {
+        const uint32_t j = node - c.internal;
         bool cc = j != i && j != k && j != l && c.status[j] != kStatusBuried;
         if (cc) {
             const F4 fsj = c.sorted[j];
@@ -355,12 +355,9 @@ __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L
     }
 }
 
-__global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_t* __restrict__ tasks,
-                                                        uint32_t maxRestarts) {
-    __shared__ EdgeLds S;
+__device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t i, uint32_t maxRestarts) {
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
-    const uint32_t i = tasks[blockIdx.x];
     const uint8_t st = c.status[i];
     if (st == kStatusCertified || st == kStatusBuried) return;
     WaveCell w;
@@ -371,6 +368,7 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
     waveSync();
     const float kr = c.knownRadii[i];
     const D4 si = loadSite(c, i);
+    const uint32_t root = rootNode(c, cellMolecule(c, i));
     const F4 fsi = toF(si);
     uint32_t flags = 0;
     bool didChange = false;
@@ -442,7 +440,7 @@ __global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_
         // vertex solutions of which one is clear of the cell's neighbours
         uint32_t found = kNone;
         {
-            Trav tr{0u, 0u, need};
+            Trav tr{root, 0u, need};
             D4 sk{0, 0, 0, 0}, sl{0, 0, 0, 0};
             if (need) {
                 sk = loadSite(c, k);
@@ -514,14 +512,14 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*st
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
     const uint32_t node = t.node;
-    if (node < n - 1) {
+    if (node < c.internal) {
         const Node nn = c.nodes[node];
         Box lb, rb;
         nodeBoxes(nn, lb, rb);
         const uint32_t L0 = uint32_t(nn.d.x), R0 = uint32_t(nn.d.y);
         bool ef = false, ec = false;
         uint32_t farChild = 0, closeChild = 0;
-        if (L0 >= 2 * n - 1 || R0 >= 2 * n - 1) {
+        if (L0 >= nodeCount(c) || R0 >= nodeCount(c)) {
             atomicOr(c.err, kErrKnnChild);
         } else {
             const float dL = boxDistance(lb, bs), dR = boxDistance(rb, bs);
@@ -549,8 +547,8 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*st
         return kNone;
     }
     uint32_t cand = kNone;
-    if (node < 2 * n - 1) {
-        const uint32_t j = node - (n - 1);
+    if (node < nodeCount(c)) {
+        const uint32_t j = node - c.internal;
         bool cc = j != i && j != bk;
         if (cc) {
             const F4 fsj = c.sorted[j];
@@ -569,11 +567,9 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*st
     return cand;
 }
 
-__global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t maxRestarts) {
-    __shared__ BisectorLds S;
+__device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, uint32_t i, uint32_t maxRestarts) {
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
-    const uint32_t i = blockIdx.x;
     const uint8_t st = c.status[i];
     if (st == kStatusBuried) return;
     WaveCell w;
@@ -610,6 +606,7 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t 
     w.V = 0;
     const float kr = c.knownRadii[i];
     const D4 si = loadSite(c, i);
+    const uint32_t root = rootNode(c, cellMolecule(c, i));
     const F3 fci{float(si.x), float(si.y), float(si.z)};
     const float fri = float(si.w);
     bool didChange = false;
@@ -641,7 +638,7 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t 
         }
         // the first site of each lane's traversal whose trisector with i, bk
         // has two points, the second clear of the cell's neighbours
-        Trav tr{0u, 0u, need};
+        Trav tr{root, 0u, need};
         uint32_t cand = kNone;
         while (true) {
             const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
@@ -690,43 +687,101 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t 
     storeAppended(c, L, w, i, didChange, kBytesBisectorVal, edgesLoaded);
 }
 
-__global__ void k_compact_tasks(const uint8_t* __restrict__ status, uint32_t n, uint32_t* tasks, uint32_t* count) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && status[i] == kStatusInValidation) tasks[atomicAdd(count, 1u)] = i;
+// ---------------------------------------------------------------------------
+// Persistent launches: every wave takes cells from a device work queue
+// until it is drained, so the grid is sized to the chip, not to the task
+// count, and no launch depends on a count read back by the host.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint32_t* __restrict__ tasks,
+                                                          uint32_t* counters, uint32_t* roundTasks,
+                                                          uint32_t maxRestarts) {
+    __shared__ VertexLds S;
+    const uint32_t count = __builtin_amdgcn_readfirstlane(counters[kTaskCount]);
+    if (blockIdx.x == 0 && laneId() == 0) *roundTasks = count;
+    for (;;) {
+        const uint32_t t = popWork(counters + kVertexQueue, count);
+        if (t == kNone) break;
+        vertexTask(c, S, tasks[t], maxRestarts);
+        waveSync();
+    }
 }
 
-__global__ void k_zero1(uint32_t* p) { *p = 0u; }
+__global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_t* __restrict__ tasks,
+                                                        uint32_t* counters, uint32_t maxRestarts) {
+    __shared__ EdgeLds S;
+    const uint32_t count = __builtin_amdgcn_readfirstlane(counters[kTaskCount]);
+    for (;;) {
+        const uint32_t t = popWork(counters + kEdgeQueue, count);
+        if (t == kNone) break;
+        edgeTask(c, S, tasks[t], maxRestarts);
+        waveSync();
+    }
+}
 
-hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, uint32_t iterations, uint32_t maxRestarts,
-                            bool bisector, hipStream_t st, ValidationTiming* tm) {
+__global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t* counters, uint32_t maxRestarts) {
+    __shared__ BisectorLds S;
+    for (;;) {
+        const uint32_t i = popWork(counters + kBisectorQueue, c.n);
+        if (i == kNone) break;
+        bisectorTask(c, S, i, maxRestarts);
+        waveSync();
+    }
+}
+
+// Cells still in validation, appended in wave order (one atomic per wave).
+__global__ __launch_bounds__(256) void k_compact_tasks(const uint8_t* __restrict__ status, uint32_t n, uint32_t* tasks,
+                                                       uint32_t* count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool take = i < n && status[i] == kStatusInValidation;
+    const uint64_t m = ballot(take);
+    uint32_t base = 0;
+    if (laneId() == 0 && m) base = atomicAdd(count, popc64(m));
+    base = __shfl(base, 0, 64);
+    if (take) tasks[base + countBelow(m)] = i;
+}
+
+// Clears the task count and the vertex / edge queue heads of a round.
+__global__ void k_round_reset(uint32_t* counters) {
+    const uint32_t t = threadIdx.x;
+    if (t < kRoundWords) counters[kTaskCount + t] = 0u;
+}
+
+hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks) {
+    int v = 0, e = 0, b = 0;
+    hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_vertex_validation, 64, 0);
+    if (r == hipSuccess) r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_edge_validation, 64, 0);
+    if (r == hipSuccess) r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bisector_validation, 64, 0);
+    *vertexBlocks = uint32_t(v > 0 ? v : 1);
+    *edgeBlocks = uint32_t(e > 0 ? e : 1);
+    *bisectorBlocks = uint32_t(b > 0 ? b : 1);
+    return r;
+}
+
+hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const ValidationGrid& g,
+                            uint32_t iterations, uint32_t maxRestarts, bool bisector, hipStream_t st,
+                            ValidationTiming* tm) {
     if (bisector) {
-        hipLaunchKernelGGL(k_bisector_validation, dim3(c.n), dim3(64), 0, st, c, maxRestarts);
+        hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, v.counters + kBisectorQueue, kXcdQueueWords, 0u);
+        hipLaunchKernelGGL(k_bisector_validation, dim3(g.bisector), dim3(64), 0, st, c, v.counters, maxRestarts);
         return hipGetLastError();
     }
-    for (uint32_t it = 0; it < iterations; it++) {
-        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, v.counters);
+    if (iterations > kMaxRounds) return hipErrorInvalidValue;
+    hipError_t e = hipSuccess;
+    for (uint32_t it = 0; it < iterations && e == hipSuccess; it++) {
+        hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, st, v.counters);
         hipLaunchKernelGGL(k_compact_tasks, dim3((c.n + 255) / 256), dim3(256), 0, st, c.status, c.n, v.tasks,
-                           v.counters);
-        uint32_t count = 0;
-        hipError_t e = hipMemcpyAsync(&count, v.counters, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) return e;
-        if (count == 0) break;
+                           v.counters + kTaskCount);
         const bool timed = tm && it < kTimedRounds;
-        if (timed) {
-            tm->tasks[it] = count;
-            tm->rounds = it + 1;
-            e = hipEventRecord(tm->ev[it][0], st);
-        }
-        if (tm) tm->totalTasks += count;
-        hipLaunchKernelGGL(k_vertex_validation, dim3(count), dim3(64), 0, st, c, v.tasks, maxRestarts);
+        if (timed) e = hipEventRecord(tm->ev[it][0], st);
+        hipLaunchKernelGGL(k_vertex_validation, dim3(g.vertex), dim3(64), 0, st, c, v.tasks, v.counters,
+                           v.counters + kRoundTasks + it, maxRestarts);
         if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][1], st);
-        hipLaunchKernelGGL(k_edge_validation, dim3(count), dim3(64), 0, st, c, v.tasks, maxRestarts);
+        hipLaunchKernelGGL(k_edge_validation, dim3(g.edge), dim3(64), 0, st, c, v.tasks, v.counters, maxRestarts);
         if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][2], st);
         if (e == hipSuccess) e = hipGetLastError();
-        if (e != hipSuccess) return e;
     }
-    return hipSuccess;
+    if (tm) tm->rounds = iterations < kTimedRounds ? iterations : kTimedRounds;
+    return e;
 }
 
 }  // namespace apo_hip

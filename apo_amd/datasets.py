@@ -8,6 +8,9 @@ to FP32 once.
 """
 from __future__ import annotations
 
+import ctypes
+import os
+
 import numpy as np
 
 
@@ -57,9 +60,26 @@ PROTEIN_RADII = np.array([1.70, 1.55, 1.52, 1.80])
 PROTEIN_FREQ = np.array([0.62, 0.17, 0.19, 0.02])
 
 
+def _samples_lib():
+    global _SAMPLES
+    if _SAMPLES is None:
+        from . import build as _build
+        if not os.path.exists(_build.SAMPLES):
+            _build.build_samples()
+        L = ctypes.CDLL(_build.SAMPLES)
+        L.apo_samples_protein.restype = ctypes.c_int
+        L.apo_samples_protein.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double,
+                                          ctypes.POINTER(ctypes.c_float)]
+        _SAMPLES = L
+    return _SAMPLES
+
+
+_SAMPLES = None
+
+
 def protein_like_sites(n: int, seed: int = 2, density: float = 0.05, seed_tag: int = 0) -> np.ndarray:
     """Protein-like atoms in Å, packed at `density` atoms/Å^3 in a cube of
-    side (n / density)^(1/3).
+    side (n / density)^(1/3) (generator in apo_amd/csrc/samples.cpp).
 
     Chains are random walks of 3.8 Å steps with bond angle uniform in
     [80°, 150°] and torsion uniform in [0°, 360°); each residue carries 1-4
@@ -67,88 +87,34 @@ def protein_like_sites(n: int, seed: int = 2, density: float = 0.05, seed_tag: i
     an existing one is redrawn (up to 24 times, then the chain ends); a step
     leaving the cube also ends the chain.  Elements C/N/O/S are drawn with
     frequencies .62/.17/.19/.02 and get radii 1.70/1.55/1.52/1.80 Å.
-    Randomness: numpy's PCG64 seeded with (seed, seed_tag).
+    Randomness: PCG32 with state seed `seed` and stream `seed_tag`.
     """
-    rng = np.random.Generator(np.random.PCG64([seed, seed_tag]))
-    side = (n / density) ** (1.0 / 3.0)
-    cell = 1.2
-    dims = max(1, int(side / cell))
-    grid: dict = {}
-    pos = np.empty((n, 3), np.float64)
-    count = 0
+    out = np.zeros((n, 4), np.float32)
+    st = _samples_lib().apo_samples_protein(n, seed, seed_tag, density,
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    if st != 0:
+        raise ValueError("invalid protein-like sample arguments")
+    return out
 
-    def key(p):
-        return (min(int(p[0] / cell), dims - 1), min(int(p[1] / cell), dims - 1), min(int(p[2] / cell), dims - 1))
 
-    def free(p):
-        if not (0.0 <= p[0] < side and 0.0 <= p[1] < side and 0.0 <= p[2] < side):
-            return False
-        kx, ky, kz = key(p)
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    for q in grid.get((kx + dx, ky + dy, kz + dz), ()):
-                        d = p - pos[q]
-                        if d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < cell * cell:
-                            return False
-        return True
+def molecule_sizes(count: int = 512, seed: int = 4, lo: int = 500, hi: int = 50000) -> np.ndarray:
+    """Atom counts log-uniform in [lo, hi]: round(lo * (hi/lo)^u) with u the
+    reference stream's draw (seed << 28) + 4m."""
+    base = np.uint32((seed << 28) & 0xFFFFFFFF)
+    with np.errstate(over="ignore"):
+        u = pcg_unit(np.arange(count, dtype=np.uint32) * np.uint32(4) + base)
+    return np.rint(lo * np.power(hi / lo, u)).astype(np.int64)
 
-    def add(p):
-        nonlocal count
-        pos[count] = p
-        grid.setdefault(key(p), []).append(count)
-        count += 1
 
-    def unit():
-        v = rng.normal(size=3)
-        return v / np.linalg.norm(v)
-
-    while count < n:
-        p0 = rng.uniform(0.0, side, 3)
-        if not free(p0):
-            continue
-        add(p0)
-        prev, prev2 = p0, None
-        residues = int(rng.integers(50, 400))
-        for _ in range(residues):
-            if count >= n:
-                break
-            placed = False
-            for _try in range(24):
-                if prev2 is None:
-                    q = prev + 3.8 * unit()
-                else:
-                    # next backbone atom from bond angle and torsion (NeRF)
-                    theta = np.radians(rng.uniform(80.0, 150.0))
-                    phi = rng.uniform(0.0, 2.0 * np.pi)
-                    b = prev - prev2
-                    b /= np.linalg.norm(b)
-                    a = unit()
-                    nrm = np.cross(b, a)
-                    nn = np.linalg.norm(nrm)
-                    if nn < 1e-6:
-                        continue
-                    nrm /= nn
-                    m = np.cross(nrm, b)
-                    d = np.array([-np.cos(theta), np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi)])
-                    q = prev + 3.8 * (d[0] * b + d[1] * m + d[2] * nrm)
-                if free(q):
-                    placed = True
-                    break
-            if not placed:
-                break
-            add(q)
-            prev2, prev = prev, q
-            for _s in range(int(rng.integers(1, 5))):
-                if count >= n:
-                    break
-                for _try in range(24):
-                    s = q + 1.5 * unit()
-                    if free(s):
-                        add(s)
-                        break
-    elem = rng.choice(4, size=n, p=PROTEIN_FREQ)
-    out = np.empty((n, 4), np.float64)
-    out[:, :3] = pos
-    out[:, 3] = PROTEIN_RADII[elem]
-    return out.astype(np.float32)
+def batched_molecules(count: int = 512, seed: int = 4, lo: int = 500, hi: int = 50000,
+                      density: float = 0.05):
+    """A batch of independent protein-like molecules: molecule m has
+    molecule_sizes(...)[m] atoms from protein_like_sites(seed=seed, seed_tag=m),
+    each in its own frame.  Returns (sites (N, 4) float32, offsets (count+1,) uint32)."""
+    sizes = molecule_sizes(count, seed, lo, hi)
+    offsets = np.zeros(count + 1, np.uint32)
+    offsets[1:] = np.cumsum(sizes)
+    sites = np.empty((int(offsets[-1]), 4), np.float32)
+    for m in range(count):
+        sites[offsets[m]:offsets[m + 1]] = protein_like_sites(int(sizes[m]), seed=seed, density=density, seed_tag=m)
+    return sites, offsets

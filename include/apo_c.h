@@ -38,11 +38,15 @@ typedef struct apo_config {
     uint32_t max_restarts;    /* restarts per cell per validation kernel (100, :272)  */
     uint32_t stage_mask;      /* bit0 cells, bit1 vertex/edge validation, bit2 bisector,
                                  bit3 stop after the kNN pass */
-    uint32_t waves_per_cu;    /* persistent-grid occupancy of the cell kernels, 0 = auto */
+    uint32_t waves_per_cu;    /* waves per CU of the persistent cell and validation grids,
+                                 0 = each kernel's full occupancy */
     uint32_t device;          /* HIP device ordinal */
     float proposal_cost;      /* learned proposal: cost of validating one cell, in neighbour updates;
                                  a site gets the class c minimising k_c + cost * P(not certified at k_c).
                                  0 = the most probable class */
+    uint32_t graph;           /* 1: capture the whole device build (LBVH to vertex output) in a HIP graph
+                                 and replay it while the site pointer, site and molecule counts and this
+                                 config stay the same; stats then hold ms_total only */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,
@@ -96,6 +100,23 @@ apo_status apo_build(apo_ctx* ctx, const float* xyzr, uint32_t n);
 /* Same, for sites already resident in device memory (n float4 records). */
 apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n);
 
+/* ---- batches of independent molecules.  No counterpart in the reference,
+ * whose callers build one molecule per AlgorithmGPU in a host loop
+ * (app/benchmarks/full_benchmarks_proteins.cpp:43-79, one
+ * computeApolloniusDiagram per protein).  Molecule m is sites
+ * [offsets[m], offsets[m+1]) of xyzr (offsets: m+1 host values, offsets[0] = 0,
+ * at least 2 sites per molecule, at most 2^20 molecules); its diagram is the
+ * one apo_build gives for those sites alone, with site ids offset by
+ * offsets[m].  All molecules go through one launch per stage: an
+ * offset-indexed LBVH (one tree per molecule) and persistent work-queue
+ * cell and validation kernels.  Vertices come out molecule after molecule;
+ * artificial sites of molecule m have ids n + 6m + 0..5. ---- */
+apo_status apo_build_batched(apo_ctx* ctx, const float* xyzr, const uint32_t* offsets, uint32_t molecules);
+apo_status apo_build_batched_device(apo_ctx* ctx, const float* xyzr_device, const uint32_t* offsets,
+                                    uint32_t molecules);
+/* First output vertex of each molecule of the last build (molecules + 1 values). */
+apo_status apo_batch_vertex_offsets(apo_ctx* ctx, uint64_t* offsets);
+
 /* Vertex output of AlgorithmGPU<K>::toHost (algorithm.inl:1596-1646):
  * each Apollonius vertex once, from the cell of its smallest site,
  * centre + empty-sphere radius in FP64, the 4 tangent sites as original
@@ -128,7 +149,7 @@ apo_status apo_get_topology_sets(apo_ctx* ctx, uint32_t* bisectors, uint64_t bis
 /* Per-site cell topology (the cell records of AlgorithmGPU's dVertices /
  * dEdges, algorithm.cuh:38-43) as CSR in original site order with original
  * ids: vertex records (j, k, l, type) and edge records (j, k, vertexNb, 0);
- * artificial sites keep ids n..n+5, vertexNb 0xffffffff marks a closed
+ * artificial sites keep ids n..n+5 (n + 6m.. in molecule m of a batch), vertexNb 0xffffffff marks a closed
  * edge, buried sites have no records.  vertex_offsets / edge_offsets: n+1
  * entries (may be NULL); NULL record arrays only fill the offsets. */
 apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_t* vertices, uint64_t vertex_cap,
@@ -172,7 +193,9 @@ apo_status apo_proposal_classes(apo_ctx* ctx, uint8_t* classes);
 apo_status apo_get_sorted_sites(apo_ctx* ctx, float* xyzr, uint32_t* indices);
 /* LBVH in the reference node numbering (lbvh.cuh:12-37): internal nodes
  * [0, n-1), leaves [n-1, 2n-1).  aabb: 6 floats per node (min, max);
- * max_radius: largest site radius below each node. */
+ * max_radius: largest site radius below each node.  After a batch of m
+ * molecules: 2n - m nodes, internal [0, n-m) (molecule i's tree rooted at
+ * offsets[i] - i), leaves [n-m, 2n-m). */
 apo_status apo_get_lbvh(apo_ctx* ctx, uint64_t* sorted_codes, int32_t* parent, int32_t* left, int32_t* right,
                         float* aabb, uint32_t* sizes, float* max_radius);
 /* The 6 artificial bounding sites appended after the n input sites

@@ -37,7 +37,133 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=60_000, help="sites in the CPU-baseline sub-cube")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
     ap.add_argument("--proposal-cost", type=float, default=400.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-molecules workload")
+    ap.add_argument("--batch-molecules", type=int, default=512)
     return ap.parse_args()
+
+
+def _vertices(ctx, L, _lib, v):
+    xyzw = np.zeros((v, 4), np.float64)
+    ids = np.zeros((v, 4), np.uint32)
+    _lib.check(ctx, L.apo_get_vertices(ctx, _lib.dptr(xyzw), _lib.uptr(ids), v))
+    return xyzw, ids
+
+
+def graph_replay(dev, n, k, steps, warmup, say):
+    """The same build captured once in a HIP graph and replayed: one graph
+    launch per step, bracketed by device synchronisations."""
+    import torch
+    from apo_amd import _lib
+    cfg = _lib.Config()
+    _lib.lib().apo_default_config(_lib.byref(cfg))
+    cfg.k = k
+    cfg.graph = 1
+    ctx = _lib.create(cfg)
+    L = _lib.lib()
+    ptr = _lib.ctypes.c_void_p(dev.data_ptr())
+    for _ in range(max(warmup, 1)):  # the first build captures the graph
+        _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+    wall, dev_ms = 0.0, 0.0
+    s = _lib.Stats()
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+        dev_ms += s.ms_total
+    ms = 1000.0 * wall / steps
+    say(f"graph replay: {ms:.1f} ms per step")
+    L.apo_destroy(ctx)
+    return {"cells_per_s": n / (ms / 1000.0), "ms_per_step": ms, "device_ms_per_replay": dev_ms / steps,
+            "vertices": int(s.vertices)}
+
+
+def batch_workload(count, steps, warmup, say):
+    """512 independent protein-like molecules, atom counts log-uniform in
+    [500, 50000] (seed 4), in one batched build: offset-indexed LBVH and
+    persistent work-queue kernels, captured in a HIP graph and replayed.
+    Checks: the graph replay equals the eager build, and a spread of
+    molecules equals their own single-molecule builds, vertex for vertex."""
+    import torch
+    from apo_amd import _lib, datasets
+    t0 = time.perf_counter()
+    sites, offsets = datasets.batched_molecules(count=count, seed=4, lo=500, hi=50000)
+    n = len(sites)
+    say(f"batch: {count} molecules, {n} atoms generated in {time.perf_counter() - t0:.1f} s")
+    dev = torch.from_numpy(sites).to("cuda")
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    ptr = _lib.ctypes.c_void_p(dev.data_ptr())
+    off = np.ascontiguousarray(offsets, np.uint32)
+
+    def make(graph):
+        cfg = _lib.Config()
+        L.apo_default_config(_lib.byref(cfg))
+        cfg.graph = 1 if graph else 0
+        return _lib.create(cfg)
+
+    # eager build: stage split
+    ctx = make(False)
+    _lib.check(ctx, L.apo_build_batched_device(ctx, ptr, _lib.uptr(off), count))
+    s = _lib.Stats()
+    _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+    stage = {k: float(getattr(s, k)) for k in ("ms_total", "ms_lbvh", "ms_knn", "ms_cells", "ms_validation",
+                                                 "ms_bisector", "ms_output")}
+    v_eager = int(s.vertices)
+    x_e, i_e = _vertices(ctx, L, _lib, v_eager)
+    vo = np.zeros(count + 1, np.uint64)
+    _lib.check(ctx, L.apo_batch_vertex_offsets(ctx, _lib.u64ptr(vo)))
+    cnt = _lib.c_uint64()
+    _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
+    L.apo_destroy(ctx)
+    say(f"batch eager build: {stage['ms_total']:.1f} ms, {v_eager} vertices")
+
+    # graph replays: the timed steps
+    gctx = make(True)
+    for _ in range(max(warmup, 1)):
+        _lib.check(gctx, L.apo_build_batched_device(gctx, ptr, _lib.uptr(off), count))
+    wall, dev_ms = 0.0, 0.0
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(gctx, L.apo_build_batched_device(gctx, ptr, _lib.uptr(off), count))
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+        _lib.check(gctx, L.apo_get_stats(gctx, _lib.byref(s)))
+        dev_ms += s.ms_total
+    ms = 1000.0 * wall / steps
+    x_g, i_g = _vertices(gctx, L, _lib, int(s.vertices))
+    L.apo_destroy(gctx)
+    same_graph = bool(np.array_equal(i_g, i_e) and np.array_equal(x_g.view(np.uint64), x_e.view(np.uint64)))
+    say(f"batch graph replay: {ms:.1f} ms per step, identical to eager: {same_graph}")
+
+    # spot check: molecules across the size range against single builds
+    sizes = np.diff(offsets.astype(np.int64))
+    order = np.argsort(sizes)
+    picks = sorted({int(order[q]) for q in np.linspace(0, count - 1, 6).astype(int)})
+    single = make(False)
+    matched = 0
+    for m in picks:
+        a, b = int(offsets[m]), int(offsets[m + 1])
+        sub = np.ascontiguousarray(sites[a:b])
+        _lib.check(single, L.apo_build(single, _lib.fptr(sub), b - a))
+        sv = _lib.c_uint64()
+        _lib.check(single, L.apo_vertex_count(single, _lib.byref(sv)))
+        xs, is_ = _vertices(single, L, _lib, int(sv.value))
+        xb, ib = x_e[vo[m]:vo[m + 1]], i_e[vo[m]:vo[m + 1]] - np.uint32(a)
+        matched += int(np.array_equal(is_, ib) and np.array_equal(xs.view(np.uint64), xb.view(np.uint64)))
+    L.apo_destroy(single)
+    say(f"batch spot check: {matched} of {len(picks)} molecules equal their single builds")
+    return {"workload": f"{count} protein-like molecules (C2 generator per molecule, seed (4, m)), atom counts "
+                        f"log-uniform [500, 50000], {n} atoms, FP32 A",
+            "molecules": count, "sites": n, "cells_per_s": n / (ms / 1000.0), "molecules_per_s": count / (ms / 1000.0),
+            "ms_per_step": ms, "device_ms_per_replay": dev_ms / steps, "steps": steps, "path": "HIP graph replay",
+            "eager_stage_ms": stage, "vertices": v_eager, "vertices_per_s": v_eager / (ms / 1000.0),
+            "graph_equals_eager": same_graph, "spot_molecules": picks, "spot_equal_single_build": matched,
+            "empty_sphere_violations": int(cnt.value),
+            "note": "violations are the reference method's own on overlapping atoms (counted, as its precision check does)"}
 
 
 def learned_comparison(sites, dev, n, k, steps, warmup, cost, plain_ids, say):
@@ -245,6 +371,8 @@ def main():
                      "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
     }
+    say("graph replay of the same build")
+    out["graph_replay"] = graph_replay(dev, n, args.k, args.steps, args.warmup, say)
     if not args.no_learned:
         xyzw_p = np.zeros((vertices, 4), np.float64)
         ids_p = np.zeros((vertices, 4), np.uint32)
@@ -254,6 +382,8 @@ def main():
         lc = learned_comparison(sites, dev, n, args.k, args.steps, args.warmup, args.proposal_cost, plain_ids, say)
         lc["speedup_vs_plain"] = lc["cells_per_s"] / out["value"]
         out["learned"] = lc
+    if not args.no_batch:
+        out["batch"] = batch_workload(args.batch_molecules, min(args.steps, 3), 1, say)
     if not args.no_cpu_baseline:
         say(f"CPU baseline: oracle on a sub-cube of about {args.cpu_sample} sites")
         out["cpu_baseline"] = cpu_baseline(sites, args.cpu_sample)

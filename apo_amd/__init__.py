@@ -38,11 +38,13 @@ class AlgorithmGPU:
     offsets: for a batch, the molecule boundaries (m + 1 values, molecule i is
     sites[offsets[i]:offsets[i+1]]), built in one pass of offset-indexed
     LBVHs and persistent work-queue kernels.  graph: capture the device build
-    in a HIP graph and replay it on later builds of the same shape."""
+    in a HIP graph and replay it on later builds of the same shape.  Other
+    keywords set fields of apo_config (include/apo_c.h), e.g. cover_ratio,
+    waves_edge or proposal_fallback_k."""
 
     def __init__(self, sites, k: int = 16, max_iterations: int = 10, max_restarts: int = 100,
                  stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0, proposal=None,
-                 proposal_cost: float = 0.0, offsets=None, graph: bool = False):
+                 proposal_cost: float = 0.0, offsets=None, graph: bool = False, **tuning):
         self.sites = np.ascontiguousarray(np.asarray(sites, dtype=np.float32).reshape(-1, 4))
         self.offsets = None if offsets is None else np.ascontiguousarray(offsets, np.uint32)
         self.k = int(k)
@@ -56,6 +58,12 @@ class AlgorithmGPU:
         cfg.device = device
         cfg.proposal_cost = proposal_cost
         cfg.graph = 1 if graph else 0
+        # further apo_config fields (cover_ratio, waves_edge, proposal_fallback_k ...)
+        names = {f for f, _ in cfg._fields_}
+        for key, val in tuning.items():
+            if key not in names:
+                raise TypeError(f"unknown configuration field {key}")
+            setattr(cfg, key, val)
         self._ctx = _lib.create(cfg)
         self.built = False
         if proposal is not None:

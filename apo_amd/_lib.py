@@ -24,7 +24,9 @@ class ApoError(RuntimeError):
 class Config(ctypes.Structure):
     _fields_ = [("k", ctypes.c_uint32), ("max_iterations", ctypes.c_uint32), ("max_restarts", ctypes.c_uint32),
                 ("stage_mask", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("device", ctypes.c_uint32),
-                ("proposal_cost", ctypes.c_float), ("graph", ctypes.c_uint32)]
+                ("proposal_cost", ctypes.c_float), ("graph", ctypes.c_uint32), ("cover_ratio", ctypes.c_float),
+                ("waves_cells", ctypes.c_uint32), ("waves_vertex", ctypes.c_uint32), ("waves_edge", ctypes.c_uint32),
+                ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32)]
 
 
 class Stats(ctypes.Structure):
@@ -62,6 +64,7 @@ SIGNATURES = {
     "apo_build_batched": (ctypes.c_int, [_vp, _f32p, _u32p, _u32]),
     "apo_build_batched_device": (ctypes.c_int, [_vp, _vp, _u32p, _u32]),
     "apo_batch_vertex_offsets": (ctypes.c_int, [_vp, _u64p]),
+    "apo_launch_occupancy": (ctypes.c_int, [_vp, _u32p]),
     "apo_vertex_count": (ctypes.c_int, [_vp, _u64p]),
     "apo_get_vertices": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64]),
     "apo_get_device_vertices": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),

@@ -29,13 +29,13 @@ namespace {
 // of any buffer drops the graph).
 struct GraphKey {
     const void* sites = nullptr;
-    uint32_t n = 0, molecules = 0, k = 0, stageMask = 0, maxIterations = 0, maxRestarts = 0, wavesPerCu = 0;
+    uint32_t n = 0, molecules = 0;
     bool learned = false;
     uint64_t outCap = 0;
+    apo_config cfg{};
     bool operator==(const GraphKey& o) const {
-        return sites == o.sites && n == o.n && molecules == o.molecules && k == o.k && stageMask == o.stageMask &&
-               maxIterations == o.maxIterations && maxRestarts == o.maxRestarts && wavesPerCu == o.wavesPerCu &&
-               learned == o.learned && outCap == o.outCap;
+        return sites == o.sites && n == o.n && molecules == o.molecules && learned == o.learned &&
+               outCap == o.outCap && std::memcmp(&cfg, &o.cfg, sizeof(cfg)) == 0;
     }
 };
 
@@ -252,13 +252,14 @@ CellCtx cellCtx(apo_ctx* ctx) {
     c.enb = ctx->enb;
     c.err = ctx->err;
     c.bytes = ctx->bytes;
+    c.coverRatio = ctx->cfg.cover_ratio < 0.f ? __builtin_inff() : ctx->cfg.cover_ratio;
     return c;
 }
 
-// Blocks of a persistent launch: the CUs times the waves each holds at full
-// occupancy, or times config.waves_per_cu when that is set.
-uint32_t persistentBlocks(apo_ctx* ctx, uint32_t occupancy) {
-    const uint32_t perCu = ctx->cfg.waves_per_cu ? ctx->cfg.waves_per_cu : occupancy;
+// Blocks of a persistent launch: the CUs times the kernel's own waves per
+// CU setting, else config.waves_per_cu, else its full occupancy.
+uint32_t persistentBlocks(apo_ctx* ctx, uint32_t own, uint32_t occupancy) {
+    const uint32_t perCu = own ? own : ctx->cfg.waves_per_cu ? ctx->cfg.waves_per_cu : occupancy;
     return uint32_t(ctx->cus) * perCu;
 }
 
@@ -339,19 +340,21 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
         // features of the 16 nearest, proposal classes, then each cell with its
         // proposed neighbour count and the plain certification test
         APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st));
-        APO_HIP(mlpInferLaunch(ctx->feat, n, ctx->pw, ctx->cls, nullptr, ctx->cfg.proposal_cost, st));
+        APO_HIP(mlpInferLaunch(ctx->feat, n, ctx->pw, ctx->cls, nullptr, ctx->cfg.proposal_cost,
+                               ctx->cfg.proposal_fallback_k, st));
         APO_HIP(mark(7));
-        APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, ctx->cls, nullptr, kProposal, st));
+        APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, ctx->cls, nullptr, kProposal, ctx->cfg.proposal_fallback_k, st));
     } else {
         APO_HIP(mark(7));
         APO_HIP(cellsLaunch(cc, ctx->knn, (ctx->cfg.stage_mask & 1u) ? K : 0u, ctx->counters + kCellQueue,
-                            persistentBlocks(ctx, ctx->occCells), st));
+                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st));
     }
     APO_TIMED_STAGE("cells");
     APO_HIP(mark(3));
     const ValidationBuffers vb{ctx->tasks, ctx->counters};
-    const ValidationGrid vg{persistentBlocks(ctx, ctx->occVertex), persistentBlocks(ctx, ctx->occEdge),
-                            persistentBlocks(ctx, ctx->occBisector)};
+    const ValidationGrid vg{persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
+                            persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge),
+                            persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector)};
     if (ctx->cfg.stage_mask & 2u)
         APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st,
                                  timed ? &ctx->vt : nullptr));
@@ -467,11 +470,7 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     key.sites = sitesDev;
     key.n = n;
     key.molecules = ctx->molecules;
-    key.k = ctx->cfg.k;
-    key.stageMask = ctx->cfg.stage_mask;
-    key.maxIterations = ctx->cfg.max_iterations;
-    key.maxRestarts = ctx->cfg.max_restarts;
-    key.wavesPerCu = ctx->cfg.waves_per_cu;
+    key.cfg = ctx->cfg;
     key.learned = learned;
     key.outCap = ctx->outCap;
     if (!ctx->graphExec || !(key == ctx->graphKey)) {
@@ -505,7 +504,7 @@ apo_status runLabelPass(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     APO_HIP(lbvhBuild(lbvhBuffers(ctx), sitesDev, n, segments(ctx), st));
     APO_HIP(knnLaunch(cc, 64u, ctx->knn, st));
     APO_HIP(featuresLaunch(cc, ctx->knn, 64u, ctx->feat, st));
-    APO_HIP(cellsKLaunch(cc, ctx->knn, 64u, nullptr, ctx->labels, kLabels, st));
+    APO_HIP(cellsKLaunch(cc, ctx->knn, 64u, nullptr, ctx->labels, kLabels, 16u, st));
     uint32_t guard = 0;
     APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
@@ -528,6 +527,9 @@ void apo_default_config(apo_config* cfg) {
     cfg->device = 0;
     cfg->proposal_cost = 0.f;
     cfg->graph = 0;
+    cfg->cover_ratio = 2.f;
+    cfg->waves_cells = cfg->waves_vertex = cfg->waves_edge = cfg->waves_bisector = 0;
+    cfg->proposal_fallback_k = 16;
 }
 
 apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
@@ -536,8 +538,9 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     apo_ctx* ctx = new apo_ctx();
     if (cfg) ctx->cfg = *cfg;
     else apo_default_config(&ctx->cfg);
-    const uint32_t k = ctx->cfg.k;
-    if (!(k == 16 || k == 24 || k == 32 || k == 48 || k == 64) || ctx->cfg.max_iterations > kMaxRounds) {
+    const uint32_t k = ctx->cfg.k, fk = ctx->cfg.proposal_fallback_k;
+    if (!(k == 16 || k == 24 || k == 32 || k == 48 || k == 64) || ctx->cfg.max_iterations > kMaxRounds ||
+        !(fk == 16 || fk == 24 || fk == 32 || fk == 48 || fk == 64)) {
         delete ctx;
         return APO_ERR_INVALID;
     }
@@ -652,6 +655,16 @@ apo_status apo_build_batched(apo_ctx* ctx, const float* xyzr, const uint32_t* of
     if (s != APO_OK) return s;
     APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, ctx->stream));
     return runPipeline(ctx, ctx->sitesIn, n);
+}
+
+apo_status apo_launch_occupancy(apo_ctx* ctx, uint32_t* out5) {
+    if (!ctx || !out5) return APO_ERR_INVALID;
+    out5[0] = ctx->occCells;
+    out5[1] = ctx->occVertex;
+    out5[2] = ctx->occEdge;
+    out5[3] = ctx->occBisector;
+    out5[4] = uint32_t(ctx->cus);
+    return APO_OK;
 }
 
 apo_status apo_batch_vertex_offsets(apo_ctx* ctx, uint64_t* offsets) {
@@ -1122,7 +1135,8 @@ apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t coun
     APO_HIP(dC.alloc(count));
     if (logits) APO_HIP(dL.alloc(size_t(count) * kProposalClasses));
     APO_HIP(hipMemcpyAsync(dF.p, features, size_t(count) * kFeatures * 4, hipMemcpyHostToDevice, st));
-    APO_HIP(mlpInferLaunch(dF.p, count, ctx->pw, dC.p, logits ? dL.p : nullptr, ctx->cfg.proposal_cost, st));
+    APO_HIP(mlpInferLaunch(dF.p, count, ctx->pw, dC.p, logits ? dL.p : nullptr, ctx->cfg.proposal_cost,
+                           ctx->cfg.proposal_fallback_k, st));
     APO_HIP(hipMemcpyAsync(classes, dC.p, count, hipMemcpyDeviceToHost, st));
     if (logits) APO_HIP(hipMemcpyAsync(logits, dL.p, size_t(count) * kProposalClasses * 4, hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));

@@ -109,6 +109,7 @@ struct CellCtx {
     uint32_t* enb;
     uint32_t* err;          // device guard bits
     unsigned long long* bytes;  // algorithmic HBM bytes per kernel family, kByteSlots counters each
+    float coverRatio;       // edge searches with a sphere above coverRatio x known radius use the edge cover
 };
 
 __device__ __forceinline__ uint32_t nodeCount(const CellCtx& c) { return c.n + c.internal; }
@@ -223,9 +224,10 @@ constexpr uint32_t kProposalParams = kFeatures * kHidden + kHidden + kHidden * k
 constexpr uint32_t kLabelBuried = 255;
 enum CellsMode : uint32_t { kProposal = 0, kLabels = 1 };
 // Neighbours a class asks for; the last class (not certified within 64)
-// runs the plain 16 and goes to validation.
-__host__ __device__ constexpr uint32_t proposalK(uint32_t c) {
-    return c == 0 ? 20u : c == 1 ? 24u : c == 2 ? 28u : c == 3 ? 32u : c == 4 ? 40u : c == 5 ? 48u : c == 6 ? 64u : 16u;
+// runs `fallback` (16 by default) and goes to validation.
+__host__ __device__ constexpr uint32_t proposalK(uint32_t c, uint32_t fallback = 16u) {
+    return c == 0 ? 20u : c == 1 ? 24u : c == 2 ? 28u : c == 3 ? 32u : c == 4 ? 40u : c == 5 ? 48u : c == 6 ? 64u
+                                                                                                       : fallback;
 }
 // Class of a label (neighbours used at certification, 0 = never within 64).
 __host__ __device__ constexpr uint32_t proposalClass(uint32_t label) {
@@ -245,10 +247,10 @@ struct TrainBuffers {
     float *w, *wq, *grad, *m, *v, *loss;
 };
 hipError_t cellsKLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, const uint8_t* cls, uint8_t* labels,
-                        uint32_t mode, hipStream_t st);
+                        uint32_t mode, uint32_t fallbackK, hipStream_t st);
 hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st);
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
-                          float cost, hipStream_t st);
+                          float cost, uint32_t fallbackK, hipStream_t st);
 hipError_t packWeightsLaunch(const float* params, const ProposalWeights& w, hipStream_t st);
 hipError_t mlpTrainStep(const TrainBuffers& t, uint32_t rows, float lr, uint32_t step, hipStream_t st);
 hipError_t gatherLaunch(const float* feat, const uint8_t* labels, const uint32_t* order, uint32_t rows, float* x,

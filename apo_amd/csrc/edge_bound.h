@@ -32,7 +32,6 @@
 namespace apo_hip {
 
 constexpr int kCoverBalls = 8;       // balls per cover
-constexpr float kCoverRatio = 2.0f;  // cover an edge whose search sphere exceeds this x known radius
 constexpr uint32_t kCoverRoots = 512;
 constexpr int kCoverRuns = kCoverBalls + 1;
 

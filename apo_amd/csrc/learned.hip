@@ -26,7 +26,8 @@ namespace apo_hip {
 // test after every update (mode kLabels, getCellsContribution).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_cells_k(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
-                                                const uint8_t* __restrict__ cls, uint8_t* labels, uint32_t mode) {
+                                                const uint8_t* __restrict__ cls, uint8_t* labels, uint32_t mode,
+                                                uint32_t fallbackK) {
     __shared__ CellLds L;
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
@@ -43,7 +44,7 @@ __global__ __launch_bounds__(64) void k_cells_k(CellCtx c, const uint32_t* __res
         uint32_t use = K;
         if (mode == kProposal) {
             const uint32_t q = cls[i];
-            use = q < kProposalClasses ? proposalK(q) : K;
+            use = q < kProposalClasses ? proposalK(q, fallbackK) : K;
             if (use > K) use = K;
         }
         uint32_t myJ = kNone;
@@ -199,7 +200,7 @@ __device__ __forceinline__ void mlpLayer(const uint16_t (*in)[kHidden], const ui
 __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat, uint32_t n, const uint16_t* w1,
                                                   const float* b1, const uint16_t* w2, const float* b2,
                                                   const uint16_t* w3, const float* b3, uint8_t* cls,
-                                                  float* logitsOut, float cost) {
+                                                  float* logitsOut, float cost, uint32_t fallbackK) {
     __shared__ uint16_t X[16][kHidden];
     __shared__ uint16_t H[16][kHidden];
     __shared__ float Lg[16][16];
@@ -231,13 +232,13 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
             if (cost > 0.f) {
                 // expected work: k_c updates, plus a validation pass when the
                 // cell is not certified at k_c (classes above c); the last
-                // class runs 16 and always validates
+                // class runs fallbackK neighbours and always validates
                 float p[kProposalClasses], z = 0.f;
                 for (uint32_t q = 0; q < kProposalClasses; q++) {
                     p[q] = __expf(Lg[lane][q] - bv);
                     z += p[q];
                 }
-                float bestCost = float(proposalK(kProposalClasses - 1)) + cost, cum = 0.f;
+                float bestCost = float(proposalK(kProposalClasses - 1, fallbackK)) + cost, cum = 0.f;
                 best = kProposalClasses - 1;
                 for (uint32_t q = 0; q + 1 < kProposalClasses; q++) {
                     cum += p[q] / z;
@@ -375,9 +376,9 @@ __global__ void k_pack_weights(const float* p, uint16_t* w1, float* b1, uint16_t
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t cellsKLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, const uint8_t* cls, uint8_t* labels,
-                        uint32_t mode, hipStream_t st) {
+                        uint32_t mode, uint32_t fallbackK, hipStream_t st) {
     if (K > 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_cells_k, dim3(c.n), dim3(64), 0, st, c, knn, K, cls, labels, mode);
+    hipLaunchKernelGGL(k_cells_k, dim3(c.n), dim3(64), 0, st, c, knn, K, cls, labels, mode, fallbackK);
     return hipGetLastError();
 }
 
@@ -388,11 +389,11 @@ hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, flo
 }
 
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
-                          float cost, hipStream_t st) {
+                          float cost, uint32_t fallbackK, hipStream_t st) {
     const uint32_t tiles = (n + 15u) / 16u;
     const uint32_t grid = tiles < 8192u ? tiles : 8192u;
     hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64), 0, st, feat, n, w.w1, w.b1, w.w2, w.b2, w.w3, w.b3, cls,
-                       logits, cost);
+                       logits, cost, fallbackK);
     return hipGetLastError();
 }
 

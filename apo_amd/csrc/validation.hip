@@ -424,7 +424,7 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
         }
         // searches whose sphere is large get a cover of the edge's empty
         // spheres, built by the whole wave one edge at a time
-        uint64_t bm = ballot(need && q.bs.w > kCoverRatio * kr);
+        uint64_t bm = ballot(need && q.bs.w > c.coverRatio * kr);
         while (bm) {
             const uint32_t t = uint32_t(__builtin_ctzll(bm));
             bm &= bm - 1;

@@ -47,6 +47,15 @@ typedef struct apo_config {
     uint32_t graph;           /* 1: capture the whole device build (LBVH to vertex output) in a HIP graph
                                  and replay it while the site pointer, site and molecule counts and this
                                  config stay the same; stats then hold ms_total only */
+    float cover_ratio;        /* edge validation: a search whose sphere exceeds cover_ratio x the cell's known
+                                 radius is pruned by a cover of the edge's empty spheres (exact pruning: the
+                                 results do not depend on it); < 0 = never.  Default 2 */
+    uint32_t waves_cells;     /* waves per CU of each persistent grid (cell pass, vertex, edge and bisector */
+    uint32_t waves_vertex;    /* validation); 0 = waves_per_cu when that is set, else full occupancy */
+    uint32_t waves_edge;
+    uint32_t waves_bisector;
+    uint32_t proposal_fallback_k; /* learned proposal: neighbours for sites of the last class (not certified
+                                     within 64): 16, 24, 32, 48 or 64; default 16 */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,
@@ -117,6 +126,12 @@ apo_status apo_build_batched_device(apo_ctx* ctx, const float* xyzr_device, cons
 /* First output vertex of each molecule of the last build (molecules + 1 values). */
 apo_status apo_batch_vertex_offsets(apo_ctx* ctx, uint64_t* offsets);
 
+/* out5[0..3]: waves per CU at full occupancy of the persistent cell,
+ * vertex, edge and bisector kernels on the context's device (the largest
+ * useful waves_* settings); out5[4]: its CU count.  No counterpart in the
+ * reference; read by the autotuner. */
+apo_status apo_launch_occupancy(apo_ctx* ctx, uint32_t* out5);
+
 /* Vertex output of AlgorithmGPU<K>::toHost (algorithm.inl:1596-1646):
  * each Apollonius vertex once, from the cell of its smallest site,
  * centre + empty-sphere radius in FP64, the 4 tangent sites as original
@@ -161,7 +176,8 @@ apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_
  * neighbours the cell pass runs before the plain certification test
  * (algorithm.inl:197-201); cells the test does not certify go through the
  * plain validation.  Classes ask for 20, 24, 28, 32, 40, 48, 64
- * neighbours; class 7 (not certified within 64) runs 16. ---- */
+ * neighbours; class 7 (not certified within 64) runs
+ * config.proposal_fallback_k (16 by default). ---- */
 #define APO_PROPOSAL_FEATURES 52
 #define APO_PROPOSAL_CLASSES 8
 #define APO_PROPOSAL_PARAMS 8072 /* W1[52][64] b1[64] W2[64][64] b2[64] W3[64][8] b3[8], weights [in][out] */

@@ -633,10 +633,10 @@ struct Context {
 };
 
 // Neighbours asked for by each class of the learned proposal (mirrors the
-// product's mapping; class 7, not certified within 64, runs 16).
-inline uint32_t proposalK(uint32_t c) {
-    static const uint32_t k[8] = {20, 24, 28, 32, 40, 48, 64, 16};
-    return c < 8 ? k[c] : 64;
+// product's mapping; class 7, not certified within 64, runs `fallback`).
+inline uint32_t proposalK(uint32_t c, uint32_t fallback) {
+    static const uint32_t k[7] = {20, 24, 28, 32, 40, 48, 64};
+    return c < 7 ? k[c] : fallback;
 }
 
 inline D4 vertexPosition(const Context& C, const D4& si, const Vtx& v) {
@@ -1345,7 +1345,7 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
                     k = cell.E[eid].j;
                     l = cell.E[eid].k;
                 }
-                EdgeQuery q{{0, 0, 0, 0}, false, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                EdgeQuery q{};
                 if (needSearch) {
                     const D4 sk = C.S[k], sl = C.S[l];
                     D4 ev[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -1664,6 +1664,7 @@ void orc_default_config(orc_config* cfg) {
     cfg->threads = 0;
     cfg->stage_mask = 7;
     cfg->edge_bound = 0;
+    cfg->fallback_k = 16;
 }
 
 static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {
@@ -1700,7 +1701,7 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
     C.flags.assign(n, 0);
     if (classes) {
         C.siteK.resize(n);
-        for (uint32_t i = 0; i < n; i++) C.siteK[i] = proposalK(classes[d->bvh.indices[i]]);
+        for (uint32_t i = 0; i < n; i++) C.siteK[i] = proposalK(classes[d->bvh.indices[i]], cfg.fallback_k);
         cfg.k = 64;
         C.cfg.k = 64;
     }

@@ -77,6 +77,7 @@ typedef struct orc_config {
     uint32_t edge_bound;        /* edge searches pruned by the cover of the edge's empty spheres: 0 never
                                    (the reference's search), 1 when the search sphere exceeds 2x the known
                                    radius, 2 always; results are identical in every mode */
+    uint32_t fallback_k;        /* learned proposal: neighbours of the last class (not certified within 64), 16 */
 } orc_config;
 
 void orc_default_config(orc_config* cfg);
@@ -86,7 +87,7 @@ typedef struct orc_diagram orc_diagram;
 orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfg);
 /* The learned-proposal path given per-site classes (input order): 64-nearest
  * lists, each cell runs the neighbour count of its class (20, 24, 28, 32,
- * 40, 48, 64; class 7: 16) before certification and validation. */
+ * 40, 48, 64; class 7: cfg->fallback_k) before certification and validation. */
 orc_diagram* orc_build_proposal(const float* xyzr, uint32_t n, const orc_config* cfg, const uint8_t* classes);
 void         orc_free(orc_diagram* d);
 

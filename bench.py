@@ -5,7 +5,9 @@ Workload: 1,000,000 sites, centres uniform in [0,1]^3, radii log-uniform in
 [1e-4, 1e-2] (FP32, seed 3).  One step = one full construction (scene box,
 LBVH, kNN, cell update, vertex/edge/bisector validation, vertex output
 compaction) from sites resident in HBM to the compacted vertex buffer in
-HBM.  value = cells per second over the timed steps.
+HBM.  value = cells per second over the timed steps, each one replay of the
+build captured in a HIP graph; the eager steps before them give the stage
+split and the per-kernel times of the roofline.
 
 Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
 """
@@ -36,7 +38,8 @@ def parse():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=60_000, help="sites in the CPU-baseline sub-cube")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
-    ap.add_argument("--proposal-cost", type=float, default=400.0)
+    ap.add_argument("--proposal-cost", type=float, default=None, help="override the tuned proposal cost")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 5M-site learned evaluation")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-molecules workload")
     ap.add_argument("--batch-molecules", type=int, default=512)
     return ap.parse_args()
@@ -49,16 +52,12 @@ def _vertices(ctx, L, _lib, v):
     return xyzw, ids
 
 
-def graph_replay(dev, n, k, steps, warmup, say):
+def graph_replay(dev, n, tuning, steps, warmup, say):
     """The same build captured once in a HIP graph and replayed: one graph
     launch per step, bracketed by device synchronisations."""
     import torch
     from apo_amd import _lib
-    cfg = _lib.Config()
-    _lib.lib().apo_default_config(_lib.byref(cfg))
-    cfg.k = k
-    cfg.graph = 1
-    ctx = _lib.create(cfg)
+    ctx = _ctx(tuning, graph=True)
     L = _lib.lib()
     ptr = _lib.ctypes.c_void_p(dev.data_ptr())
     for _ in range(max(warmup, 1)):  # the first build captures the graph
@@ -98,11 +97,12 @@ def batch_workload(count, steps, warmup, say):
     ptr = _lib.ctypes.c_void_p(dev.data_ptr())
     off = np.ascontiguousarray(offsets, np.uint32)
 
+    from apo_amd import autotune
+    rec = autotune.load("batch_128")
+    tuning = dict(rec["plain"]) if rec else {}
+
     def make(graph):
-        cfg = _lib.Config()
-        L.apo_default_config(_lib.byref(cfg))
-        cfg.graph = 1 if graph else 0
-        return _lib.create(cfg)
+        return _ctx(tuning, graph=graph)
 
     # eager build: stage split
     ctx = make(False)
@@ -162,17 +162,35 @@ def batch_workload(count, steps, warmup, say):
             "ms_per_step": ms, "device_ms_per_replay": dev_ms / steps, "steps": steps, "path": "HIP graph replay",
             "eager_stage_ms": stage, "vertices": v_eager, "vertices_per_s": v_eager / (ms / 1000.0),
             "graph_equals_eager": same_graph, "spot_molecules": picks, "spot_equal_single_build": matched,
-            "empty_sphere_violations": int(cnt.value),
+            "empty_sphere_violations": int(cnt.value), "config": tuning,
             "note": "violations are the reference method's own on overlapping atoms (counted, as its precision check does)"}
 
 
-def learned_comparison(sites, dev, n, k, steps, warmup, cost, plain_ids, say):
-    """Train the neighbour-count proposal on C5-style labels (two 1M
-    log-uniform sets, seeds 501/502, untimed), then time the same workload
-    with it.  Cells the plain test does not certify after the proposed count
-    go through the plain validation."""
-    import torch
-    from apo_amd import _lib, datasets, proposal
+def _ctx(tuning, proposal=None, graph=False):
+    """A context with the default config plus the tuned fields."""
+    from apo_amd import _lib
+    cfg = _lib.Config()
+    _lib.lib().apo_default_config(_lib.byref(cfg))
+    for key, val in (tuning or {}).items():
+        setattr(cfg, key, val)
+    cfg.graph = 1 if graph else 0
+    ctx = _lib.create(cfg)
+    if proposal is not None:
+        p32 = np.ascontiguousarray(proposal, np.float32)
+        _lib.check(ctx, _lib.lib().apo_proposal_set(ctx, _lib.fptr(p32)))
+    return ctx
+
+
+def _row_keys(ids):
+    """Vertex tangent-site quadruples as sortable 16-byte keys."""
+    a = np.ascontiguousarray(np.sort(ids, axis=1))
+    return a.view(np.dtype((np.void, 16))).ravel()
+
+
+def train_proposal(say):
+    """Labels from the plain method on two 1M log-uniform sets (seeds
+    501/502: 2M labelled sites) and 8 epochs of the HIP training kernels."""
+    from apo_amd import datasets, proposal
     t0 = time.perf_counter()
     fa, la = proposal.labels(datasets.loguniform_sites(1_000_000, seed=501))
     fb, lb = proposal.labels(datasets.loguniform_sites(1_000_000, seed=502))
@@ -183,14 +201,17 @@ def learned_comparison(sites, dev, n, k, steps, warmup, cost, plain_ids, say):
     t_train = time.perf_counter() - t0
     say(f"proposal trained on 2M labelled sites: labels {t_labels:.1f} s, training {t_train:.1f} s, "
         f"loss {loss[0]:.3f} -> {loss[-1]:.3f}")
-    cfg = _lib.Config()
-    _lib.lib().apo_default_config(_lib.byref(cfg))
-    cfg.k = k
-    cfg.proposal_cost = cost
-    ctx = _lib.create(cfg)
+    return params, {"labelled_sites": int(len(la) + len(lb)), "epochs": 8, "loss_first_epoch": float(loss[0]),
+                    "loss_last_epoch": float(loss[-1]), "label_s": t_labels, "train_s": t_train}
+
+
+def run_learned(params, dev, n, tuning, steps, warmup, plain_keys):
+    """Time the workload with the proposal.  Cells the plain test does not
+    certify after the proposed count go through the plain validation."""
+    import torch
+    from apo_amd import _lib
+    ctx = _ctx(tuning, proposal=params)
     L = _lib.lib()
-    p32 = np.ascontiguousarray(params, np.float32)
-    _lib.check(ctx, L.apo_proposal_set(ctx, _lib.fptr(p32)))
     ptr = _lib.ctypes.c_void_p(dev.data_ptr())
     for _ in range(warmup):
         _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
@@ -204,26 +225,54 @@ def learned_comparison(sites, dev, n, k, steps, warmup, cost, plain_ids, say):
         wall += time.perf_counter() - t0
     _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
     v = int(s.vertices)
-    xyzw = np.zeros((v, 4), np.float64)
-    ids = np.zeros((v, 4), np.uint32)
-    _lib.check(ctx, L.apo_get_vertices(ctx, _lib.dptr(xyzw), _lib.uptr(ids), v))
+    _, ids = _vertices(ctx, L, _lib, v)
     cnt = _lib.c_uint64()
     _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
-    learned_ids = {tuple(r) for r in np.sort(ids, axis=1)}
     L.apo_destroy(ctx)
+    keys = _row_keys(ids)
     ms = 1000.0 * wall / steps
     non_buried = n - int(s.buried)
-    return {"cells_per_s": n / (ms / 1000.0), "ms_per_step": ms, "proposal_cost": cost,
+    return {"cells_per_s": n / (ms / 1000.0), "ms_per_step": ms, "config": dict(tuning or {}),
             "cells_certified_by_cell_pass": int(s.cells_certified),
             "fallback_rate": 1.0 - int(s.cells_certified) / max(non_buried, 1),
             "stage_ms": {"knn": s.ms_knn, "proposal": s.ms_proposal, "cells": s.ms_cells,
                          "validation": s.ms_validation, "bisector": s.ms_bisector},
-            "vertices": v, "vertices_only_in_plain": len(plain_ids - learned_ids),
-            "vertices_only_in_learned": len(learned_ids - plain_ids),
-            "empty_sphere_violations": int(cnt.value),
-            "training": {"labelled_sites": int(len(la) + len(lb)), "epochs": 8,
-                         "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1]),
-                         "label_s": t_labels, "train_s": t_train}}
+            "vertices": v, "vertices_only_in_plain": int(len(np.setdiff1d(plain_keys, keys))),
+            "vertices_only_in_learned": int(len(np.setdiff1d(keys, plain_keys))),
+            "empty_sphere_violations": int(cnt.value)}
+
+
+def c5_evaluation(params, plain_tuning, learned_tuning, say, n=5_000_000):
+    """Inference + certified construction on 5M sites (uniform centres,
+    log-uniform radii [1e-4, 1e-2], seed 55) with the proposal trained on
+    the 2M labelled sites, beside the plain path on the same set."""
+    import torch
+    from apo_amd import _lib, datasets
+    sites = datasets.loguniform_sites(n, seed=55)
+    dev = torch.from_numpy(sites).to("cuda")
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    ptr = _lib.ctypes.c_void_p(dev.data_ptr())
+    ctx = _ctx(plain_tuning)
+    _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+    torch.cuda.synchronize()
+    ms_plain = 1000.0 * (time.perf_counter() - t0)
+    s = _lib.Stats()
+    _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+    _, ids = _vertices(ctx, L, _lib, int(s.vertices))
+    L.apo_destroy(ctx)
+    say(f"C5 plain: {ms_plain:.1f} ms for {n} sites")
+    lr = run_learned(params, dev, n, learned_tuning, 1, 1, _row_keys(ids))
+    say(f"C5 learned: {lr['ms_per_step']:.1f} ms, fallback rate {lr['fallback_rate']:.3f}")
+    lr["plain_cells_per_s"] = n / (ms_plain / 1000.0)
+    lr["speedup_vs_plain"] = lr["cells_per_s"] / lr["plain_cells_per_s"]
+    lr["workload"] = f"{n} sites, centres U[0,1]^3, radii log-uniform [1e-4,1e-2], seed 55"
+    del dev
+    torch.cuda.empty_cache()
+    return lr
 
 
 def cpu_baseline(sites: np.ndarray, target: int):
@@ -261,10 +310,13 @@ def main():
     dev = torch.from_numpy(sites).to("cuda")
     torch.cuda.synchronize()
 
-    cfg = _lib.Config()
-    _lib.lib().apo_default_config(_lib.byref(cfg))
-    cfg.k = args.k
-    ctx = _lib.create(cfg)
+    from apo_amd import autotune
+    tuned = autotune.load("loguniform_1m") if n == 1_000_000 and args.k == 16 else None
+    plain_tuning = dict(tuned["plain"]) if tuned else {}
+    learned_tuning = dict(tuned.get("learned", tuned["plain"])) if tuned else {"proposal_cost": 400.0}
+    plain_tuning["k"] = args.k
+    learned_tuning["k"] = args.k
+    ctx = _ctx(plain_tuning)
     L = _lib.lib()
     ptr = _lib.ctypes.c_void_p(dev.data_ptr())
 
@@ -286,6 +338,9 @@ def main():
             "k_bisector_validation": [0.0, 0, 0]}
     s = _lib.Stats()
     wall = 0.0
+    # every build launches each validation kernel once per round (rounds
+    # whose task list is empty end at once)
+    rounds = 10
     for _ in range(args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -294,9 +349,8 @@ def main():
         wall += time.perf_counter() - t0
         _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
         for name, ms, b, launches in (
-                ("k_edge_validation", s.ms_edge_validation_kernel, s.bytes_edge_validation, s.validation_launches),
-                ("k_vertex_validation", s.ms_vertex_validation_kernel, s.bytes_vertex_validation,
-                 s.validation_launches),
+                ("k_edge_validation", s.ms_edge_validation_kernel, s.bytes_edge_validation, rounds),
+                ("k_vertex_validation", s.ms_vertex_validation_kernel, s.bytes_vertex_validation, rounds),
                 ("k_cells", s.ms_cells, s.bytes_cells, 1),
                 ("k_bisector_validation", s.ms_bisector, s.bytes_bisector_validation, 1)):
             kern[name][0] += ms
@@ -343,22 +397,28 @@ def main():
         _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
         violations = int(cnt.value)
 
+    say("graph replay of the same build (the timed steps of value)")
+    gr = graph_replay(dev, n, plain_tuning, args.steps, args.warmup, say)
     out = {
         "metric": "cells/sec at 1M sites: learned-k vs plain on 1 GPU, 0 empty-sphere violations",
-        "value": n / (ms_per_step / 1000.0),
+        "value": gr["cells_per_s"],
         "unit": "cells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": gr["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{n} sites, centres U[0,1]^3, radii log-uniform [1e-4,1e-2], seed 3, plain k={args.k}",
-                   "path": "plain (the learned-k proposal is reported under learned)", "sites": n, "k": args.k},
-        "vertices_per_s": vertices / (ms_per_step / 1000.0),
+                   "path": "plain, one HIP graph replay per step (the learned-k proposal is reported under "
+                           "learned)", "sites": n, "k": args.k},
+        "vertices_per_s": vertices / (gr["ms_per_step"] / 1000.0),
+        "graph_replay": gr,
+        "eager": {"cells_per_s": n / (ms_per_step / 1000.0), "ms_per_step": ms_per_step,
+                  "note": "stage events and per-kernel events on; the roofline figures come from these steps"},
         "vertices": vertices,
         "stage_ms": stage,
         "buried_sites": int(s.buried),
@@ -371,17 +431,23 @@ def main():
                      "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
     }
-    say("graph replay of the same build")
-    out["graph_replay"] = graph_replay(dev, n, args.k, args.steps, args.warmup, say)
+    out["tuning"] = {"source": autotune.cache_path("loguniform_1m") if tuned else None,
+                     "plain": plain_tuning, "learned": learned_tuning}
     if not args.no_learned:
-        xyzw_p = np.zeros((vertices, 4), np.float64)
-        ids_p = np.zeros((vertices, 4), np.uint32)
-        _lib.check(ctx, L.apo_get_vertices(ctx, _lib.dptr(xyzw_p), _lib.uptr(ids_p), vertices))
-        plain_ids = {tuple(r) for r in np.sort(ids_p, axis=1)}
+        _, ids_p = _vertices(ctx, L, _lib, vertices)
+        plain_keys = _row_keys(ids_p)
         say("learned neighbour-count proposal: training, then the same workload")
-        lc = learned_comparison(sites, dev, n, args.k, args.steps, args.warmup, args.proposal_cost, plain_ids, say)
-        lc["speedup_vs_plain"] = lc["cells_per_s"] / out["value"]
+        params, training = train_proposal(say)
+        ltune = dict(learned_tuning)
+        if args.proposal_cost is not None:
+            ltune["proposal_cost"] = args.proposal_cost
+        lc = run_learned(params, dev, n, ltune, args.steps, args.warmup, plain_keys)
+        lc["training"] = training
+        lc["speedup_vs_plain"] = lc["cells_per_s"] / out["eager"]["cells_per_s"]
         out["learned"] = lc
+        if not args.no_c5:
+            say("C5: the proposal on a 5M-site set")
+            out["learned_5m"] = c5_evaluation(params, plain_tuning, ltune, say)
     if not args.no_batch:
         out["batch"] = batch_workload(args.batch_molecules, min(args.steps, 3), 1, say)
     if not args.no_cpu_baseline:

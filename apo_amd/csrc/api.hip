@@ -243,6 +243,7 @@ CellCtx cellCtx(apo_ctx* ctx) {
     c.sorted = ctx->sorted;
     c.scene = ctx->scene;
     c.nodes = ctx->nodes;
+    c.parent = ctx->parent;
     c.status = ctx->status;
     c.knownRadii = ctx->knownRadii;
     c.flags = ctx->flags;

@@ -100,6 +100,7 @@ struct CellCtx {
     const F4* sorted;
     const Scene* scene;     // per molecule; artificial sites in scene[m].art
     const Node* nodes;
+    const int32_t* parent;  // parent of every node (-1 at a root)
     uint8_t* status;
     float* knownRadii;
     uint32_t* flags;

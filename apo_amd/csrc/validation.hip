@@ -37,6 +37,9 @@ struct EdgeLds {
         CoverLds cover;  // edge covers are built while no traversal runs
     };
     CoverOut coverOut;
+    uint32_t share[64];   // donor lane of each receiving rank (work sharing)
+    uint32_t res[64];     // per owner lane: earliest contributor found by a helper
+    uint32_t settled[64]; // per owner lane: its own first item found the contributor
 };
 struct BisectorLds {
     CellLds cell;
@@ -45,10 +48,16 @@ struct BisectorLds {
 };
 static_assert(sizeof(CoverLds) <= sizeof(uint32_t) * kStackDepth * 64, "cover buffers overlay the stack");
 
-// Traversal state of one lane: next node and stack height.
+// Traversal state of one lane: next node and stack height.  With work
+// sharing (edge searches) the stack occupies slots [sb, sp): the slots
+// below sb were given to other lanes, and `base` is the depth the item's
+// stack bottom has in the reference's single-stack traversal (its overflow
+// test uses that depth, so a shared search overflows where the reference's
+// would).
 struct Trav {
     uint32_t node, sp;
     bool live;
+    uint32_t sb = 0, base = 0;
 };
 
 __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
@@ -229,7 +238,7 @@ struct EdgeQuery {
 
 // Pop the next node of a lane's depth-first traversal, or end it.
 __device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
-    if (t.sp == 0) {
+    if (t.sp == t.sb) {
         t.live = false;
     } else {
         t.sp--;
@@ -293,7 +302,7 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, uint32_t (*stNode
         if (ec || ef) {
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
-                if (t.sp < kStackDepth) {
+                if (t.base + t.sp < kStackDepth) {
                     stNode[t.sp][lane] = farChild;
                     t.sp++;
                 } else {
@@ -353,6 +362,181 @@ __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L
         // appended vertices, status and counts written
         countBytes(c, family, 29u + 16u * uint64_t(edgesLoaded + w.E + (total - baseV)) + 9u);
     }
+}
+
+// Is leaf a visited before leaf b by a closest-child-first traversal for
+// the search sphere bs?  Both leaves hang below their lowest common
+// ancestor, whose closer child (by box distance to bs, the traversal's own
+// order test) is visited first.
+__device__ __forceinline__ bool visitedBefore(const CellCtx& c, uint32_t a, uint32_t b, F4 bs) {
+    constexpr uint32_t kMaxDepth = 4096;  // bounds every climb, whatever the tree holds
+    uint32_t na = c.internal + a, nb = c.internal + b;
+    uint32_t da = 0, db = 0;
+    for (int32_t p = c.parent[na]; p >= 0 && da < kMaxDepth; p = c.parent[p]) da++;
+    for (int32_t p = c.parent[nb]; p >= 0 && db < kMaxDepth; p = c.parent[p]) db++;
+    if (da >= kMaxDepth || db >= kMaxDepth) {
+        atomicOr(c.err, kErrValidation);
+        return a < b;
+    }
+    while (da > db) {
+        na = uint32_t(c.parent[na]);
+        da--;
+    }
+    while (db > da) {
+        nb = uint32_t(c.parent[nb]);
+        db--;
+    }
+    for (uint32_t guard = 0; guard <= kMaxDepth; guard++) {
+        const int32_t pa = c.parent[na], pb = c.parent[nb];
+        if (pa == pb || pa < 0 || pb < 0) {
+            if (pa != pb || pa < 0) return a < b;  // different trees cannot happen for one query
+            const Node nn = c.nodes[pa];
+            Box lb, rb;
+            nodeBoxes(nn, lb, rb);
+            const bool leftFirst = boxDistance(lb, bs) < boxDistance(rb, bs);
+            const bool aLeft = uint32_t(nn.d.x) == na;
+            return aLeft == leftFirst;
+        }
+        na = uint32_t(pa);
+        nb = uint32_t(pb);
+    }
+    atomicOr(c.err, kErrValidation);
+    return a < b;
+}
+
+__device__ __forceinline__ F4 shflF4(F4 v, uint32_t src) {
+    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64), __shfl(v.w, int(src), 64)};
+}
+__device__ __forceinline__ F3 shflF3(F3 v, uint32_t src) {
+    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64)};
+}
+__device__ __forceinline__ D4 shflD4(D4 v, uint32_t src) {
+    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64), __shfl(v.w, int(src), 64)};
+}
+
+// Lanes with no search left take one when at least this many are idle.
+constexpr uint32_t kShareIdle = 4;
+
+// The contributor searches of all lanes' edges (algorithm.inl:616-793),
+// each returning its first contributor in the reference's traversal order.
+//
+// Work sharing: a lane whose search is over takes the bottom entry of a
+// busy lane's stack -- the subtree that lane would visit last -- together
+// with a copy of that search's query, and searches it with a stack whose
+// depth counts from that entry's depth (so overflow happens where the
+// reference's would).  Everything a lane still holds precedes everything
+// it gave away, so the owner lane's own item is always the earliest part of
+// its search: a contributor found there is the answer and settles the
+// search.  A contributor found in a shared item ends that item and is kept
+// per owner when it precedes the one kept so far (visitedBefore); it is
+// the answer if the owner's own item finds none.
+__device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, const CellLds& L, const WaveCell& w,
+                                              uint32_t i, const D4& si, uint32_t root, bool need, uint32_t k,
+                                              uint32_t l, const EdgeQuery& q, uint32_t& flags) {
+    const uint32_t lane = laneId();
+    S.res[lane] = kNone;
+    S.settled[lane] = 0u;
+    // the item this lane works on: the query it serves and its owner
+    EdgeQuery wq = q;
+    uint32_t wk = k, wl = l, owner = need ? lane : kNone;
+    bool own = need;  // the owner's own (earliest) item
+    D4 wsk{0, 0, 0, 0}, wsl{0, 0, 0, 0};
+    if (need) {
+        wsk = loadSite(c, k);
+        wsl = loadSite(c, l);
+    }
+    Trav tr{root, 0u, need};
+    uint32_t found = kNone;   // contributor of this lane's own item
+    uint32_t cand = kNone;    // candidate waiting for the wave's test
+    waveSync();
+    while (true) {
+        // a shared item of a settled search has nothing left to find
+        if (tr.live && !own && owner != kNone && S.settled[owner]) tr.live = false;
+        const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
+        if (stepping == 0 && waiting == 0) break;
+        if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
+            // test the waiting candidates together: not yet a neighbour,
+            // two vertex solutions, one clear of the neighbours
+            uint32_t shared = kNone;
+            if (cand != kNone) {
+                bool ok = !knownLane(L, w.F, cand);
+                if (ok) {
+                    D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
+                    uint32_t mask = quadrisector(si, loadSite(c, cand), wsk, wsl, x0, x1);
+                    if (mask != 3u) mask = 0u;
+                    uint32_t valid = mask;
+                    if ((mask & 1u) && !clearOfNeighbours(c, L, w.F, x0, wk, wl)) valid &= ~1u;
+                    if ((mask & 2u) && !clearOfNeighbours(c, L, w.F, x1, wk, wl)) valid &= ~2u;
+                    ok = valid != 0u;
+                }
+                if (ok) {
+                    if (own) {
+                        found = cand;
+                        S.settled[lane] = 1u;
+                    } else {
+                        shared = cand;
+                    }
+                    tr.live = false;
+                }
+                cand = kNone;
+            }
+            // keep the earliest contributor of the shared items, per owner
+            uint64_t hm = ballot(shared != kNone);
+            while (hm) {
+                const uint32_t t = uint32_t(__builtin_ctzll(hm));
+                hm &= hm - 1;
+                if (lane == t) {
+                    const uint32_t cur = S.res[owner];
+                    if (cur == kNone || visitedBefore(c, shared, cur, wq.bs)) S.res[owner] = shared;
+                }
+                waveSync();
+            }
+            waveSync();
+            continue;
+        }
+        // idle lanes take the last subtree of busy lanes' stacks
+        const uint64_t idle = ballot(!tr.live && cand == kNone);
+        const uint64_t donors = ballot(tr.live && cand == kNone && tr.sp > tr.sb);
+        if (popc64(idle) >= kShareIdle && donors != 0) {
+            const uint32_t m = popc64(idle) < popc64(donors) ? popc64(idle) : popc64(donors);
+            const uint32_t dr = countBelow(donors), ir = countBelow(idle);
+            const bool give = ((donors >> lane) & 1u) && dr < m;
+            const bool take = ((idle >> lane) & 1u) && ir < m;
+            if (give) S.share[dr] = lane;
+            waveSync();
+            const uint32_t src = take ? S.share[ir] : lane;
+            const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
+            const uint32_t dOwner = __shfl(owner, int(src), 64);
+            const uint32_t node = take ? S.stNode[dSb][src] : 0u;
+            wq.bs = shflF4(wq.bs, src);
+            wq.hyperbola = __shfl(wq.hyperbola ? 1 : 0, int(src), 64) != 0;
+            wq.n1 = shflF3(wq.n1, src);
+            wq.n2 = shflF3(wq.n2, src);
+            wq.pp1 = shflF3(wq.pp1, src);
+            wq.pp2 = shflF3(wq.pp2, src);
+            wq.cover = __shfl(wq.cover, int(src), 64);
+#pragma unroll
+            for (int b = 0; b < kCoverBalls; b++) wq.ball[b] = shflF4(wq.ball[b], src);
+            wk = __shfl(wk, int(src), 64);
+            wl = __shfl(wl, int(src), 64);
+            wsk = shflD4(wsk, src);
+            wsl = shflD4(wsl, src);
+            if (take) {
+                owner = dOwner;
+                own = false;
+                tr.node = node;
+                tr.sp = 0u;
+                tr.sb = 0u;
+                tr.base = dBase + dSb;
+                tr.live = true;
+            }
+            if (give) tr.sb++;
+            waveSync();
+        }
+        if (tr.live && cand == kNone) cand = edgeStep(c, S.stNode, i, wk, wl, wq, tr, flags);
+    }
+    waveSync();
+    return need && found == kNone ? S.res[lane] : found;
 }
 
 __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t i, uint32_t maxRestarts) {
@@ -438,44 +622,8 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
         }
         // the first contributor of each lane's traversal: a candidate with two
         // vertex solutions of which one is clear of the cell's neighbours
-        uint32_t found = kNone;
-        {
-            Trav tr{root, 0u, need};
-            D4 sk{0, 0, 0, 0}, sl{0, 0, 0, 0};
-            if (need) {
-                sk = loadSite(c, k);
-                sl = loadSite(c, l);
-            }
-            uint32_t cand = kNone;  // candidate waiting for the wave's test
-            while (true) {
-                const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
-                if (stepping == 0 && waiting == 0) break;
-                if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
-                    // test the waiting candidates together: not yet a neighbour,
-                    // two vertex solutions, one clear of the neighbours
-                    if (cand != kNone) {
-                        bool ok = !knownLane(L, w.F, cand);
-                        if (ok) {
-                            D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
-                            uint32_t mask = quadrisector(si, loadSite(c, cand), sk, sl, x0, x1);
-                            if (mask != 3u) mask = 0u;
-                            uint32_t valid = mask;
-                            if ((mask & 1u) && !clearOfNeighbours(c, L, w.F, x0, k, l)) valid &= ~1u;
-                            if ((mask & 2u) && !clearOfNeighbours(c, L, w.F, x1, k, l)) valid &= ~2u;
-                            ok = valid != 0u;
-                        }
-                        if (ok) {
-                            found = cand;
-                            tr.live = false;
-                        }
-                        cand = kNone;
-                    }
-                    continue;
-                }
-                if (tr.live && cand == kNone) cand = edgeStep(c, S.stNode, i, k, l, q, tr, flags);
-            }
-            if (need && found == kNone) L.edg[e].w = 1u;
-        }
+        const uint32_t found = edgeSearches(c, S, L, w, i, si, root, need, k, l, q, flags);
+        if (need && found == kNone) L.edg[e].w = 1u;
         const uint64_t m = ballot(found != kNone);
         waveSync();
         bool changed = false;

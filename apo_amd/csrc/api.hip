@@ -244,6 +244,7 @@ CellCtx cellCtx(apo_ctx* ctx) {
     c.scene = ctx->scene;
     c.nodes = ctx->nodes;
     c.parent = ctx->parent;
+    c.treeHeight = ctx->counters + kTreeHeight;
     c.status = ctx->status;
     c.knownRadii = ctx->knownRadii;
     c.flags = ctx->flags;
@@ -352,6 +353,8 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     }
     APO_TIMED_STAGE("cells");
     APO_HIP(mark(3));
+    // tree height: lets vertex validation skip a search that cannot overflow
+    APO_HIP(treeHeightLaunch(ctx->parent, n, n - ctx->molecules, ctx->counters + kTreeHeight, st));
     const ValidationBuffers vb{ctx->tasks, ctx->counters};
     const ValidationGrid vg{persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
                             persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge),

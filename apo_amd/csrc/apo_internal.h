@@ -89,6 +89,8 @@ struct LbvhBuffers {
 constexpr uint32_t kRefitLeaves = 2048;
 
 size_t sortScratchWords(uint32_t n);
+// Deepest leaf of the trees into *height (cleared beforehand; read by validation).
+hipError_t treeHeightLaunch(const int32_t* parent, uint32_t n, uint32_t internal, uint32_t* height, hipStream_t st);
 hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const Segments& seg, hipStream_t st);
 
 // Device view of the construction state.
@@ -101,6 +103,7 @@ struct CellCtx {
     const Scene* scene;     // per molecule; artificial sites in scene[m].art
     const Node* nodes;
     const int32_t* parent;  // parent of every node (-1 at a root)
+    const uint32_t* treeHeight;  // deepest leaf of the trees: a traversal stack never grows past it
     uint8_t* status;
     float* knownRadii;
     uint32_t* flags;
@@ -148,6 +151,7 @@ constexpr uint32_t kVertexQueue = 16;
 constexpr uint32_t kEdgeQueue = 24;
 constexpr uint32_t kRoundWords = kEdgeQueue + kXcdQueueWords - kTaskCount;  // cleared per round
 constexpr uint32_t kBisectorQueue = 32;
+constexpr uint32_t kTreeHeight = 40;   // deepest leaf of the build's trees (root = 0)
 constexpr uint32_t kRoundTasks = 64;
 constexpr uint32_t kMaxRounds = 256;
 constexpr uint32_t kCounterWords = kRoundTasks + kMaxRounds;

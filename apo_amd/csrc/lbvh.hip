@@ -628,9 +628,30 @@ __global__ __launch_bounds__(256) void k_copy_sorted(const uint64_t* __restrict_
     }
 }
 
+// Depth of the deepest leaf (root depth 0).  A depth-first traversal keeps
+// at most one stacked node per level of its current path, so no stack of
+// this depth or more can overflow.
+__global__ __launch_bounds__(256) void k_tree_height(const int32_t* __restrict__ parent, uint32_t n,
+                                                     uint32_t internal, uint32_t* height) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t d = 0;
+    if (i < n)
+        for (int32_t p = parent[internal + i]; p >= 0 && d < n; p = parent[p]) d++;
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t other = __shfl_xor(d, o, 64);
+        d = other > d ? other : d;
+    }
+    if ((threadIdx.x & 63u) == 0u) atomicMax(height, d);
+}
+
 // ---------------------------------------------------------------------------
 // host-side launcher
 // ---------------------------------------------------------------------------
+hipError_t treeHeightLaunch(const int32_t* parent, uint32_t n, uint32_t internal, uint32_t* height, hipStream_t st) {
+    hipLaunchKernelGGL(k_tree_height, dim3((n + 255u) / 256u), dim3(256), 0, st, parent, n, internal, height);
+    return hipGetLastError();
+}
+
 size_t sortScratchWords(uint32_t n) {
     const size_t tiles = (n + kTileKeys - 1) / kTileKeys;
     return size_t(kMaxRadixPasses) * 256u + kMaxRadixPasses + size_t(kMaxRadixPasses) * tiles * 256u + 1u;

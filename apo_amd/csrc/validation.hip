@@ -29,6 +29,8 @@ struct VertexLds {
     CellLds cell;
     uint32_t stNode[kStackDepth][64];
     float stDist[kStackDepth][64];
+    uint32_t share[64];   // donor lane of each receiving rank (work sharing)
+    uint32_t hit[64];     // per owner lane: a site came within the margin
 };
 struct EdgeLds {
     CellLds cell;
@@ -45,6 +47,9 @@ struct BisectorLds {
     CellLds cell;
     uint32_t stNode[kStackDepth][64];
     uint32_t bis[kBisectorCap];
+    uint32_t share[64];   // donor lane of each receiving rank (work sharing)
+    uint32_t res[64];     // per owner lane: earliest contributor found by a helper
+    uint32_t settled[64]; // per owner lane: its own first item found the contributor
 };
 static_assert(sizeof(CoverLds) <= sizeof(uint32_t) * kStackDepth * 64, "cover buffers overlay the stack");
 
@@ -59,6 +64,16 @@ struct Trav {
     bool live;
     uint32_t sb = 0, base = 0;
 };
+
+// Pop the next node of a lane's depth-first traversal, or end it.
+__device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
+    if (t.sp == t.sb) {
+        t.live = false;
+    } else {
+        t.sp--;
+        t.node = stNode[t.sp][laneId()];
+    }
+}
 
 __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
     const uint32_t lane = laneId();
@@ -132,7 +147,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
         if (ec || ef) {
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
-                if (t.sp < kStackDepth) {
+                if (t.base + t.sp < kStackDepth) {
                     stNode[t.sp][lane] = farChild;
                     stDist[t.sp][lane] = minFar;
                     t.sp++;
@@ -153,7 +168,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
         }
     }
     bool more = false;
-    while (t.sp > 0) {
+    while (t.sp > t.sb) {
         t.sp--;
         if (stDist[t.sp][lane] > nd) continue;
         t.node = stNode[t.sp][lane];
@@ -161,6 +176,204 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
         break;
     }
     if (!more) t.live = false;
+}
+
+// Is leaf a visited before leaf b by a closest-child-first traversal for
+// the search sphere bs?  Both leaves hang below their lowest common
+// ancestor, whose closer child (by box distance to bs, the traversal's own
+// order test) is visited first.
+__device__ __forceinline__ bool visitedBefore(const CellCtx& c, uint32_t a, uint32_t b, F4 bs) {
+    constexpr uint32_t kMaxDepth = 4096;  // bounds every climb, whatever the tree holds
+    uint32_t na = c.internal + a, nb = c.internal + b;
+    uint32_t da = 0, db = 0;
+    for (int32_t p = c.parent[na]; p >= 0 && da < kMaxDepth; p = c.parent[p]) da++;
+    for (int32_t p = c.parent[nb]; p >= 0 && db < kMaxDepth; p = c.parent[p]) db++;
+    if (da >= kMaxDepth || db >= kMaxDepth) {
+        atomicOr(c.err, kErrValidation);
+        return a < b;
+    }
+    while (da > db) {
+        na = uint32_t(c.parent[na]);
+        da--;
+    }
+    while (db > da) {
+        nb = uint32_t(c.parent[nb]);
+        db--;
+    }
+    for (uint32_t guard = 0; guard <= kMaxDepth; guard++) {
+        const int32_t pa = c.parent[na], pb = c.parent[nb];
+        if (pa == pb || pa < 0 || pb < 0) {
+            if (pa != pb || pa < 0) return a < b;  // different trees cannot happen for one query
+            const Node nn = c.nodes[pa];
+            Box lb, rb;
+            nodeBoxes(nn, lb, rb);
+            const bool leftFirst = boxDistance(lb, bs) < boxDistance(rb, bs);
+            const bool aLeft = uint32_t(nn.d.x) == na;
+            return aLeft == leftFirst;
+        }
+        na = uint32_t(pa);
+        nb = uint32_t(pb);
+    }
+    atomicOr(c.err, kErrValidation);
+    return a < b;
+}
+
+__device__ __forceinline__ F4 shflF4(F4 v, uint32_t src) {
+    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64), __shfl(v.w, int(src), 64)};
+}
+__device__ __forceinline__ F3 shflF3(F3 v, uint32_t src) {
+    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64)};
+}
+__device__ __forceinline__ D4 shflD4(D4 v, uint32_t src) {
+    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64), __shfl(v.w, int(src), 64)};
+}
+
+// Lanes with no search left take one when at least this many are idle.
+constexpr uint32_t kShareIdle = 4;
+
+// One step of a lane's range search: does a site the nearest-site search
+// could return (not i, not the vertex's sites, not buried, not a cell
+// neighbour) come within `reach` of the vertex sphere?  A site's gap is at
+// least its box's gap when the vertex centre lies outside the site, and
+// negative only when the centre lies inside it (and so inside every box
+// above it): a node is skipped when its box gap exceeds `boxReach` and the
+// centre is outside the box.  Order and stack are those of vertexStep.
+__device__ __forceinline__ bool vertexRangeStep(const CellCtx& c, const CellLds& L, uint32_t (*stNode)[64],
+                                                uint32_t F, uint32_t i, uint4 vx, F4 cur, float reach,
+                                                float boxReach, Trav& t, uint32_t& flags) {
+    const uint32_t lane = laneId();
+    const uint32_t node = t.node;
+    if (node < c.internal) {
+        const Node nn = c.nodes[node];
+        Box lb, rb;
+        nodeBoxes(nn, lb, rb);
+        const uint32_t L0 = uint32_t(nn.d.x), R0 = uint32_t(nn.d.y);
+        bool ef = false, ec = false;
+        uint32_t farChild = 0, closeChild = 0;
+        if (L0 >= nodeCount(c) || R0 >= nodeCount(c)) {
+            atomicOr(c.err, kErrKnnChild);
+        } else {
+            const F3 p = xyzf(cur);
+            const float eL = lenf(subf(p, closestPoint(lb, cur))), eR = lenf(subf(p, closestPoint(rb, cur)));
+            const bool inL = eL == 0.f || eL - cur.w <= boxReach, inR = eR == 0.f || eR - cur.w <= boxReach;
+            const bool lc = eL < eR;
+            farChild = lc ? R0 : L0;
+            closeChild = lc ? L0 : R0;
+            ef = lc ? inR : inL;
+            ec = lc ? inL : inR;
+        }
+        if (ec || ef) {
+            t.node = ec ? closeChild : farChild;
+            if (ec && ef) {
+                if (t.base + t.sp < kStackDepth) {
+                    stNode[t.sp][lane] = farChild;
+                    t.sp++;
+                } else {
+                    flags |= kFlagStackOverflow;
+                }
+            }
+            return false;
+        }
+        popNode(stNode, t);
+        return false;
+    }
+    bool hit = false;
+    if (node < nodeCount(c)) {
+        const uint32_t j = node - c.internal;
+        if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried)
+            hit = sphereDistF(c.sorted[j], cur) <= reach && !knownLane(L, F, j);
+    }
+    popNode(stNode, t);
+    return hit;
+}
+
+// The nearest-site searches of all lanes' vertices (algorithm.inl:305-414).
+//
+// The caller uses one thing of a search: whether its site intersects the
+// vertex sphere, and then which site.  So each vertex first asks a range
+// question, shared across the wave like the edge searches (idle lanes take
+// the last subtree of busy lanes' stacks): does any site the search could
+// return come within a rounding margin of the sphere?  If none does, the
+// site the reference's search returns cannot intersect (its FP32 gap
+// exceeds the margin, so its FP64 gap exceeds the intersection tolerance)
+// and the answer is "no intersection", whatever that site is.  A vertex
+// with a site inside the margin reruns the reference's own traversal.
+//
+// The shortcut also needs the reference's search not to overflow its stack
+// (the overflow is recorded in the cell's flags): it is taken only when no
+// leaf is deeper than the stack, which makes an overflow impossible.
+// Returns the site of the exact search, or kNone.
+__device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& S, const CellLds& L,
+                                                   const WaveCell& w, uint32_t i, uint32_t root, bool need,
+                                                   const uint4 vx0, const F4 cur0, uint32_t& flags) {
+    const uint32_t lane = laneId();
+    const float reach = 1e-5f * (__builtin_fabsf(cur0.x) + __builtin_fabsf(cur0.y) + __builtin_fabsf(cur0.z) +
+                                 __builtin_fabsf(cur0.w)) + 1e-6f;
+    bool exact = need;
+    if (__builtin_amdgcn_readfirstlane(*c.treeHeight) <= kStackDepth) {
+        S.hit[lane] = 0u;
+        uint4 vx = vx0;
+        F4 cur = cur0;
+        float wreach = reach;
+        uint32_t owner = need ? lane : kNone, rflags = 0;
+        Trav tr{root, 0u, need};
+        waveSync();
+        while (true) {
+            if (tr.live && S.hit[owner]) tr.live = false;  // the owner already has its answer
+            const uint64_t live = ballot(tr.live);
+            if (live == 0) break;
+            const uint64_t idle = ~live;
+            const uint64_t donors = ballot(tr.live && tr.sp > tr.sb);
+            if (popc64(idle) >= kShareIdle && donors != 0) {
+                const uint32_t m = popc64(idle) < popc64(donors) ? popc64(idle) : popc64(donors);
+                const uint32_t dr = countBelow(donors), ir = countBelow(idle);
+                const bool give = ((donors >> lane) & 1u) && dr < m;
+                const bool take = ((idle >> lane) & 1u) && ir < m;
+                if (give) S.share[dr] = lane;
+                waveSync();
+                const uint32_t src = take ? S.share[ir] : lane;
+                const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
+                const uint32_t dOwner = __shfl(owner, int(src), 64);
+                const uint32_t node = take ? S.stNode[dSb][src] : 0u;
+                cur = shflF4(cur, src);
+                wreach = __shfl(wreach, int(src), 64);
+                vx.x = __shfl(vx.x, int(src), 64);
+                vx.y = __shfl(vx.y, int(src), 64);
+                vx.z = __shfl(vx.z, int(src), 64);
+                if (take) {
+                    owner = dOwner;
+                    tr.node = node;
+                    tr.sp = 0u;
+                    tr.sb = 0u;
+                    tr.base = dBase + dSb;
+                    tr.live = true;
+                }
+                if (give) tr.sb++;
+                waveSync();
+            }
+            bool hit = false;
+            if (tr.live) hit = vertexRangeStep(c, L, S.stNode, w.F, i, vx, cur, wreach, 2.f * wreach, tr, rflags);
+            // an overflow cannot happen below the height bound; were it to, the
+            // search is rerun exactly
+            if (rflags) hit = true;
+            if (hit) {
+                tr.live = false;
+                S.hit[owner] = 1u;  // idempotent store: concurrent writers agree
+            }
+            waveSync();
+        }
+        exact = need && S.hit[lane] != 0u;
+    }
+    // the reference's traversal for the vertices that need its site
+    uint32_t nb = kNone;
+    if (ballot(exact) != 0) {
+        Trav t2{root, 0u, exact};
+        float nd = FLT_MAX;
+        while (ballot(t2.live) != 0) {
+            if (t2.live) vertexStep(c, L, S.stNode, S.stDist, w.F, i, vx0, cur0, t2, nd, nb, flags);
+        }
+    }
+    return exact ? nb : kNone;
 }
 
 __device__ __forceinline__ void vertexTask(const CellCtx& c, VertexLds& S, uint32_t i, uint32_t maxRestarts) {
@@ -189,16 +402,8 @@ __device__ __forceinline__ void vertexTask(const CellCtx& c, VertexLds& S, uint3
             if (2.0 * cur.w < double(kr)) need = false;
             L.vtx[v].w = (r.w & 0xffu) | (1u << 8);
         }
-        {
-            // nearest non-neighbour site to the vertex sphere, all lanes' searches in step
-            Trav tr{root, 0u, need};
-            float nd = FLT_MAX;
-            const uint4 r = v < w.V ? L.vtx[v] : uint4{0, 0, 0, 0};
-            const F4 fcur = toF(cur);
-            while (ballot(tr.live) != 0) {
-                if (tr.live) vertexStep(c, L, S.stNode, S.stDist, w.F, i, r, fcur, tr, nd, nb, flags);
-            }
-        }
+        // nearest non-neighbour site to the vertex sphere, all lanes' searches in step
+        nb = vertexSearches(c, S, L, w, i, root, need, v < w.V ? L.vtx[v] : uint4{0, 0, 0, 0}, toF(cur), flags);
         bool invalid = false;
         if (nb != kNone) invalid = intersectD(cur, loadSite(c, nb));
         const uint64_t m = ballot(invalid);
@@ -236,15 +441,6 @@ struct EdgeQuery {
     F4 ball[kCoverBalls];     // every contributor j has |p_j - c| - r_j <= r for one ball (edge_bound.h)
 };
 
-// Pop the next node of a lane's depth-first traversal, or end it.
-__device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
-    if (t.sp == t.sb) {
-        t.live = false;
-    } else {
-        t.sp--;
-        t.node = stNode[t.sp][laneId()];
-    }
-}
 
 // One step of a lane's contributor traversal (algorithm.inl:616-793): visit
 // one internal node (descend to the closer accepted child, stack the other)
@@ -363,59 +559,6 @@ __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L
         countBytes(c, family, 29u + 16u * uint64_t(edgesLoaded + w.E + (total - baseV)) + 9u);
     }
 }
-
-// Is leaf a visited before leaf b by a closest-child-first traversal for
-// the search sphere bs?  Both leaves hang below their lowest common
-// ancestor, whose closer child (by box distance to bs, the traversal's own
-// order test) is visited first.
-__device__ __forceinline__ bool visitedBefore(const CellCtx& c, uint32_t a, uint32_t b, F4 bs) {
-    constexpr uint32_t kMaxDepth = 4096;  // bounds every climb, whatever the tree holds
-    uint32_t na = c.internal + a, nb = c.internal + b;
-    uint32_t da = 0, db = 0;
-    for (int32_t p = c.parent[na]; p >= 0 && da < kMaxDepth; p = c.parent[p]) da++;
-    for (int32_t p = c.parent[nb]; p >= 0 && db < kMaxDepth; p = c.parent[p]) db++;
-    if (da >= kMaxDepth || db >= kMaxDepth) {
-        atomicOr(c.err, kErrValidation);
-        return a < b;
-    }
-    while (da > db) {
-        na = uint32_t(c.parent[na]);
-        da--;
-    }
-    while (db > da) {
-        nb = uint32_t(c.parent[nb]);
-        db--;
-    }
-    for (uint32_t guard = 0; guard <= kMaxDepth; guard++) {
-        const int32_t pa = c.parent[na], pb = c.parent[nb];
-        if (pa == pb || pa < 0 || pb < 0) {
-            if (pa != pb || pa < 0) return a < b;  // different trees cannot happen for one query
-            const Node nn = c.nodes[pa];
-            Box lb, rb;
-            nodeBoxes(nn, lb, rb);
-            const bool leftFirst = boxDistance(lb, bs) < boxDistance(rb, bs);
-            const bool aLeft = uint32_t(nn.d.x) == na;
-            return aLeft == leftFirst;
-        }
-        na = uint32_t(pa);
-        nb = uint32_t(pb);
-    }
-    atomicOr(c.err, kErrValidation);
-    return a < b;
-}
-
-__device__ __forceinline__ F4 shflF4(F4 v, uint32_t src) {
-    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64), __shfl(v.w, int(src), 64)};
-}
-__device__ __forceinline__ F3 shflF3(F3 v, uint32_t src) {
-    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64)};
-}
-__device__ __forceinline__ D4 shflD4(D4 v, uint32_t src) {
-    return {__shfl(v.x, int(src), 64), __shfl(v.y, int(src), 64), __shfl(v.z, int(src), 64), __shfl(v.w, int(src), 64)};
-}
-
-// Lanes with no search left take one when at least this many are idle.
-constexpr uint32_t kShareIdle = 4;
 
 // The contributor searches of all lanes' edges (algorithm.inl:616-793),
 // each returning its first contributor in the reference's traversal order.
@@ -682,7 +825,7 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*st
         if (ec || ef) {
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
-                if (t.sp < kStackDepth) {
+                if (t.base + t.sp < kStackDepth) {
                     stNode[t.sp][lane] = farChild;
                     t.sp++;
                 } else {
@@ -713,6 +856,95 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*st
     }
     popNode(stNode, t);
     return cand;
+}
+
+// The contributor searches of all lanes' bisectors (algorithm.inl:1019-1176),
+// with the work sharing of the edge searches: the owner's own item holds
+// the earliest part of its search and settles it; contributors of shared
+// items are kept per owner in traversal order.
+__device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorLds& S, const CellLds& L,
+                                                     const WaveCell& w, uint32_t i, const D4& si, uint32_t root,
+                                                     bool need, uint32_t bk, D4 sk, F3 fci, F3 fck, F4 bsph,
+                                                     float cylR, uint32_t& flags) {
+    const uint32_t lane = laneId();
+    S.res[lane] = kNone;
+    S.settled[lane] = 0u;
+    uint32_t owner = need ? lane : kNone;
+    bool own = need;
+    Trav tr{root, 0u, need};
+    uint32_t found = kNone, cand = kNone;
+    waveSync();
+    while (true) {
+        if (tr.live && !own && owner != kNone && S.settled[owner]) tr.live = false;
+        const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
+        if (stepping == 0 && waiting == 0) break;
+        if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
+            uint32_t shared = kNone;
+            if (cand != kNone) {
+                bool ok = !knownLane(L, w.F, cand);
+                if (ok) {
+                    D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
+                    const uint32_t cnt = trisector(si, loadSite(c, cand), sk, x0, x1);
+                    ok = cnt == 2u && clearOfNeighbours(c, L, w.F, x1, bk, bk);
+                }
+                if (ok) {
+                    if (own) {
+                        found = cand;
+                        S.settled[lane] = 1u;
+                    } else {
+                        shared = cand;
+                    }
+                    tr.live = false;
+                }
+                cand = kNone;
+            }
+            uint64_t hm = ballot(shared != kNone);
+            while (hm) {
+                const uint32_t t = uint32_t(__builtin_ctzll(hm));
+                hm &= hm - 1;
+                if (lane == t) {
+                    const uint32_t cur = S.res[owner];
+                    if (cur == kNone || visitedBefore(c, shared, cur, bsph)) S.res[owner] = shared;
+                }
+                waveSync();
+            }
+            waveSync();
+            continue;
+        }
+        const uint64_t idle = ballot(!tr.live && cand == kNone);
+        const uint64_t donors = ballot(tr.live && cand == kNone && tr.sp > tr.sb);
+        if (popc64(idle) >= kShareIdle && donors != 0) {
+            const uint32_t m = popc64(idle) < popc64(donors) ? popc64(idle) : popc64(donors);
+            const uint32_t dr = countBelow(donors), ir = countBelow(idle);
+            const bool give = ((donors >> lane) & 1u) && dr < m;
+            const bool take = ((idle >> lane) & 1u) && ir < m;
+            if (give) S.share[dr] = lane;
+            waveSync();
+            const uint32_t src = take ? S.share[ir] : lane;
+            const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
+            const uint32_t dOwner = __shfl(owner, int(src), 64);
+            const uint32_t node = take ? S.stNode[dSb][src] : 0u;
+            bk = __shfl(bk, int(src), 64);
+            sk = shflD4(sk, src);
+            fck = shflF3(fck, src);
+            bsph = shflF4(bsph, src);
+            cylR = __shfl(cylR, int(src), 64);
+            if (take) {
+                owner = dOwner;
+                own = false;
+                tr.node = node;
+                tr.sp = 0u;
+                tr.sb = 0u;
+                tr.base = dBase + dSb;
+                tr.live = true;
+            }
+            if (give) tr.sb++;
+            waveSync();
+        }
+        if (tr.live && cand == kNone) cand = bisectorStep(c, S.stNode, i, bk, fci, fck, bsph, cylR, tr, flags);
+    }
+    waveSync();
+    return need && found == kNone ? S.res[lane] : found;
 }
 
 __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, uint32_t i, uint32_t maxRestarts) {
@@ -786,29 +1018,7 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
         }
         // the first site of each lane's traversal whose trisector with i, bk
         // has two points, the second clear of the cell's neighbours
-        Trav tr{root, 0u, need};
-        uint32_t cand = kNone;
-        while (true) {
-            const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
-            if (stepping == 0 && waiting == 0) break;
-            if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
-                if (cand != kNone) {
-                    bool ok = !knownLane(L, w.F, cand);
-                    if (ok) {
-                        D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
-                        const uint32_t cnt = trisector(si, loadSite(c, cand), sk, x0, x1);
-                        ok = cnt == 2u && clearOfNeighbours(c, L, w.F, x1, bk, bk);
-                    }
-                    if (ok) {
-                        found = cand;
-                        tr.live = false;
-                    }
-                    cand = kNone;
-                }
-                continue;
-            }
-            if (tr.live && cand == kNone) cand = bisectorStep(c, S.stNode, i, bk, fci, fck, bsph, cylR, tr, flags);
-        }
+        found = bisectorSearches(c, S, L, w, i, si, root, need, bk, sk, fci, fck, bsph, cylR, flags);
         if (need && found == kNone) S.bis[bid] = kNone;
         const uint64_t m = ballot(found != kNone);
         waveSync();

@@ -30,7 +30,8 @@ struct VertexLds {
     uint32_t stNode[kStackDepth][64];
     float stDist[kStackDepth][64];
     uint32_t share[64];   // donor lane of each receiving rank (work sharing)
-    uint32_t hit[64];     // per owner lane: a site came within the margin
+    float resD[64];       // per owner lane: smallest gap found by shared items
+    uint32_t resN[64];    // ... and its site (earliest in traversal order on ties)
 };
 struct EdgeLds {
     CellLds cell;
@@ -231,16 +232,18 @@ __device__ __forceinline__ D4 shflD4(D4 v, uint32_t src) {
 // Lanes with no search left take one when at least this many are idle.
 constexpr uint32_t kShareIdle = 4;
 
-// One step of a lane's range search: does a site the nearest-site search
-// could return (not i, not the vertex's sites, not buried, not a cell
-// neighbour) come within `reach` of the vertex sphere?  A site's gap is at
-// least its box's gap when the vertex centre lies outside the site, and
-// negative only when the centre lies inside it (and so inside every box
-// above it): a node is skipped when its box gap exceeds `boxReach` and the
-// centre is outside the box.  Order and stack are those of vertexStep.
-__device__ __forceinline__ bool vertexRangeStep(const CellCtx& c, const CellLds& L, uint32_t (*stNode)[64],
-                                                uint32_t F, uint32_t i, uint4 vx, F4 cur, float reach,
-                                                float boxReach, Trav& t, uint32_t& flags) {
+// One step of a lane's search for the nearest site within its bound `nd`
+// among the sites the reference's nearest-site search can return (not i,
+// not the vertex's sites, not buried, not a cell neighbour).  Children are
+// visited in the reference's order (vertexStep) but pruned soundly: a
+// site's gap is at least its box's gap minus `slack` (FP32 rounding of boxes
+// and gaps) when the vertex centre lies outside the box, and unbounded
+// below when it lies inside (a centre inside a site gives a negative gap),
+// so a box within `slack` of the centre is never pruned.  Leaves improve
+// the bound strictly, so the first of equal gaps in this order is kept.
+__device__ __forceinline__ void vertexNearStep(const CellCtx& c, const CellLds& L, uint32_t (*stNode)[64],
+                                               uint32_t F, uint32_t i, uint4 vx, F4 cur, float slack, Trav& t,
+                                               float& nd, uint32_t& nb, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t node = t.node;
     if (node < c.internal) {
@@ -253,10 +256,11 @@ __device__ __forceinline__ bool vertexRangeStep(const CellCtx& c, const CellLds&
         if (L0 >= nodeCount(c) || R0 >= nodeCount(c)) {
             atomicOr(c.err, kErrKnnChild);
         } else {
+            const bool lc = boxDistance(lb, cur) < boxDistance(rb, cur);
             const F3 p = xyzf(cur);
             const float eL = lenf(subf(p, closestPoint(lb, cur))), eR = lenf(subf(p, closestPoint(rb, cur)));
-            const bool inL = eL == 0.f || eL - cur.w <= boxReach, inR = eR == 0.f || eR - cur.w <= boxReach;
-            const bool lc = eL < eR;
+            const bool inL = eL <= slack || eL - cur.w - slack <= nd;
+            const bool inR = eR <= slack || eR - cur.w - slack <= nd;
             farChild = lc ? R0 : L0;
             closeChild = lc ? L0 : R0;
             ef = lc ? inR : inL;
@@ -272,37 +276,65 @@ __device__ __forceinline__ bool vertexRangeStep(const CellCtx& c, const CellLds&
                     flags |= kFlagStackOverflow;
                 }
             }
-            return false;
+            return;
         }
         popNode(stNode, t);
-        return false;
+        return;
     }
-    bool hit = false;
     if (node < nodeCount(c)) {
         const uint32_t j = node - c.internal;
-        if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried)
-            hit = sphereDistF(c.sorted[j], cur) <= reach && !knownLane(L, F, j);
+        if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried) {
+            const float d = sphereDistF(c.sorted[j], cur);
+            if (d < nd && !knownLane(L, F, j)) {
+                nd = d;
+                nb = j;
+            }
+        }
     }
     popNode(stNode, t);
-    return hit;
+}
+
+// Does the reference's search (vertexStep) reach leaf j and keep it?  It
+// descends into a node, or pops it from its stack, only when the node's box
+// gap (as vertexStep computes it) is below its running best, which never
+// drops below the smallest gap d over all eligible sites.  So when every
+// node on the path from the root to leaf j has a box gap strictly below d
+// -- the gap of j, the smallest, and the first of equal gaps in the
+// traversal order -- the search reaches j with a best above d, keeps it,
+// and nothing later improves on it.
+__device__ __forceinline__ bool searchKeeps(const CellCtx& c, uint32_t j, float d, F4 cur) {
+    uint32_t node = c.internal + j;
+    for (uint32_t guard = 0; guard < nodeCount(c); guard++) {
+        const int32_t p = c.parent[node];
+        if (p < 0) return true;
+        const Node nn = c.nodes[p];
+        Box lb, rb;
+        nodeBoxes(nn, lb, rb);
+        const Box b = selBox(uint32_t(nn.d.x) == node, lb, rb);
+        if (!(boxDistance(b, cur) < d)) return false;
+        node = uint32_t(p);
+    }
+    return false;
 }
 
 // The nearest-site searches of all lanes' vertices (algorithm.inl:305-414).
 //
 // The caller uses one thing of a search: whether its site intersects the
-// vertex sphere, and then which site.  So each vertex first asks a range
-// question, shared across the wave like the edge searches (idle lanes take
-// the last subtree of busy lanes' stacks): does any site the search could
-// return come within a rounding margin of the sphere?  If none does, the
-// site the reference's search returns cannot intersect (its FP32 gap
-// exceeds the margin, so its FP64 gap exceeds the intersection tolerance)
-// and the answer is "no intersection", whatever that site is.  A vertex
-// with a site inside the margin reruns the reference's own traversal.
-//
-// The shortcut also needs the reference's search not to overflow its stack
-// (the overflow is recorded in the cell's flags): it is taken only when no
-// leaf is deeper than the stack, which makes an overflow impossible.
-// Returns the site of the exact search, or kNone.
+// vertex sphere, and then which site.  Each vertex first looks, with the
+// work sharing of the edge searches (idle lanes take the last subtree of
+// busy lanes' stacks), for the nearest eligible site within a rounding
+// margin of its sphere:
+//  * none: the site the reference returns has an FP32 gap above the margin,
+//    so its FP64 gap exceeds the intersection tolerance: no intersection,
+//    whatever that site is;
+//  * site j: if the reference's search provably reaches and keeps j
+//    (searchKeeps), j is its answer; otherwise the vertex reruns the
+//    reference's own traversal.
+// Both shortcuts also need the reference's search not to overflow its stack
+// (the overflow is recorded in the cell's flags): they are taken only when
+// no leaf is deeper than the stack, which makes an overflow impossible.
+// Returns the site the reference's search returns when it may intersect,
+// else kNone.
 __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& S, const CellLds& L,
                                                    const WaveCell& w, uint32_t i, uint32_t root, bool need,
                                                    const uint4 vx0, const F4 cur0, uint32_t& flags) {
@@ -310,16 +342,20 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
     const float reach = 1e-5f * (__builtin_fabsf(cur0.x) + __builtin_fabsf(cur0.y) + __builtin_fabsf(cur0.z) +
                                  __builtin_fabsf(cur0.w)) + 1e-6f;
     bool exact = need;
+    uint32_t site = kNone;
     if (__builtin_amdgcn_readfirstlane(*c.treeHeight) <= kStackDepth) {
-        S.hit[lane] = 0u;
+        S.resN[lane] = kNone;
+        S.resD[lane] = FLT_MAX;
         uint4 vx = vx0;
         F4 cur = cur0;
-        float wreach = reach;
+        float slack = reach;
         uint32_t owner = need ? lane : kNone, rflags = 0;
+        bool own = need;
+        float nd = reach, ownNd = reach;
+        uint32_t nb = kNone, ownNb = kNone;
         Trav tr{root, 0u, need};
         waveSync();
         while (true) {
-            if (tr.live && S.hit[owner]) tr.live = false;  // the owner already has its answer
             const uint64_t live = ballot(tr.live);
             if (live == 0) break;
             const uint64_t idle = ~live;
@@ -334,14 +370,22 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                 const uint32_t src = take ? S.share[ir] : lane;
                 const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
                 const uint32_t dOwner = __shfl(owner, int(src), 64);
+                const float dNd = __shfl(nd, int(src), 64);
                 const uint32_t node = take ? S.stNode[dSb][src] : 0u;
                 cur = shflF4(cur, src);
-                wreach = __shfl(wreach, int(src), 64);
+                slack = __shfl(slack, int(src), 64);
                 vx.x = __shfl(vx.x, int(src), 64);
                 vx.y = __shfl(vx.y, int(src), 64);
                 vx.z = __shfl(vx.z, int(src), 64);
                 if (take) {
+                    if (own) {
+                        ownNd = nd;
+                        ownNb = nb;
+                    }
                     owner = dOwner;
+                    own = false;
+                    nd = dNd;
+                    nb = kNone;
                     tr.node = node;
                     tr.sp = 0u;
                     tr.sb = 0u;
@@ -351,29 +395,51 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                 if (give) tr.sb++;
                 waveSync();
             }
-            bool hit = false;
-            if (tr.live) hit = vertexRangeStep(c, L, S.stNode, w.F, i, vx, cur, wreach, 2.f * wreach, tr, rflags);
-            // an overflow cannot happen below the height bound; were it to, the
-            // search is rerun exactly
-            if (rflags) hit = true;
-            if (hit) {
-                tr.live = false;
-                S.hit[owner] = 1u;  // idempotent store: concurrent writers agree
+            const bool wasLive = tr.live;
+            if (tr.live) vertexNearStep(c, L, S.stNode, w.F, i, vx, cur, slack, tr, nd, nb, rflags);
+            // a shared item that ended hands its nearest site to its owner
+            const bool report = wasLive && !tr.live && !own && nb != kNone;
+            uint64_t hm = ballot(report);
+            while (hm) {
+                const uint32_t t = uint32_t(__builtin_ctzll(hm));
+                hm &= hm - 1;
+                if (lane == t) {
+                    const float bd = S.resD[owner];
+                    const uint32_t bn = S.resN[owner];
+                    if (bn == kNone || nd < bd || (nd == bd && visitedBefore(c, nb, bn, cur))) {
+                        S.resD[owner] = nd;
+                        S.resN[owner] = nb;
+                    }
+                }
+                waveSync();
             }
-            waveSync();
         }
-        exact = need && S.hit[lane] != 0u;
+        waveSync();
+        if (own) {
+            ownNd = nd;
+            ownNb = nb;
+        }
+        float best = ownNd;
+        site = ownNb;
+        // the lane's own item holds the earliest part of its search: it wins ties
+        if (need && S.resN[lane] != kNone && (site == kNone || S.resD[lane] < best)) {
+            best = S.resD[lane];
+            site = S.resN[lane];
+        }
+        // an overflow cannot happen below the height bound; were it to, rerun
+        exact = need && (rflags != 0u || (site != kNone && !searchKeeps(c, site, best, cur0)));
+        if (!need || exact) site = kNone;
     }
-    // the reference's traversal for the vertices that need its site
-    uint32_t nb = kNone;
+    // the reference's traversal for the vertices that need it
+    uint32_t nb2 = kNone;
     if (ballot(exact) != 0) {
         Trav t2{root, 0u, exact};
-        float nd = FLT_MAX;
+        float nd2 = FLT_MAX;
         while (ballot(t2.live) != 0) {
-            if (t2.live) vertexStep(c, L, S.stNode, S.stDist, w.F, i, vx0, cur0, t2, nd, nb, flags);
+            if (t2.live) vertexStep(c, L, S.stNode, S.stDist, w.F, i, vx0, cur0, t2, nd2, nb2, flags);
         }
     }
-    return exact ? nb : kNone;
+    return exact ? nb2 : site;
 }
 
 __device__ __forceinline__ void vertexTask(const CellCtx& c, VertexLds& S, uint32_t i, uint32_t maxRestarts) {

@@ -75,7 +75,7 @@ struct apo_ctx {
     uint32_t* boxTicket = nullptr;
     uint32_t* sortScratch = nullptr;
     uint32_t* range = nullptr;
-    uint32_t* crossList = nullptr;
+    uint32_t* startLists = nullptr;
     uint64_t* codesA = nullptr;
     uint64_t* codesB = nullptr;
     uint32_t* idxA = nullptr;
@@ -191,7 +191,7 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     APO_HIP(hipGetLastError());
     APO_HIP(alloc(ctx->sortScratch, sortScratchWords(cap)));
     APO_HIP(alloc(ctx->range, 2 * size_t(cap)));
-    APO_HIP(alloc(ctx->crossList, cap));
+    APO_HIP(alloc(ctx->startLists, refitStartWords(cap)));
     APO_HIP(alloc(ctx->codesA, cap));
     APO_HIP(alloc(ctx->codesB, cap));
     APO_HIP(alloc(ctx->idxA, cap));
@@ -214,7 +214,7 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     APO_HIP(alloc(ctx->err, 1));
     APO_HIP(alloc(ctx->tasks, cap));
     APO_HIP(alloc(ctx->cellCount, cap));
-    APO_HIP(alloc(ctx->blockSums, (cap + 255) / 256));
+    APO_HIP(alloc(ctx->blockSums, (cap + 255) / 256 + 2));
     APO_HIP(alloc(ctx->total, 1));
     ctx->outCap = uint64_t(cap) * 8 + 1024;
     APO_HIP(alloc(ctx->outXyzw, ctx->outCap));
@@ -298,7 +298,7 @@ LbvhBuffers lbvhBuffers(apo_ctx* ctx) {
     return LbvhBuffers{ctx->scene,  ctx->boxPartials, ctx->boxTicket,    ctx->sortScratch, ctx->codesA,
                        ctx->codesB, ctx->idxA,        ctx->idxB,         ctx->sorted,      ctx->nodes,
                        ctx->parent, ctx->sizes,       ctx->refitCounter, ctx->rootBox,     ctx->err,
-                       ctx->range,  ctx->crossList};
+                       ctx->range,  ctx->startLists};
 }
 
 Segments segments(apo_ctx* ctx) { return Segments{ctx->molecules, ctx->molecules > 1u ? ctx->offsets : nullptr}; }
@@ -354,7 +354,11 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     APO_TIMED_STAGE("cells");
     APO_HIP(mark(3));
     // tree height: lets vertex validation skip a search that cannot overflow
-    APO_HIP(treeHeightLaunch(ctx->parent, n, n - ctx->molecules, ctx->counters + kTreeHeight, st));
+    // (the task list, cell counts and block sums are free until the validation
+    // rounds and the output)
+    APO_HIP(treeHeightLaunch(ctx->range, n, n - ctx->molecules, reinterpret_cast<int32_t*>(ctx->tasks),
+                             reinterpret_cast<int32_t*>(ctx->cellCount), reinterpret_cast<int2*>(ctx->blockSums),
+                             ctx->counters + kTreeHeight, st));
     const ValidationBuffers vb{ctx->tasks, ctx->counters};
     const ValidationGrid vg{persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
                             persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge),
@@ -574,7 +578,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->codesB, ctx->idxA, ctx->idxB, ctx->sorted, ctx->nodes, ctx->parent, ctx->sizes,
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
-                    ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->crossList,
+                    ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
                     ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)

@@ -69,7 +69,7 @@ struct LbvhBuffers {
     Scene* scene;             // one per molecule
     double* boxPartials;   // kBoxBlocks * 8
     uint32_t* boxTicket;   // 1
-    uint32_t* sortScratch; // hist [P*256] + tile counters [P] + look-back [P*tiles*256] + 1
+    uint32_t* sortScratch; // hist [P*256] + tile counters [P] + look-back [P*tiles*256]
     uint64_t* codesA;
     uint64_t* codesB;
     uint32_t* idxA;        // sorted position -> original index (final)
@@ -82,15 +82,45 @@ struct LbvhBuffers {
     float* rootBox;        // 7 floats: min, max, max radius
     uint32_t* err;         // device guard bits
     uint32_t* range;       // 2(n-1): first and last leaf of each internal node
-    uint32_t* crossList;   // n-1: internal nodes whose leaves span two refit blocks
+    uint32_t* startLists;  // refitStartWords(n): climb starts of the later refit phases
 };
 
-// Leaves per block of the first refit phase (lbvh.hip).
-constexpr uint32_t kRefitLeaves = 2048;
+// Refit phases (lbvh.hip): phase 0 over blocks of 2^kRefitShift leaves,
+// phase p over blocks of 2^(kRefitShift + kRefitFanShift * p) leaves.
+constexpr uint32_t kRefitShift = 9;
+constexpr uint32_t kRefitLeaves = 1u << kRefitShift;
+constexpr uint32_t kRefitFanShift = 6;
+constexpr uint32_t kRefitLevels = 4;  // levels 0..3 cover every 32-bit leaf range
+
+// Start lists of phases 1..kRefitLevels: level L holds one slot per block
+// boundary of phase L (slot = boundary >> shift), ~0 when no climb starts
+// there.
+struct RefitStarts {
+    uint32_t* list;
+    uint32_t n;
+};
+__host__ __device__ inline uint32_t refitSlots(uint32_t n, uint32_t level) {
+    return n > 1u ? ((n - 1u) >> (kRefitShift + kRefitFanShift * level)) + 1u : 1u;
+}
+__host__ __device__ inline uint32_t refitSlotOffset(uint32_t n, uint32_t level) {
+    uint32_t o = 0;
+    for (uint32_t l = 0; l < level; l++) o += refitSlots(n, l);
+    return o;
+}
+__host__ __device__ inline uint32_t refitStartWords(uint32_t n) { return refitSlotOffset(n, kRefitLevels); }
+
+// Phases after the first: enough that the last one's block holds every leaf.
+inline uint32_t refitPhases(uint32_t n) {
+    if (n < 2u || ((n - 1u) >> kRefitShift) == 0u) return 0u;
+    const uint32_t hb = 31u - uint32_t(__builtin_clz(n - 1u));
+    return (hb - kRefitShift) / kRefitFanShift + 1u;
+}
 
 size_t sortScratchWords(uint32_t n);
-// Deepest leaf of the trees into *height (cleared beforehand; read by validation).
-hipError_t treeHeightLaunch(const int32_t* parent, uint32_t n, uint32_t internal, uint32_t* height, hipStream_t st);
+// Deepest leaf of the trees into *height (read by validation); diff and
+// far: n words of scratch each, blocks: n / 4096 + 1 pairs of scratch.
+hipError_t treeHeightLaunch(const uint32_t* range, uint32_t n, uint32_t internal, int32_t* diff, int32_t* far,
+                            int2* blocks, uint32_t* height, hipStream_t st);
 hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const Segments& seg, hipStream_t st);
 
 // Device view of the construction state.

@@ -230,14 +230,18 @@ __global__ __launch_bounds__(256) void k_morton(const F4* __restrict__ sites, ui
 }
 
 // ---------------------------------------------------------------------------
-// Onesweep LSD radix sort pass: 8-bit digit, 4096 keys per tile (4 waves x
-// 16 rounds x 64 lanes).  Ranking inside a wave uses 8 ballots to find the
+// Onesweep LSD radix sort pass: 8-bit digit, 4096 keys per tile (16 waves x
+// 4 rounds x 64 lanes).  Ranking inside a wave uses 8 ballots to find the
 // lanes holding the same digit, so equal digits keep input order (stable).
 // Tiles take an index from an atomic counter and chain their per-digit
 // prefixes with decoupled look-back.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kTileKeys = 4096;
+constexpr uint32_t kSortWaves = 16;
+constexpr uint32_t kSortThreads = 64u * kSortWaves;
+constexpr uint32_t kSortRounds = kTileKeys / kSortThreads;
 constexpr uint32_t kFlagAgg = 1u << 30, kFlagInc = 2u << 30, kValueMask = (1u << 30) - 1u;
+constexpr uint32_t kLookBack = 8;  // predecessor tiles read per look-back step
 
 // Digit of a key / value pair in pass p: a key byte, or (kMol) a digit of
 // the molecule of the value (an original site index).
@@ -248,14 +252,16 @@ __device__ __forceinline__ uint32_t sortDigit(uint64_t key, uint32_t val, int p,
 }
 
 template <bool kMol>
-__global__ __launch_bounds__(256) void k_onesweep(const uint64_t* __restrict__ keysIn, const uint32_t* __restrict__ valsIn,
-                                                  uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut,
-                                                  uint32_t n, int pass, Segments seg,
-                                                  const uint32_t* __restrict__ digitHist, uint32_t* lookback,
-                                                  uint32_t* tileCounter, uint32_t numTiles, uint32_t* err) {
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __restrict__ keysIn,
+                                                           const uint32_t* __restrict__ valsIn,
+                                                           uint64_t* __restrict__ keysOut,
+                                                           uint32_t* __restrict__ valsOut, uint32_t n, int pass,
+                                                           Segments seg, const uint32_t* __restrict__ digitHist,
+                                                           uint32_t* lookback, uint32_t* tileCounter,
+                                                           uint32_t numTiles, uint32_t* err) {
     __shared__ uint64_t sKeys[kTileKeys];
     __shared__ uint32_t sVals[kTileKeys];
-    __shared__ uint32_t sWave[4][256];
+    __shared__ uint32_t sWave[kSortWaves][256];  // per-wave digit counts, then per-wave digit offsets
     __shared__ uint32_t sTileStart[256];
     __shared__ uint32_t sGlobal[256];
     __shared__ uint32_t sScan[4];
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint64_t* __restrict__ k
 
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = laneId();
     if (tid == 0) sTile = atomicAdd(tileCounter, 1u);
-    sWave[0][tid] = sWave[1][tid] = sWave[2][tid] = sWave[3][tid] = 0u;
+    for (uint32_t t = tid; t < kSortWaves * 256u; t += kSortThreads) (&sWave[0][0])[t] = 0u;
     __syncthreads();
     const uint32_t tile = sTile;
     if (tile >= numTiles) {
@@ -272,17 +278,21 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint64_t* __restrict__ k
         if (tid == 0) atomicOr(err, kErrSortTile);
         return;
     }
-    const uint32_t base = tile * kTileKeys + wave * 1024u;
+    const uint32_t base = tile * kTileKeys + wave * (kSortRounds * 64u);
 
-    uint64_t key[16];
-    uint32_t val[16];
-    uint32_t rank[16];
+    uint64_t key[kSortRounds];
+    uint32_t val[kSortRounds];
+    uint32_t rank[kSortRounds];
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const uint32_t idx = base + uint32_t(r) * 64u + lane;
+    for (uint32_t r = 0; r < kSortRounds; r++) {
+        const uint32_t idx = base + r * 64u + lane;
         const bool valid = idx < n;
         key[r] = valid ? keysIn[idx] : 0ull;
         val[r] = valid ? valsIn[idx] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kSortRounds; r++) {
+        const bool valid = base + r * 64u + lane < n;
         const uint32_t d = sortDigit<kMol>(key[r], val[r], pass, seg);
         uint64_t peers = ballot(valid);
 #pragma unroll
@@ -299,61 +309,82 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint64_t* __restrict__ k
     }
     __syncthreads();
 
-    // per-digit tile totals, per-wave exclusive offsets
+    // digit threads: tile totals, per-wave exclusive offsets, look-back, scans
+    uint32_t tileCount = 0, excl = 0, g = 0, gInc = 0, tInc = 0;
     const uint32_t d = tid;
-    const uint32_t c0 = sWave[0][d], c1 = sWave[1][d], c2 = sWave[2][d], c3 = sWave[3][d];
-    const uint32_t tileCount = c0 + c1 + c2 + c3;
-    // publish the tile aggregate, then look back for the exclusive prefix
-    uint32_t* lb = lookback + size_t(tile) * 256u + d;
-    uint32_t excl = 0;
-    if (tile == 0) {
-        __hip_atomic_store(lb, kFlagInc | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-        __hip_atomic_store(lb, kFlagAgg | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int t = int(tile) - 1;
-        uint32_t spins = 0;
-        while (t >= 0) {
-            const uint32_t v = __hip_atomic_load(lookback + size_t(t) * 256u + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t flag = v & ~kValueMask;
-            if (flag == 0u) {
-                if (++spins > (1u << 22)) {
-                    atomicOr(err, kErrSortSpin);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += v & kValueMask;
-            if (flag == kFlagInc) break;
-            t--;
+    if (tid < 256u) {
+        for (uint32_t w = 0; w < kSortWaves; w++) {
+            const uint32_t c = sWave[w][d];
+            sWave[w][d] = tileCount;
+            tileCount += c;
         }
-        __hip_atomic_store(lb, kFlagInc | (excl + tileCount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // publish the tile aggregate, then look back for the exclusive prefix
+        uint32_t* lb = lookback + size_t(tile) * 256u + d;
+        if (tile == 0) {
+            __hip_atomic_store(lb, kFlagInc | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(lb, kFlagAgg | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // kLookBack predecessors are read at once; their values are taken
+            // in order up to the first inclusive one, or up to the first not
+            // yet published, where the next read starts
+            int t = int(tile) - 1;
+            uint32_t spins = 0;
+            while (t >= 0) {
+                uint32_t v[kLookBack];
+#pragma unroll
+                for (int j = 0; j < int(kLookBack); j++)
+                    v[j] = t - j >= 0 ? __hip_atomic_load(lookback + size_t(t - j) * 256u + d, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : kFlagInc;
+                int used = 0;
+                bool done = false;
+#pragma unroll
+                for (int j = 0; j < int(kLookBack); j++) {
+                    const uint32_t flag = v[j] & ~kValueMask;
+                    if (flag == 0u) break;
+                    excl += v[j] & kValueMask;
+                    used = j + 1;
+                    if (flag == kFlagInc) {
+                        done = true;
+                        break;
+                    }
+                }
+                if (done) break;
+                t -= used;
+                if (used < int(kLookBack)) {
+                    if (++spins > (1u << 22)) {
+                        atomicOr(err, kErrSortSpin);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __hip_atomic_store(lb, kFlagInc | (excl + tileCount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // exclusive scans over the 256 digits: global digit starts and tile digit starts
+        g = digitHist[d];
+        gInc = waveInclusiveSum(g);
+        tInc = waveInclusiveSum(tileCount);
+        if (lane == 63) {
+            sScan[wave] = gInc;
+            sScan2[wave] = tInc;
+        }
     }
-    // exclusive scans over the 256 digits: global digit starts and tile digit starts
-    const uint32_t g = digitHist[d];
-    uint32_t gInc = waveInclusiveSum(g);
-    uint32_t tInc = waveInclusiveSum(tileCount);
-    if (lane == 63) {
-        sScan[wave] = gInc;
-        sScan2[wave] = tInc;
-    }
-    sWave[1][d] = c0;
-    sWave[2][d] = c0 + c1;
-    sWave[3][d] = c0 + c1 + c2;
-    sWave[0][d] = 0u;
     __syncthreads();
-    uint32_t gOff = 0, tOff = 0;
-    for (uint32_t w = 0; w < wave; w++) {
-        gOff += sScan[w];
-        tOff += sScan2[w];
+    if (tid < 256u) {
+        uint32_t gOff = 0, tOff = 0;
+        for (uint32_t w = 0; w < wave; w++) {
+            gOff += sScan[w];
+            tOff += sScan2[w];
+        }
+        sGlobal[d] = gOff + gInc - g + excl;
+        sTileStart[d] = tOff + tInc - tileCount;
     }
-    sGlobal[d] = gOff + gInc - g + excl;
-    sTileStart[d] = tOff + tInc - tileCount;
     __syncthreads();
 
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const uint32_t idx = base + uint32_t(r) * 64u + lane;
+    for (uint32_t r = 0; r < kSortRounds; r++) {
+        const uint32_t idx = base + r * 64u + lane;
         if (idx < n) {
             const uint32_t dg = sortDigit<kMol>(key[r], val[r], pass, seg);
             const uint32_t pos = sTileStart[dg] + sWave[wave][dg] + rank[r];
@@ -368,7 +399,7 @@ __global__ __launch_bounds__(256) void k_onesweep(const uint64_t* __restrict__ k
     __syncthreads();
     const uint32_t tileBase = tile * kTileKeys;
     const uint32_t tileValid = (n - tileBase) < kTileKeys ? (n - tileBase) : kTileKeys;
-    for (uint32_t p = tid; p < tileValid; p += 256u) {
+    for (uint32_t p = tid; p < tileValid; p += kSortThreads) {
         const uint64_t k = sKeys[p];
         const uint32_t v = sVals[p];
         const uint32_t dg = sortDigit<kMol>(k, v, pass, seg);
@@ -398,9 +429,10 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ cod
                                                 F4* __restrict__ sorted, Node* __restrict__ nodes,
                                                 int32_t* __restrict__ parent, uint32_t* __restrict__ refitCounter,
                                                 uint32_t* __restrict__ sizes, uint32_t* __restrict__ range,
-                                                uint32_t* __restrict__ crossList, uint32_t* crossCount, uint32_t* err) {
+                                                RefitStarts rs, uint32_t* err) {
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= n) return;
+    if (gi < refitStartWords(n)) rs.list[gi] = ~0u;
     const uint32_t src = indices[gi];
     if (src < n) {
         sorted[gi] = sitesIn[src];
@@ -457,7 +489,6 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ cod
     range[2 * g] = first;
     range[2 * g + 1] = last;
     sizes[g] = last - first + 1u;
-    if (first / kRefitLeaves != last / kRefitLeaves) crossList[atomicAdd(crossCount, 1u)] = g;
 }
 
 // ---------------------------------------------------------------------------
@@ -498,123 +529,231 @@ __device__ __forceinline__ void nodeUnion(const Node* P, float* mn, float* mx, f
     maxr = __builtin_fmaxf(f[14], f[15]);
 }
 
-__device__ __forceinline__ bool crossesBlocks(const uint32_t* range, uint32_t node) {
-    return range[2 * node] / kRefitLeaves != range[2 * node + 1] / kRefitLeaves;
+// The refit runs in phases over nested aligned leaf blocks: phase 0 over
+// blocks of 2^kRefitShift leaves, phase p >= 1 over blocks of
+// 2^(kRefitShift + kRefitFanShift * p).  A node belongs to the first phase
+// whose blocks contain its whole leaf range, so each phase's climbs stay
+// inside one block and need only workgroup-scope ordering; later phases see
+// earlier slots through the kernel boundary.
+__device__ __forceinline__ uint32_t refitShift(uint32_t phase) { return kRefitShift + kRefitFanShift * phase; }
+
+// Level of a leaf range: the last phase whose blocks it crosses (-1: none).
+__device__ __forceinline__ int rangeLevel(uint32_t first, uint32_t last) {
+    const uint32_t x = first ^ last;
+    if ((x >> kRefitShift) == 0u) return -1;
+    return int((31u - uint32_t(__builtin_clz(x)) - kRefitShift) / kRefitFanShift);
 }
 
-// Phase 1: one block per kRefitLeaves consecutive leaves climbs through the
-// nodes whose leaves all lie in its range.  The first child to arrive at a
-// node writes its slot and stops; the second unions both slots and climbs
-// on.  Arrivals are counted in LDS, at the node's split (the last leaf of
-// its left child), which is unique among the block's nodes; slots travel
-// through the node records with workgroup-scope ordering only (no
-// device-wide fence).  A node whose leaves span two blocks only receives its
-// child's slot here.  Trees of a batch never share a node, so each climb
-// ends at its molecule's root.
-__global__ __launch_bounds__(512) void k_refit_local(const F4* __restrict__ sorted, uint32_t n, uint32_t internal,
-                                                    Node* nodes, const int32_t* __restrict__ parent,
-                                                    const uint32_t* __restrict__ range, float* rootBox, uint32_t* err) {
+// A node whose children both finish in earlier phases starts a climb in
+// its own phase.  Its split sits on a block boundary of its level, which
+// indexes its slot in that level's start list; the writer of its left slot
+// (the left child's finisher) records it when the right child is also
+// finished before the node's phase.
+__device__ __forceinline__ void recordStart(const RefitStarts& rs, uint32_t p, uint32_t pf, uint32_t pl,
+                                            uint32_t rightFirst, uint32_t* err) {
+    const int level = rangeLevel(pf, pl);
+    if (level < 0 || level >= int(kRefitLevels)) {
+        atomicOr(err, kErrRefitParent);
+        return;
+    }
+    if (((rightFirst ^ pl) >> refitShift(uint32_t(level))) != 0u) return;  // right child finishes in the same phase
+    const uint32_t slot = rightFirst >> refitShift(uint32_t(level));
+    if (slot >= refitSlots(rs.n, uint32_t(level))) {
+        atomicOr(err, kErrRefitParent);
+        return;
+    }
+    rs.list[refitSlotOffset(rs.n, uint32_t(level)) + slot] = p;
+}
+
+// Phase 0: one block per 2^kRefitShift consecutive leaves, one thread per
+// leaf, climbs through the nodes whose leaves all lie in its range.  Their
+// parents and leaf ranges are staged in LDS first: within a molecule a
+// node's index is one end of its leaf range minus the molecule index, so
+// the block's nodes form one window of at most 2^kRefitShift indices.  A
+// child is its parent's left child iff both ranges start at the same leaf.
+// Each arrival at a node inside the block leaves its box in LDS under the
+// node's split (the last leaf of its left child), one slot per side, and
+// counts itself there; the second arrival writes the node's whole record
+// and climbs on.  A node whose leaves leave the block only receives its
+// child's slot here (in the node record), and the child's finisher records
+// the node as a climb start of its phase when due.  Trees of a batch never
+// share a node, so each climb ends at its molecule's root.
+__global__ __launch_bounds__(kRefitLeaves) void k_refit_local(const F4* __restrict__ sorted, uint32_t n,
+                                                              uint32_t internal, Segments seg, Node* nodes,
+                                                              const int32_t* __restrict__ parent,
+                                                              const uint32_t* __restrict__ range, RefitStarts rs,
+                                                              float* rootBox, uint32_t* err) {
+    __shared__ int32_t wParent[kRefitLeaves];
+    __shared__ uint2 wRange[kRefitLeaves];
     __shared__ uint32_t arrivals[kRefitLeaves];
+    __shared__ float xch[2][7][kRefitLeaves];  // [side][min xyz, max xyz, max radius][split - lo]
     const uint32_t lo = blockIdx.x * kRefitLeaves;
     const uint32_t hi = (n - lo) < kRefitLeaves ? n : lo + kRefitLeaves;
-    const bool single = internal + 1u == n;
-    for (uint32_t t = threadIdx.x; t < kRefitLeaves; t += blockDim.x) arrivals[t] = 0u;
+    uint32_t wb = lo, we = hi;
+    if (seg.molecules > 1u) {
+        wb = lo - moleculeOf(seg.offsets, seg.molecules, lo);
+        we = hi - moleculeOf(seg.offsets, seg.molecules, hi - 1u);
+    }
+    we = we < internal ? we : internal;
+    const uint32_t wn = we > wb ? we - wb : 0u;
+    const uint32_t tid = threadIdx.x;
+    if (tid < wn) {
+        wParent[tid] = parent[wb + tid];
+        wRange[tid] = reinterpret_cast<const uint2*>(range)[wb + tid];
+    }
+    arrivals[tid] = 0u;
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        float mn[3], mx[3];
-        leafBox(sorted[i], mn, mx);
-        float maxr = sorted[i].w;
-        uint32_t node = internal + i, first = i, last = i;
-        for (uint32_t climb = 0; climb < n; climb++) {
-            const int p = parent[node];
-            if (p < 0) {
-                if (node >= internal || (single && node != 0u)) {
-                    atomicOr(err, kErrRefitParent);
-                } else if (single) {
-                    for (int c = 0; c < 3; c++) {
-                        rootBox[c] = mn[c];
-                        rootBox[3 + c] = mx[c];
-                    }
-                    rootBox[6] = maxr;
+    const uint32_t i = lo + tid;
+    if (i >= hi) return;
+    const bool single = internal + 1u == n;
+    float mn[3], mx[3];
+    const F4 s = sorted[i];
+    leafBox(s, mn, mx);
+    float maxr = s.w;
+    uint32_t node = internal + i, first = i, last = i;
+    int p = parent[internal + i];
+    for (uint32_t climb = 0; climb < n; climb++) {
+        if (p < 0) {
+            if (node >= internal || (single && node != 0u)) {
+                atomicOr(err, kErrRefitParent);
+            } else if (single) {
+                for (int c = 0; c < 3; c++) {
+                    rootBox[c] = mn[c];
+                    rootBox[3 + c] = mx[c];
                 }
-                break;
+                rootBox[6] = maxr;
             }
-            if (uint32_t(p) >= internal) {
-                atomicOr(err, kErrRefitParent);
-                break;
-            }
-            Node* P = nodes + p;
-            const bool left = uint32_t(P->d.x) == node;
-            writeSlot(P, left, mn, mx, maxr);
-            const uint32_t pf = range[2 * p], pl = range[2 * p + 1];
-            if (pf < lo || pl >= hi) break;  // finished by the second phase
-            const uint32_t split = left ? last : first - 1u;
-            if (split < pf || split >= pl) {
-                atomicOr(err, kErrRefitParent);
-                break;
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            const uint32_t old = atomicAdd(&arrivals[split - lo], 1u);
-            if (old == 0u) break;
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            nodeUnion(P, mn, mx, maxr);
-            node = uint32_t(p);
-            first = pf;
-            last = pl;
+            break;
         }
+        if (uint32_t(p) >= internal) {
+            atomicOr(err, kErrRefitParent);
+            break;
+        }
+        const uint32_t w = uint32_t(p) - wb;
+        const bool inWindow = w < wn;
+        uint32_t pf, pl;
+        if (inWindow) {
+            const uint2 r = wRange[w];
+            pf = r.x;
+            pl = r.y;
+        } else {
+            pf = range[2 * p];
+            pl = range[2 * p + 1];
+        }
+        const bool left = pf == first;
+        if (left ? pl <= last : (pl != last || pf >= first)) {
+            atomicOr(err, kErrRefitParent);
+            break;
+        }
+        if (pf < lo || pl >= hi) {  // finished by a later phase
+            writeSlot(nodes + p, left, mn, mx, maxr);
+            if (left) recordStart(rs, uint32_t(p), pf, pl, last + 1u, err);
+            break;
+        }
+        if (!inWindow) {
+            atomicOr(err, kErrRefitParent);
+            break;
+        }
+        const uint32_t split = (left ? last : first - 1u) - lo;
+        const uint32_t side = left ? 0u : 1u;
+        for (int c = 0; c < 3; c++) {
+            xch[side][c][split] = mn[c];
+            xch[side][3 + c][split] = mx[c];
+        }
+        xch[side][6][split] = maxr;
+        // (LDS-only ordering: the node records written here are read by later
+        // kernels, so no wait for global stores)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        const uint32_t old = atomicAdd(&arrivals[split], 1u);
+        if (old == 0u) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        float lmn[3], lmx[3], rmn[3], rmx[3];
+        for (int c = 0; c < 3; c++) {
+            lmn[c] = xch[0][c][split];
+            lmx[c] = xch[0][3 + c][split];
+            rmn[c] = xch[1][c][split];
+            rmx[c] = xch[1][3 + c][split];
+        }
+        const float lr = xch[0][6][split], rr = xch[1][6][split];
+        Node* P = nodes + p;
+        P->a = make_float4(lmn[0], lmn[1], lmn[2], lmx[0]);
+        P->b = make_float4(lmx[1], lmx[2], rmn[0], rmn[1]);
+        P->c = make_float4(rmn[2], rmx[0], rmx[1], rmx[2]);
+        reinterpret_cast<int2*>(&P->d)[1] = make_int2(__float_as_int(lr), __float_as_int(rr));
+        for (int c = 0; c < 3; c++) {
+            mn[c] = __builtin_fminf(lmn[c], rmn[c]);
+            mx[c] = __builtin_fmaxf(lmx[c], rmx[c]);
+        }
+        maxr = __builtin_fmaxf(lr, rr);
+        node = uint32_t(p);
+        first = pf;
+        last = pl;
+        p = wParent[w];
     }
 }
 
-// Phase 2: one block finishes the nodes whose leaves span two phase-1
-// blocks.  Climbs start at those with no such child; a node continues once
-// all its spanning children have arrived.  One block, so workgroup-scope
-// ordering suffices.
-__global__ __launch_bounds__(1024) void k_refit_top(uint32_t n, uint32_t internal, Node* nodes,
-                                                   const int32_t* __restrict__ parent,
-                                                   const uint32_t* __restrict__ range,
-                                                   const uint32_t* __restrict__ crossList,
-                                                   const uint32_t* __restrict__ crossCount, uint32_t* refitCounter,
-                                                   float* rootBox, uint32_t* err) {
+// Phase p >= 1: one wave per block of 2^refitShift(p) leaves.  Lane t
+// starts at the node recorded for the block's t-th boundary of level p-1
+// (if any) and climbs while the parent stays inside the block.  A parent
+// with a second child of level p-1 is continued by the later of the two
+// (arrival counter in refitCounter, zeroed by k_karras); a parent whose
+// other child finished in an earlier phase is continued at once.
+__global__ __launch_bounds__(64) void k_refit_phase(uint32_t n, uint32_t internal, uint32_t phase, Node* nodes,
+                                                    const int32_t* __restrict__ parent,
+                                                    const uint32_t* __restrict__ range, RefitStarts rs,
+                                                    uint32_t* refitCounter, float* rootBox, uint32_t* err) {
+    const uint32_t level = phase - 1u;
+    const uint32_t slot = blockIdx.x * 64u + threadIdx.x;
+    if (slot >= refitSlots(n, level)) return;
+    uint32_t node = rs.list[refitSlotOffset(n, level) + slot];
+    if (node == ~0u) return;
+    if (node >= internal) {
+        atomicOr(err, kErrRefitParent);
+        return;
+    }
     const bool single = internal + 1u == n;
-    const uint32_t count = *crossCount < internal ? *crossCount : internal;
-    auto spanningChildren = [&](uint32_t q) {
-        const uint32_t l = uint32_t(nodes[q].d.x), r = uint32_t(nodes[q].d.y);
-        return uint32_t(l < internal && crossesBlocks(range, l)) + uint32_t(r < internal && crossesBlocks(range, r));
-    };
-    for (uint32_t u = threadIdx.x; u < count; u += blockDim.x) {
-        uint32_t node = crossList[u];
-        if (node >= internal) {
-            atomicOr(err, kErrRefitParent);
-            continue;
-        }
-        if (spanningChildren(node) != 0u) continue;
-        for (uint32_t climb = 0; climb < n; climb++) {
-            float mn[3], mx[3], maxr;
-            nodeUnion(nodes + node, mn, mx, maxr);
-            const int p = parent[node];
-            if (p < 0) {
-                if (single) {
-                    if (node != 0u) atomicOr(err, kErrRefitParent);
-                    for (int c = 0; c < 3; c++) {
-                        rootBox[c] = mn[c];
-                        rootBox[3 + c] = mx[c];
-                    }
-                    rootBox[6] = maxr;
+    const uint32_t shift = refitShift(phase), inner = refitShift(level);
+    uint32_t first = range[2 * node], last = range[2 * node + 1];
+    for (uint32_t climb = 0; climb < n; climb++) {
+        float mn[3], mx[3], maxr;
+        nodeUnion(nodes + node, mn, mx, maxr);
+        const int p = parent[node];
+        if (p < 0) {
+            if (single) {
+                if (node != 0u) atomicOr(err, kErrRefitParent);
+                for (int c = 0; c < 3; c++) {
+                    rootBox[c] = mn[c];
+                    rootBox[3 + c] = mx[c];
                 }
-                break;
+                rootBox[6] = maxr;
             }
-            if (uint32_t(p) >= internal) {
-                atomicOr(err, kErrRefitParent);
-                break;
-            }
-            Node* P = nodes + p;
-            writeSlot(P, P->d.x == int(node), mn, mx, maxr);
+            break;
+        }
+        if (uint32_t(p) >= internal) {
+            atomicOr(err, kErrRefitParent);
+            break;
+        }
+        const uint32_t pf = range[2 * p], pl = range[2 * p + 1];
+        const bool left = pf == first;
+        if (left ? pl <= last : (pl != last || pf >= first)) {
+            atomicOr(err, kErrRefitParent);
+            break;
+        }
+        writeSlot(nodes + p, left, mn, mx, maxr);
+        if (((pf ^ pl) >> shift) != 0u) {  // finished by a later phase
+            if (left) recordStart(rs, uint32_t(p), pf, pl, last + 1u, err);
+            break;
+        }
+        const uint32_t sf = left ? last + 1u : pf, sl = left ? pl : first - 1u;
+        if (((sf ^ sl) >> inner) != 0u) {  // the sibling climbs in this phase too
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             const uint32_t old = atomicAdd(&refitCounter[p], 1u);
-            if (old + 1u < spanningChildren(uint32_t(p))) break;
+            if (old == 0u) break;
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            node = uint32_t(p);
         }
+        node = uint32_t(p);
+        first = pf;
+        last = pl;
     }
 }
 
@@ -630,38 +769,108 @@ __global__ __launch_bounds__(256) void k_copy_sorted(const uint64_t* __restrict_
 
 // Depth of the deepest leaf (root depth 0).  A depth-first traversal keeps
 // at most one stacked node per level of its current path, so no stack of
-// this depth or more can overflow.
-__global__ __launch_bounds__(256) void k_tree_height(const int32_t* __restrict__ parent, uint32_t n,
-                                                     uint32_t internal, uint32_t* height) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t d = 0;
-    if (i < n)
-        for (int32_t p = parent[internal + i]; p >= 0 && d < n; p = parent[p]) d++;
-    for (int o = 32; o >= 1; o >>= 1) {
-        const uint32_t other = __shfl_xor(d, o, 64);
-        d = other > d ? other : d;
+// this depth or more can overflow.  A leaf's depth is the number of
+// internal nodes whose leaf range holds it: each node adds +1 at its first
+// leaf and -1 after its last, and the largest prefix sum is the height.
+// Block b takes the nodes and owns the mark positions [4096 b, 4096 b +
+// 4096): marks that land there are summed in LDS and stored once; the rest
+// (large nodes and block edges) go to `far` with global atomics.
+constexpr uint32_t kDepthMarks = 4096;  // nodes and positions per block (1024 threads x 4)
+
+__global__ __launch_bounds__(1024) void k_depth_marks(const uint32_t* __restrict__ range, uint32_t n,
+                                                      uint32_t internal, int32_t* __restrict__ diff, int32_t* far) {
+    __shared__ int32_t own[kDepthMarks];
+    const uint32_t lo = blockIdx.x * kDepthMarks;
+    for (uint32_t t = threadIdx.x; t < kDepthMarks; t += blockDim.x) own[t] = 0;
+    __syncthreads();
+    auto mark = [&](uint32_t x, int32_t v) {
+        if (x - lo < kDepthMarks) atomicAdd(&own[x - lo], v);
+        else atomicAdd(&far[x], v);
+    };
+    for (uint32_t q = 0; q < kDepthMarks / 1024u; q++) {
+        const uint32_t g = lo + q * 1024u + threadIdx.x;
+        if (g >= internal) break;
+        const uint2 r = reinterpret_cast<const uint2*>(range)[g];
+        mark(r.x, 1);
+        if (r.y + 1u < n) mark(r.y + 1u, -1);
     }
-    if ((threadIdx.x & 63u) == 0u) atomicMax(height, d);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kDepthMarks && lo + t < n; t += blockDim.x) diff[lo + t] = own[t];
+}
+
+constexpr uint32_t kDepthBlock = 4096;  // prefix-sum elements per block (1024 threads x 4)
+
+// A run of marks as (sum, largest prefix sum); runs join in order.
+__device__ __forceinline__ int2 joinRuns(int2 a, int2 b) {
+    const int32_t m = a.x + b.y;
+    return make_int2(a.x + b.x, a.y > m ? a.y : m);
+}
+
+// Ordered join over the block: each thread holds consecutive runs.
+__device__ __forceinline__ int2 blockJoin(int2 r, int2* waveRuns) {
+    for (int o = 1; o < 64; o <<= 1) {
+        const int2 other = make_int2(__shfl_down(r.x, o, 64), __shfl_down(r.y, o, 64));
+        if (int(laneId()) + o < 64) r = joinRuns(r, other);
+    }
+    const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6;
+    if (laneId() == 0) waveRuns[wave] = r;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (uint32_t w = 1; w < waves; w++) r = joinRuns(r, waveRuns[w]);
+    return r;
+}
+
+// Per block of kDepthBlock marks: its sum and its largest prefix sum.
+__global__ __launch_bounds__(1024) void k_depth_scan(const int32_t* __restrict__ diff, const int32_t* __restrict__ far,
+                                                     uint32_t n, int2* blocks) {
+    __shared__ int2 waveRuns[16];
+    const uint32_t base = blockIdx.x * kDepthBlock + threadIdx.x * 4u;
+    int2 r = make_int2(0, 0);
+    for (uint32_t q = 0; q < 4u; q++) {
+        r.x += base + q < n ? diff[base + q] + far[base + q] : 0;
+        r.y = r.y > r.x ? r.y : r.x;
+    }
+    r = blockJoin(r, waveRuns);
+    if (threadIdx.x == 0) blocks[blockIdx.x] = r;
+}
+
+// One block joins the blocks' runs in order: the largest prefix overall.
+__global__ __launch_bounds__(1024) void k_depth_final(const int2* __restrict__ blocks, uint32_t count,
+                                                      uint32_t* height) {
+    __shared__ int2 waveRuns[16];
+    const uint32_t per = (count + blockDim.x - 1u) / blockDim.x;
+    const uint32_t b0 = threadIdx.x * per;
+    int2 r = make_int2(0, 0);
+    for (uint32_t b = b0; b < b0 + per && b < count; b++) r = joinRuns(r, blocks[b]);
+    r = blockJoin(r, waveRuns);
+    if (threadIdx.x == 0) *height = uint32_t(r.y > 0 ? r.y : 0);
 }
 
 // ---------------------------------------------------------------------------
 // host-side launcher
 // ---------------------------------------------------------------------------
-hipError_t treeHeightLaunch(const int32_t* parent, uint32_t n, uint32_t internal, uint32_t* height, hipStream_t st) {
-    hipLaunchKernelGGL(k_tree_height, dim3((n + 255u) / 256u), dim3(256), 0, st, parent, n, internal, height);
+hipError_t treeHeightLaunch(const uint32_t* range, uint32_t n, uint32_t internal, int32_t* diff, int32_t* far,
+                            int2* blocks, uint32_t* height, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_words, dim3((n + 255u) / 256u), dim3(256), 0, st, reinterpret_cast<uint32_t*>(far), n,
+                       0u);
+    hipLaunchKernelGGL(k_depth_marks, dim3((n + kDepthMarks - 1u) / kDepthMarks), dim3(1024), 0, st, range, n,
+                       internal, diff, far);
+    const uint32_t nb = (n + kDepthBlock - 1u) / kDepthBlock;
+    hipLaunchKernelGGL(k_depth_scan, dim3(nb), dim3(1024), 0, st, diff, far, n, blocks);
+    hipLaunchKernelGGL(k_depth_final, dim3(1), dim3(1024), 0, st, blocks, nb, height);
     return hipGetLastError();
 }
 
 size_t sortScratchWords(uint32_t n) {
     const size_t tiles = (n + kTileKeys - 1) / kTileKeys;
-    return size_t(kMaxRadixPasses) * 256u + kMaxRadixPasses + size_t(kMaxRadixPasses) * tiles * 256u + 1u;
+    return size_t(kMaxRadixPasses) * 256u + kMaxRadixPasses + size_t(kMaxRadixPasses) * tiles * 256u;
 }
 
 hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const Segments& seg, hipStream_t st) {
     const uint32_t tiles = (n + kTileKeys - 1) / kTileKeys;
     const int passes = seg.molecules > 1u ? radixPassesFor(seg.molecules) : kRadixPasses;
-    // sort counters and look-back, then the spanning-node counter
-    const uint32_t zeroWords = uint32_t(passes) * 256u + uint32_t(passes) + uint32_t(passes) * tiles * 256u + 1u;
+    // sort counters and look-back
+    const uint32_t zeroWords = uint32_t(passes) * 256u + uint32_t(passes) + uint32_t(passes) * tiles * 256u;
     if (seg.molecules > 1u) {
         hipLaunchKernelGGL(k_scene_segments, dim3(seg.molecules), dim3(256), 0, st, sitesIn, seg.offsets, b.scene,
                            b.sortScratch, zeroWords);
@@ -673,7 +882,8 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     uint32_t* hist = b.sortScratch;
     uint32_t* tileCounters = hist + passes * 256u;
     uint32_t* lookback = tileCounters + passes;
-    const uint32_t mortonBlocks = (n + 255u) / 256u < 2048u ? (n + 255u) / 256u : 2048u;
+    // (few blocks: each flushes passes x 256 histogram counts with global atomics)
+    const uint32_t mortonBlocks = (n + 255u) / 256u < 512u ? (n + 255u) / 256u : 512u;
     hipLaunchKernelGGL(k_morton, dim3(mortonBlocks), dim3(256), 0, st, sitesIn, n, seg, passes, b.scene, b.codesA,
                        b.idxA, hist);
     uint64_t* kin = b.codesA;
@@ -682,10 +892,10 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     uint32_t* vout = b.idxB;
     for (int p = 0; p < passes; p++) {
         if (p < kRadixPasses)
-            hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(256), 0, st, kin, vin, kout, vout, n, p, seg,
+            hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n, p, seg,
                                hist + p * 256, lookback + size_t(p) * tiles * 256u, tileCounters + p, tiles, b.err);
         else
-            hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(256), 0, st, kin, vin, kout, vout, n, p, seg,
+            hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n, p, seg,
                                hist + p * 256, lookback + size_t(p) * tiles * 256u, tileCounters + p, tiles, b.err);
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
@@ -696,14 +906,15 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     }
     const uint32_t blocks = (n + 255u) / 256u;
     const uint32_t internal = n - seg.molecules;
-    uint32_t* crossCount = lookback + size_t(passes) * tiles * 256u;
+    const RefitStarts rs{b.startLists, n};
     hipLaunchKernelGGL(k_karras, dim3(blocks), dim3(256), 0, st, b.codesA, b.idxA, sitesIn, n, seg, b.sorted, b.nodes,
-                       b.parent, b.refitCounter, b.sizes, b.range, b.crossList, crossCount, b.err);
+                       b.parent, b.refitCounter, b.sizes, b.range, rs, b.err);
     const uint32_t refitBlocks = (n + kRefitLeaves - 1u) / kRefitLeaves;
-    hipLaunchKernelGGL(k_refit_local, dim3(refitBlocks), dim3(512), 0, st, b.sorted, n, internal, b.nodes, b.parent,
-                       b.range, b.rootBox, b.err);
-    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, st, n, internal, b.nodes, b.parent, b.range, b.crossList,
-                       crossCount, b.refitCounter, b.rootBox, b.err);
+    hipLaunchKernelGGL(k_refit_local, dim3(refitBlocks), dim3(kRefitLeaves), 0, st, b.sorted, n, internal, seg,
+                       b.nodes, b.parent, b.range, rs, b.rootBox, b.err);
+    for (uint32_t phase = 1; phase <= refitPhases(n); phase++)
+        hipLaunchKernelGGL(k_refit_phase, dim3((refitSlots(n, phase - 1u) + 63u) / 64u), dim3(64), 0, st, n, internal,
+                           phase, b.nodes, b.parent, b.range, rs, b.refitCounter, b.rootBox, b.err);
     return hipGetLastError();
 }
 

@@ -417,20 +417,26 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
 // Karras 2012 hierarchy (lbvh.cu:144-223) + gather of the sorted sites
 // (setLeaves, lbvh.cu:88-114).  Thread i handles internal node i and leaf i.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int deltaK(const uint64_t* __restrict__ c, int i, int j, int n) {
-    if (j < 0 || j >= n) return -1;
-    const uint64_t a = c[i], b = c[j];
-    if (a != b) return __builtin_clzll(a ^ b);
-    return __builtin_clzll(uint64_t(int64_t(i ^ j))) + 64;
-}
+constexpr uint32_t kKarrasBlock = 256;
+constexpr uint32_t kKarrasHalo = 256;  // codes staged in LDS on each side of the block's own
 
-__global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ codes, const uint32_t* __restrict__ indices,
-                                                const F4* __restrict__ sitesIn, uint32_t n, Segments seg,
-                                                F4* __restrict__ sorted, Node* __restrict__ nodes,
-                                                int32_t* __restrict__ parent, uint32_t* __restrict__ refitCounter,
-                                                uint32_t* __restrict__ sizes, uint32_t* __restrict__ range,
-                                                RefitStarts rs, uint32_t* err) {
-    const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kKarrasBlock) void k_karras(const uint64_t* __restrict__ codes,
+                                                         const uint32_t* __restrict__ indices,
+                                                         const F4* __restrict__ sitesIn, uint32_t n, Segments seg,
+                                                         F4* __restrict__ sorted, Node* __restrict__ nodes,
+                                                         int32_t* __restrict__ parent,
+                                                         uint32_t* __restrict__ refitCounter,
+                                                         uint32_t* __restrict__ sizes, uint32_t* __restrict__ range,
+                                                         RefitStarts rs, uint32_t* err) {
+    // most comparisons of a node touch codes a few positions away: the block
+    // stages its codes and a halo on each side in LDS
+    __shared__ uint64_t wc[kKarrasBlock + 2 * kKarrasHalo];
+    const uint32_t blo = blockIdx.x * kKarrasBlock;
+    const uint32_t wlo = blo > kKarrasHalo ? blo - kKarrasHalo : 0u;
+    const uint32_t whi = n - blo > kKarrasBlock + kKarrasHalo ? blo + kKarrasBlock + kKarrasHalo : n;
+    for (uint32_t t = threadIdx.x; t < whi - wlo; t += kKarrasBlock) wc[t] = codes[wlo + t];
+    __syncthreads();
+    const uint32_t gi = blo + threadIdx.x;
     if (gi >= n) return;
     if (gi < refitStartWords(n)) rs.list[gi] = ~0u;
     const uint32_t src = indices[gi];
@@ -453,25 +459,34 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ cod
     const int i = int(gi - base);
     const int ni = int(count);
     if (i >= ni - 1) return;
-    const uint64_t* c = codes + base;
+    const uint64_t ci = wc[gi - wlo];
+    // delta(i, j): common prefix of the codes, or 64 + that of the indices
+    // when the codes are equal; -1 outside the molecule
+    auto delta = [&](int j) -> int {
+        if (j < 0 || j >= ni) return -1;
+        const uint32_t gp = base + uint32_t(j);
+        const uint64_t cj = gp - wlo < whi - wlo ? wc[gp - wlo] : codes[gp];
+        if (ci != cj) return __builtin_clzll(ci ^ cj);
+        return __builtin_clzll(uint64_t(int64_t(i ^ j))) + 64;
+    };
     const uint32_t nodeBase = base - m;  // the molecule's first internal node (its root)
     const uint32_t g = nodeBase + uint32_t(i);
     refitCounter[g] = 0u;
-    const int dd = deltaK(c, i, i + 1, ni) - deltaK(c, i, i - 1, ni);
+    const int dd = delta(i + 1) - delta(i - 1);
     const int d = (dd > 0) - (dd < 0);
-    const int deltaMin = deltaK(c, i, i - d, ni);
+    const int deltaMin = delta(i - d);
     int lmax = 2;
-    while (deltaK(c, i, i + lmax * d, ni) > deltaMin) lmax <<= 1;
+    while (delta(i + lmax * d) > deltaMin) lmax <<= 1;
     int l = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (deltaK(c, i, i + (l + t) * d, ni) > deltaMin) l += t;
+        if (delta(i + (l + t) * d) > deltaMin) l += t;
     const int j = i + l * d;
-    const int deltaNode = deltaK(c, i, j, ni);
+    const int deltaNode = delta(j);
     int s = 0, k = 2, t;
     do {
         t = (l + k - 1) / k;
         k <<= 1;
-        if (deltaK(c, i, i + (s + t) * d, ni) > deltaNode) s += t;
+        if (delta(i + (s + t) * d) > deltaNode) s += t;
     } while (t > 1);
     const int gamma = i + s * d + (d < 0 ? d : 0);
     // a child whose range is one leaf is that leaf, else the internal node
@@ -479,15 +494,13 @@ __global__ __launch_bounds__(256) void k_karras(const uint64_t* __restrict__ cod
     const uint32_t lc = (i < j ? i : j) == gamma ? internal + base + uint32_t(gamma) : nodeBase + uint32_t(gamma);
     const uint32_t rc = (i > j ? i : j) == gamma + 1 ? internal + base + uint32_t(gamma) + 1u
                                                      : nodeBase + uint32_t(gamma) + 1u;
-    nodes[g].d.x = int32_t(lc);
-    nodes[g].d.y = int32_t(rc);
+    reinterpret_cast<int2*>(&nodes[g].d)[0] = make_int2(int32_t(lc), int32_t(rc));
     parent[lc] = int32_t(g);
     parent[rc] = int32_t(g);
     if (i == 0) parent[g] = -1;
     // leaf range (a node's range starts or ends at its own local index)
     const uint32_t first = base + uint32_t(i < j ? i : j), last = base + uint32_t(i > j ? i : j);
-    range[2 * g] = first;
-    range[2 * g + 1] = last;
+    reinterpret_cast<uint2*>(range)[g] = make_uint2(first, last);
     sizes[g] = last - first + 1u;
 }
 
@@ -904,10 +917,9 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
         // an odd number of passes leaves the sorted data in B: move it to A
         hipLaunchKernelGGL(k_copy_sorted, dim3((n + 255u) / 256u), dim3(256), 0, st, kin, vin, b.codesA, b.idxA, n);
     }
-    const uint32_t blocks = (n + 255u) / 256u;
     const uint32_t internal = n - seg.molecules;
     const RefitStarts rs{b.startLists, n};
-    hipLaunchKernelGGL(k_karras, dim3(blocks), dim3(256), 0, st, b.codesA, b.idxA, sitesIn, n, seg, b.sorted, b.nodes,
+    hipLaunchKernelGGL(k_karras, dim3((n + kKarrasBlock - 1u) / kKarrasBlock), dim3(kKarrasBlock), 0, st, b.codesA, b.idxA, sitesIn, n, seg, b.sorted, b.nodes,
                        b.parent, b.refitCounter, b.sizes, b.range, rs, b.err);
     const uint32_t refitBlocks = (n + kRefitLeaves - 1u) / kRefitLeaves;
     hipLaunchKernelGGL(k_refit_local, dim3(refitBlocks), dim3(kRefitLeaves), 0, st, b.sorted, n, internal, seg,

@@ -430,6 +430,12 @@ def main():
                      "bytes_per_launch": bytes_k / max(int(launches), 1),
                      "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
+        # every persistent kernel's algorithmic bytes and launch time (the
+        # counter passes of each one are compared against these)
+        "kernels": {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[2] / args.steps,
+                        "bytes_per_launch": v[1] / max(v[2], 1),
+                        "achieved_gbs": v[1] / (v[0] * 1e-3) / 1e9 if v[0] > 0 else None}
+                    for k, v in kern.items()},
     }
     out["tuning"] = {"source": autotune.cache_path("loguniform_1m") if tuned else None,
                      "plain": plain_tuning, "learned": learned_tuning}

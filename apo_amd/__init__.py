@@ -5,7 +5,11 @@ This mirrors the construction API of PlathC/apo
 
     AlgorithmGPU(sites, k=16).build()           -> AlgorithmGPU::build
     AlgorithmGPU.to_host()                      -> AlgorithmGPU::toHost -> VertexDiagram
+    AlgorithmGPU.to_host_full()                 -> AlgorithmGPU::toHostFull -> FullDiagram
+    AlgorithmGPU.topology()                     -> AlgorithmGPU::toTopology -> Topology sets
     compute_apollonius_diagram(sites)           -> apo::gpu::computeApolloniusDiagram
+    compute_full_apollonius_diagram(sites)      -> apo::gpu::computeFullApolloniusDiagram
+    topology_json(topology)                     -> apo::toCsv (app/topology/topology.cpp:11-57)
     AlgorithmGPU(sites, offsets=o).build()      -> one build of many independent molecules
 
 Everything runs in the HIP library apo_amd/lib/libapo_hip.so through its C
@@ -21,7 +25,8 @@ import numpy as np
 from . import _lib
 from ._lib import ApoError, library_path  # noqa: F401
 
-__all__ = ["AlgorithmGPU", "VertexDiagram", "compute_apollonius_diagram", "ApoError", "library_path"]
+__all__ = ["AlgorithmGPU", "VertexDiagram", "FullDiagram", "compute_apollonius_diagram",
+           "compute_full_apollonius_diagram", "topology_json", "ApoError", "library_path"]
 
 
 @dataclass
@@ -30,6 +35,17 @@ class VertexDiagram:
     with w the empty-sphere radius, and the 4 tangent sites (original indices)."""
     vertices: np.ndarray      # (V, 4) float64
     vertices_id: np.ndarray   # (V, 4) uint32
+
+
+@dataclass
+class FullDiagram:
+    """Host result of apo's toHostFull (algorithm.hpp:18-26): the vertices of
+    VertexDiagram plus every closed trisector edge as its extreme point of
+    smaller empty-sphere radius (x, y, z, w) and its 3 sites (i, j, k, 0)."""
+    vertices: np.ndarray          # (V, 4) float64
+    vertices_id: np.ndarray       # (V, 4) uint32
+    closed_edges_min: np.ndarray  # (E, 4) float64
+    closed_edges_id: np.ndarray   # (E, 4) uint32
 
 
 class AlgorithmGPU:
@@ -112,6 +128,20 @@ class AlgorithmGPU:
         _lib.check(self._ctx, L.apo_get_vertices(self._ctx, _lib.dptr(xyzw), _lib.uptr(ids), v))
         return VertexDiagram(xyzw, ids)
 
+    # -- AlgorithmGPU::toHostFull -------------------------------------------
+    def to_host_full(self) -> FullDiagram:
+        """Vertices and closed trisector edges (algorithm.inl:1648-1730)."""
+        L = _lib.lib()
+        vd = self.to_host()
+        cnt = _lib.c_uint64()
+        _lib.check(self._ctx, L.apo_closed_edge_count(self._ctx, _lib.byref(cnt)))
+        e = int(cnt.value)
+        xyzw = np.zeros((e, 4), np.float64)
+        ids = np.zeros((e, 4), np.uint32)
+        if e:
+            _lib.check(self._ctx, L.apo_get_closed_edges(self._ctx, _lib.dptr(xyzw), _lib.uptr(ids), e))
+        return FullDiagram(vd.vertices, vd.vertices_id, xyzw, ids)
+
     def stats(self) -> dict:
         s = _lib.Stats()
         _lib.check(self._ctx, _lib.lib().apo_get_stats(self._ctx, _lib.byref(s)))
@@ -130,6 +160,23 @@ class AlgorithmGPU:
                                                     _lib.byref(cnt))
         _lib.check(self._ctx, st)
         return int(cnt.value)
+
+    def precision(self) -> dict:
+        """Interval validator of app/precision/precision.cpp:14-163 on the
+        build's vertices and closed edges: counts of points, of bad ones (some
+        other site certainly closer) and of uncertain (near-degenerate) ones."""
+        r = _lib.Precision()
+        _lib.check(self._ctx, _lib.lib().apo_precision_check(self._ctx, _lib.byref(r)))
+        return {name: int(getattr(r, name)) for name, _ in r._fields_}
+
+    def precision_points(self, xyzw, ids, arity: int = 4):
+        """The interval validator on given points: (bad, uncertain)."""
+        x = np.ascontiguousarray(xyzw, np.float64).reshape(-1, 4)
+        i = np.ascontiguousarray(ids, np.uint32).reshape(-1, 4)
+        bad, unc = _lib.c_uint64(), _lib.c_uint64()
+        _lib.check(self._ctx, _lib.lib().apo_precision_check_points(self._ctx, _lib.dptr(x), _lib.uptr(i), len(x),
+                                                                    arity, _lib.byref(bad), _lib.byref(unc)))
+        return int(bad.value), int(unc.value)
 
     # -- device state (public members of AlgorithmGPU) ------------------------
     def sorted_sites(self):
@@ -224,3 +271,29 @@ def compute_apollonius_diagram(sites, k: int = 16) -> VertexDiagram:
         return alg.build().to_host()
     finally:
         alg.close()
+
+
+def compute_full_apollonius_diagram(sites, k: int = 16) -> FullDiagram:
+    """apo::gpu::computeFullApolloniusDiagram (apo/src/gpu/algorithm.cu:17-26)."""
+    alg = AlgorithmGPU(sites, k=k)
+    try:
+        return alg.build().to_host_full()
+    finally:
+        alg.close()
+
+
+def topology_json(topology: dict) -> str:
+    """The topology document of the reference's topology app (apo::toCsv,
+    app/topology/topology.cpp:11-57): a JSON object with "bisectors",
+    "trisectors" and "quadrisectors" arrays, one tab-indented entry per line,
+    in the order AlgorithmGPU.topology() returns them."""
+    out = ["{\n"]
+    for key, width, last in (("bisectors", 2, False), ("trisectors", 3, False), ("quadrisectors", 4, True)):
+        rows = np.asarray(topology[key], np.uint32).reshape(-1, width)
+        out.append(f'\t"{key}": [\n')
+        m = len(rows)
+        for r, row in enumerate(rows):
+            out.append("\t\t[ " + ", ".join(str(int(x)) for x in row) + " ]" + ("" if r == m - 1 else ",") + "\n")
+        out.append("\t]\n" if last else "\t],\n")
+    out.append("}\n")
+    return "".join(out)

@@ -29,6 +29,13 @@ class Config(ctypes.Structure):
                 ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32)]
 
 
+class Precision(ctypes.Structure):
+    """apo_precision (include/apo_c.h)."""
+    _fields_ = [(name, ctypes.c_uint64) for name in (
+        "quadrisectors", "bad_quadrisectors", "uncertain_quadrisectors",
+        "trisectors", "bad_trisectors", "uncertain_trisectors")]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("ms_total", ctypes.c_double), ("ms_lbvh", ctypes.c_double), ("ms_knn", ctypes.c_double),
                 ("ms_cells", ctypes.c_double), ("ms_validation", ctypes.c_double),
@@ -68,6 +75,10 @@ SIGNATURES = {
     "apo_vertex_count": (ctypes.c_int, [_vp, _u64p]),
     "apo_get_vertices": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64]),
     "apo_get_device_vertices": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "apo_closed_edge_count": (ctypes.c_int, [_vp, _u64p]),
+    "apo_precision_check": (ctypes.c_int, [_vp, ctypes.POINTER(Precision)]),
+    "apo_precision_check_points": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64, _u32, _u64p, _u64p]),
+    "apo_get_closed_edges": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64]),
     "apo_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
     "apo_check_empty_spheres": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64, ctypes.c_double, _u64p]),
     "apo_get_sorted_sites": (ctypes.c_int, [_vp, _f32p, _u32p]),

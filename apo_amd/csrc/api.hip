@@ -712,6 +712,55 @@ apo_status apo_get_device_vertices(apo_ctx* ctx, const double** xyzw, const uint
     return APO_OK;
 }
 
+namespace {
+// Closed trisector edges of the current build: count (and write, when xyzw
+// or ids is given) on the context's stream with temporary buffers; the
+// build's own output buffers are left untouched.
+apo_status closedEdges(apo_ctx* ctx, uint64_t* count, double* xyzw, uint32_t* ids, uint64_t capacity) {
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t n = ctx->n;
+    const uint32_t blocks = uint32_t((n + 255u) / 256u);
+    DeviceTemp<uint32_t> cnt, sums, total;
+    APO_HIP(cnt.alloc(n));
+    APO_HIP(sums.alloc(blocks));
+    APO_HIP(total.alloc(1));
+    const CellCtx cc = cellCtx(ctx);
+    OutputBuffers ob{cnt.p, sums.p, total.p, nullptr, nullptr, 0};
+    APO_HIP(closedEdgesCountLaunch(cc, ob, st));
+    uint32_t e = 0;
+    APO_HIP(hipMemcpyAsync(&e, total.p, 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    *count = e;
+    if (!xyzw && !ids) return APO_OK;
+    if (capacity < e) return fail(ctx, APO_ERR_CAPACITY, "output buffer too small");
+    DeviceTemp<D4> dx;
+    DeviceTemp<uint4> di;
+    APO_HIP(dx.alloc(e));
+    APO_HIP(di.alloc(e));
+    ob.xyzw = dx.p;
+    ob.ids = di.p;
+    ob.capacity = e;
+    APO_HIP(closedEdgesWriteLaunch(cc, ob, ctx->idxA, st));
+    if (xyzw) APO_HIP(hipMemcpyAsync(xyzw, dx.p, size_t(e) * sizeof(D4), hipMemcpyDeviceToHost, st));
+    if (ids) APO_HIP(hipMemcpyAsync(ids, di.p, size_t(e) * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+}  // namespace
+
+apo_status apo_closed_edge_count(apo_ctx* ctx, uint64_t* count) {
+    if (!ctx || !count) return APO_ERR_INVALID;
+    return closedEdges(ctx, count, nullptr, nullptr, 0);
+}
+
+apo_status apo_get_closed_edges(apo_ctx* ctx, double* xyzw, uint32_t* ids, uint64_t capacity) {
+    if (!ctx || (!xyzw && !ids)) return APO_ERR_INVALID;
+    uint64_t count = 0;
+    return closedEdges(ctx, &count, xyzw, ids, capacity);
+}
+
 apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats) {
     if (!ctx || !stats) return APO_ERR_INVALID;
     if (!ctx->built) return fail(ctx, APO_ERR_STATE, "no build");
@@ -776,6 +825,81 @@ apo_status apo_check_empty_spheres(apo_ctx* ctx, const double* xyzw, const uint3
     if (ui) APO_HIP(hipFree(ui));
     if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped in the empty-sphere check: " + guardText(guard));
     *violations = bad;
+    return APO_OK;
+}
+
+namespace {
+// The interval validator over device points; counts[0] bad, counts[1] uncertain.
+apo_status precisionOnDevice(apo_ctx* ctx, const D4* dx, const uint4* di, uint64_t count, uint32_t arity,
+                             uint64_t* bad, uint64_t* uncertain) {
+    hipStream_t st = ctx->stream;
+    DeviceTemp<unsigned long long> dc;
+    APO_HIP(dc.alloc(2));
+    APO_HIP(hipMemsetAsync(dc.p, 0, 2 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
+    APO_HIP(precisionLaunch(cellCtx(ctx), ctx->idxA, ctx->cellCount, dx, di, count, arity, dc.p, st));
+    unsigned long long h[2] = {0, 0};
+    uint32_t guard = 0;
+    APO_HIP(hipMemcpyAsync(h, dc.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(guard), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped in the precision check: " + guardText(guard));
+    *bad = h[0];
+    *uncertain = h[1];
+    return APO_OK;
+}
+}  // namespace
+
+apo_status apo_precision_check_points(apo_ctx* ctx, const double* xyzw, const uint32_t* ids, uint64_t count,
+                                      uint32_t arity, uint64_t* bad, uint64_t* uncertain) {
+    if (!ctx || !bad || !uncertain || (count && (!xyzw || !ids)) || (arity != 3u && arity != 4u))
+        return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    APO_HIP(hipSetDevice(ctx->device));
+    DeviceTemp<D4> dx;
+    DeviceTemp<uint4> di;
+    APO_HIP(dx.alloc(count));
+    APO_HIP(di.alloc(count));
+    if (count) {
+        APO_HIP(hipMemcpyAsync(dx.p, xyzw, count * sizeof(D4), hipMemcpyHostToDevice, ctx->stream));
+        APO_HIP(hipMemcpyAsync(di.p, ids, count * sizeof(uint4), hipMemcpyHostToDevice, ctx->stream));
+    }
+    return precisionOnDevice(ctx, dx.p, di.p, count, arity, bad, uncertain);
+}
+
+apo_status apo_precision_check(apo_ctx* ctx, apo_precision* out) {
+    if (!ctx || !out) return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    APO_HIP(hipSetDevice(ctx->device));
+    apo_precision r{};
+    r.quadrisectors = ctx->vertexCount;
+    apo_status s = precisionOnDevice(ctx, ctx->outXyzw, ctx->outIds, ctx->vertexCount, 4u, &r.bad_quadrisectors,
+                                     &r.uncertain_quadrisectors);
+    if (s != APO_OK) return s;
+    // closed trisector edges, written to temporary device buffers
+    uint64_t e = 0;
+    s = apo_closed_edge_count(ctx, &e);
+    if (s != APO_OK) return s;
+    r.trisectors = e;
+    if (e) {
+        const size_t n = ctx->n;
+        const uint32_t blocks = uint32_t((n + 255u) / 256u);
+        DeviceTemp<uint32_t> cnt, sums, total;
+        DeviceTemp<D4> dx;
+        DeviceTemp<uint4> di;
+        APO_HIP(cnt.alloc(n));
+        APO_HIP(sums.alloc(blocks));
+        APO_HIP(total.alloc(1));
+        APO_HIP(dx.alloc(e));
+        APO_HIP(di.alloc(e));
+        const CellCtx cc = cellCtx(ctx);
+        const OutputBuffers ob{cnt.p, sums.p, total.p, dx.p, di.p, e};
+        APO_HIP(closedEdgesCountLaunch(cc, ob, ctx->stream));
+        APO_HIP(closedEdgesWriteLaunch(cc, ob, ctx->idxA, ctx->stream));
+        s = precisionOnDevice(ctx, dx.p, di.p, e, 3u, &r.bad_trisectors, &r.uncertain_trisectors);
+        if (s != APO_OK) return s;
+    }
+    *out = r;
     return APO_OK;
 }
 

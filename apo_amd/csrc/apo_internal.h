@@ -249,6 +249,11 @@ struct OutputBuffers {
 };
 hipError_t outputCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st);
 hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices, hipStream_t st);
+// Closed trisector edges of the full diagram (algorithm.inl:1309-1383):
+// count into cellCount / blockSums / total, then write at most capacity.
+hipError_t closedEdgesCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st);
+hipError_t closedEdgesWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices,
+                                  hipStream_t st);
 // Vertex offsets of each molecule of a batch (molecules + 1 values).
 hipError_t moleculeVerticesLaunch(const CellCtx& c, const OutputBuffers& o, uint64_t* out, hipStream_t st);
 
@@ -301,5 +306,11 @@ hipError_t cellRecordsLaunch(const CellCtx& c, const uint32_t* indices, const ui
 hipError_t checkEmptyLaunch(const CellCtx& c, const uint32_t* indices, uint32_t* rank, const D4* xyzw,
                             const uint4* ids, uint64_t count, double tol, unsigned long long* violations,
                             hipStream_t st);
+// Interval validator (app/precision/precision.cpp:14-163): counts2[0] +=
+// points some other site is certainly closer to, counts2[1] += the others
+// some site may be closer to.  arity 4: vertices, 3: closed-edge points.
+hipError_t precisionLaunch(const CellCtx& c, const uint32_t* indices, uint32_t* rank, const D4* xyzw,
+                           const uint4* ids, uint64_t count, uint32_t arity, unsigned long long* counts2,
+                           hipStream_t st);
 
 }  // namespace apo_hip

@@ -188,6 +188,81 @@ hipError_t outputWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uin
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Closed trisector edges of the full diagram: count -> scan -> write
+// (getWriteableClosedEdges / saveClosedEdges, algorithm.inl:1309-1383).  An
+// edge record (j < k, kNone vertices) is a closed elliptic trisector; the
+// cell with the smallest id emits it (i <= j, both real sites), at the
+// extreme point of smaller empty-sphere radius.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool closedEmitted(uint32_t i, uint4 e, uint32_t n) {
+    return !(e.z != kNone || i > e.x || e.x >= n || e.y >= n);
+}
+
+__global__ __launch_bounds__(256) void k_closed_count(CellCtx c, uint32_t* cellCount, uint32_t* blockSums) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    uint32_t cnt = 0;
+    if (i < c.n && c.status[i] != kStatusBuried) {
+        const uint4* ge = c.edg + size_t(i) * kEdgeCap;
+        const uint32_t E = min(c.enb[i], kEdgeCap);
+        for (uint32_t e = 0; e < E; e++) cnt += closedEmitted(i, ge[e], c.n) ? 1u : 0u;
+    }
+    if (i < c.n) cellCount[i] = cnt;
+    __shared__ uint32_t s[4];
+    const uint32_t sum = waveSumU(cnt);
+    if (laneId() == 0) s[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) blockSums[blockIdx.x] = s[0] + s[1] + s[2] + s[3];
+}
+
+__global__ __launch_bounds__(256) void k_closed_write(CellCtx c, const uint32_t* cellCount,
+                                                      const uint32_t* blockOffsets,
+                                                      const uint32_t* __restrict__ indices, D4* xyzw, uint4* ids,
+                                                      uint64_t capacity) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t cnt = i < c.n ? cellCount[i] : 0u;
+    const uint32_t inc = waveInclusiveSum(cnt);
+    __shared__ uint32_t s[4];
+    if (laneId() == 63) s[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint32_t off = blockOffsets[blockIdx.x];
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); w++) off += s[w];
+    off += inc - cnt;
+    if (i >= c.n || cnt == 0) return;
+    const D4 si = loadSite(c, i);
+    const uint4* ge = c.edg + size_t(i) * kEdgeCap;
+    const uint32_t E = min(c.enb[i], kEdgeCap);
+    for (uint32_t e = 0; e < E; e++) {
+        const uint4 r = ge[e];
+        if (!closedEmitted(i, r, c.n)) continue;
+        // the reference leaves both extreme points uninitialised when the
+        // solve has fewer than two; here they start at zero
+        D4 x0 = {0.0, 0.0, 0.0, 0.0}, x1 = {0.0, 0.0, 0.0, 0.0};
+        trisector(si, loadSite(c, r.x), loadSite(c, r.y), x0, x1);
+        if (x1.w < x0.w) x0 = x1;
+        if (off < capacity) {
+            xyzw[off] = x0;
+            ids[off] = make_uint4(indices[i], indices[r.x], indices[r.y], 0u);
+        }
+        off++;
+    }
+}
+
+hipError_t closedEdgesCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st) {
+    const uint32_t blocks = (c.n + 255u) / 256u;
+    hipLaunchKernelGGL(k_closed_count, dim3(blocks), dim3(256), 0, st, c, o.cellCount, o.blockSums);
+    hipLaunchKernelGGL(k_out_scan, dim3(1), dim3(1024), 0, st, o.blockSums, blocks, o.total);
+    return hipGetLastError();
+}
+
+hipError_t closedEdgesWriteLaunch(const CellCtx& c, const OutputBuffers& o, const uint32_t* indices,
+                                  hipStream_t st) {
+    const uint32_t blocks = (c.n + 255u) / 256u;
+    hipLaunchKernelGGL(k_closed_write, dim3(blocks), dim3(256), 0, st, c, o.cellCount, o.blockSums, indices, o.xyzw,
+                       o.ids, o.capacity);
+    return hipGetLastError();
+}
+
 // Vertex offsets of the molecules of a batch: the exclusive prefix of the
 // per-cell vertex counts at each molecule's first cell (after the scan,
 // blockSums holds the exclusive prefix of each 256-cell block).

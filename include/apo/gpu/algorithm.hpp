@@ -1,8 +1,11 @@
 // apo/gpu/algorithm.hpp -- C++17 host interface over the C ABI (apo_c.h).
 //
 // Restores the construction API of the reference, apo/include/apo/gpu/
-// algorithm.hpp:10-59 and algorithm.cuh:12-46: VertexDiagram, AlgorithmGPU
-// (build, toHost, the stage-time Sample) and computeApolloniusDiagram.
+// algorithm.hpp:10-59 and algorithm.cuh:12-46: VertexDiagram, FullDiagram,
+// Topology, AlgorithmGPU (build, toHost, toHostFull, toTopology, the
+// stage-time Sample), computeApolloniusDiagram and
+// computeFullApolloniusDiagram; and the topology document of the reference's
+// topology app (toCsv, app/topology/topology.cpp:11-57) as apo::toJson.
 // Sites are (x, y, z, r) FP32 quadruples, 4n floats; vertices come back as
 // FP64 (x, y, z, w) with w the empty-sphere radius, and their four tangent
 // sites as original input indices.
@@ -16,6 +19,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -40,6 +44,23 @@ struct VertexDiagram {
     std::vector<double> vertices;          // {x, y, z, w} per vertex
     std::vector<uint32_t> verticesId;      // {i, j, k, l} per vertex, original site indices
     std::vector<uint32_t> invalidationId;  // never filled (as in the reference, algorithm.hpp:14)
+};
+
+// FullDiagram (algorithm.hpp:18-26): the vertices plus each closed trisector
+// edge as its extreme point of smaller empty-sphere radius.
+struct FullDiagram {
+    std::vector<double> vertices;          // {x, y, z, w} per vertex
+    std::vector<uint32_t> verticesId;      // {i, j, k, l} per vertex
+    std::vector<uint32_t> invalidationId;  // never filled
+    std::vector<uint32_t> closedEdgesId;   // {i, j, k, 0} per closed edge
+    std::vector<double> closedEdgesMin;    // {x, y, z, w} per closed edge
+};
+
+// Topology (algorithm.hpp:28-33), original site ids.
+struct Topology {
+    std::vector<uint32_t> quadrisectors;  // uint4
+    std::vector<uint32_t> trisectors;     // uint3
+    std::vector<uint32_t> bisectors;      // uint2
 };
 
 class AlgorithmGPU {
@@ -85,6 +106,39 @@ class AlgorithmGPU {
         return d;
     }
 
+    // toHostFull (algorithm.inl:1648-1730); the flag is ignored.
+    FullDiagram toHostFull(bool validate = true) {
+        VertexDiagram v = toHost(validate);
+        FullDiagram d;
+        d.vertices = std::move(v.vertices);
+        d.verticesId = std::move(v.verticesId);
+        uint64_t count = 0;
+        check(apo_closed_edge_count(m_ctx, &count), "apo_closed_edge_count");
+        d.closedEdgesMin.resize(4 * count);
+        d.closedEdgesId.resize(4 * count);
+        if (count)
+            check(apo_get_closed_edges(m_ctx, d.closedEdgesMin.data(), d.closedEdgesId.data(), count),
+                  "apo_get_closed_edges");
+        return d;
+    }
+
+    // toTopology (algorithm.inl:1732-1821)
+    Topology toTopology() {
+        uint64_t c[3] = {0, 0, 0};
+        check(apo_get_topology_sets(m_ctx, nullptr, 0, nullptr, 0, nullptr, 0, c), "apo_get_topology_sets");
+        Topology t;
+        t.bisectors.resize(2 * c[0] + 2);
+        t.trisectors.resize(3 * c[1] + 3);
+        t.quadrisectors.resize(4 * c[2] + 4);
+        check(apo_get_topology_sets(m_ctx, t.bisectors.data(), c[0], t.trisectors.data(), c[1],
+                                    t.quadrisectors.data(), c[2], c),
+              "apo_get_topology_sets");
+        t.bisectors.resize(2 * c[0]);
+        t.trisectors.resize(3 * c[1]);
+        t.quadrisectors.resize(4 * c[2]);
+        return t;
+    }
+
     apo_stats stats() {
         apo_stats s;
         check(apo_get_stats(m_ctx, &s), "apo_get_stats");
@@ -112,6 +166,39 @@ inline VertexDiagram computeApolloniusDiagram(ConstSpan sites) {
     return algo.toHost(true);
 }
 
+// computeFullApolloniusDiagram (src/gpu/algorithm.cu:17-26)
+inline FullDiagram computeFullApolloniusDiagram(ConstSpan sites) {
+    AlgorithmGPU algo(sites);
+    algo.build();
+    return algo.toHostFull(true);
+}
+
 }  // namespace apo::gpu
+
+namespace apo {
+
+// The topology document the reference's topology app writes (toCsv,
+// app/topology/topology.cpp:11-57): bisectors, trisectors, quadrisectors.
+inline std::string toJson(const gpu::Topology& t) {
+    std::ostringstream o;
+    auto section = [&o](const char* name, const std::vector<uint32_t>& v, std::size_t width, bool last) {
+        o << "\t\"" << name << "\": [\n";
+        const std::size_t m = v.size() / width;
+        for (std::size_t r = 0; r < m; r++) {
+            o << "\t\t[ ";
+            for (std::size_t c = 0; c < width; c++) o << v[r * width + c] << (c + 1 < width ? ", " : " ]");
+            o << (r + 1 == m ? "" : ",") << "\n";
+        }
+        o << (last ? "\t]\n" : "\t],\n");
+    };
+    o << "{\n";
+    section("bisectors", t.bisectors, 2, false);
+    section("trisectors", t.trisectors, 3, false);
+    section("quadrisectors", t.quadrisectors, 4, true);
+    o << "}\n";
+    return o.str();
+}
+
+}  // namespace apo
 
 #endif  // APO_GPU_ALGORITHM_HPP

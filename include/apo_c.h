@@ -141,6 +141,17 @@ apo_status apo_get_vertices(apo_ctx* ctx, double* xyzw, uint32_t* ids, uint64_t 
 /* Device pointers to the vertex records (valid until the next build). */
 apo_status apo_get_device_vertices(apo_ctx* ctx, const double** xyzw, const uint32_t** ids);
 
+/* Closed trisector edges of AlgorithmGPU<K>::toHostFull (algorithm.inl:
+ * 1648-1730, kernels getWriteableClosedEdges / saveClosedEdges at
+ * :1309-1383; FullDiagram::closedEdgesMin / closedEdgesId of
+ * apo/include/apo/gpu/algorithm.hpp:18-26): each closed elliptic trisector
+ * once, from the cell of its smallest site, as the extreme point of smaller
+ * empty-sphere radius (x, y, z, w in FP64) and ids (i, j, k, 0) in original
+ * input indices, in cell order.  Two-call pattern: query the count, then
+ * copy (either array may be NULL, not both). */
+apo_status apo_closed_edge_count(apo_ctx* ctx, uint64_t* count);
+apo_status apo_get_closed_edges(apo_ctx* ctx, double* xyzw, uint32_t* ids, uint64_t capacity);
+
 apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats);
 
 /* FP64 empty-sphere check through the LBVH of the last build, with the
@@ -150,6 +161,28 @@ apo_status apo_get_stats(apo_ctx* ctx, apo_stats* stats);
  * (original site ids); pass NULL for both to check the build's own output. */
 apo_status apo_check_empty_spheres(apo_ctx* ctx, const double* xyzw, const uint32_t* ids, uint64_t count, double tol,
                                    uint64_t* violations);
+
+/* Interval precision validator of the reference's precision app
+ * (app/precision/precision.cpp:14-163: getBadQuadrisectorNb and
+ * getBadTrisectorNb), through the LBVH instead of a scan of every site.  For
+ * each point, the weighted distances |p - c| - r to its own sites and to
+ * every other site are FP64 intervals with outward rounding (the bounds of
+ * Boost.Interval with directed rounding); a point is bad when some other
+ * site's interval lies entirely below the largest of its own sites'
+ * intervals, and uncertain (near-degenerate: FP64 cannot decide it) when it
+ * is not bad and some other site's interval reaches below that one.  The
+ * uncertain count has no counterpart in the reference. */
+typedef struct apo_precision {
+    uint64_t quadrisectors, bad_quadrisectors, uncertain_quadrisectors;  /* the build's vertices */
+    uint64_t trisectors, bad_trisectors, uncertain_trisectors;           /* its closed edges */
+} apo_precision;
+/* The build's own vertices and closed edges. */
+apo_status apo_precision_check(apo_ctx* ctx, apo_precision* out);
+/* Host points: count x (x, y, z, w) (w unused) and count x 4 original site
+ * ids; arity 4 for vertices, 3 for closed-edge points (ids[3] unused).
+ * Each point is checked against the sites of its first site's molecule. */
+apo_status apo_precision_check_points(apo_ctx* ctx, const double* xyzw, const uint32_t* ids, uint64_t count,
+                                      uint32_t arity, uint64_t* bad, uint64_t* uncertain);
 
 /* Topology sets of AlgorithmGPU::toTopology (algorithm.inl:1732-1821): for
  * every cell in sorted order, its distinct bisectors {i, j}, trisectors

@@ -80,6 +80,10 @@ def lib():
         L.orc_vertex_count.argtypes = [ctypes.c_void_p]
         L.orc_get_vertices.restype = None
         L.orc_get_vertices.argtypes = [ctypes.c_void_p, f64p, u32p]
+        L.orc_closed_edge_count.restype = ctypes.c_uint64
+        L.orc_closed_edge_count.argtypes = [ctypes.c_void_p]
+        L.orc_get_closed_edges.restype = None
+        L.orc_get_closed_edges.argtypes = [ctypes.c_void_p, f64p, u32p]
         L.orc_get_sorted_sites.restype = None
         L.orc_get_sorted_sites.argtypes = [ctypes.c_void_p, f32p, u32p]
         L.orc_get_cell_counts.restype = None
@@ -101,6 +105,13 @@ def lib():
                                                   ctypes.c_double, ctypes.c_uint32]
         L.orc_bruteforce_vertices.restype = ctypes.c_uint64
         L.orc_bruteforce_vertices.argtypes = [f32p, ctypes.c_uint32, u32p, ctypes.c_uint64, ctypes.c_double]
+        L.orc_precision_counts.restype = None
+        L.orc_precision_counts.argtypes = [f32p, ctypes.c_uint32, f64p, u32p, ctypes.c_uint64, ctypes.c_uint32,
+                                           ctypes.c_uint32, u64p]
+        L.orc_round_dir.restype = None
+        L.orc_round_dir.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_double, f64p]
+        L.orc_interval_distance.restype = None
+        L.orc_interval_distance.argtypes = [f64p, f64p, f64p]
         _lib = L
     return _lib
 
@@ -247,6 +258,15 @@ class Diagram:
             lib().orc_get_vertices(self._h, _p(xyzw, f64p), _p(ids, u32p))
         return xyzw, ids
 
+    def closed_edges(self):
+        """toHostFull's closed trisector edges: (x, y, z, w) and (i, j, k, 0)."""
+        e = int(lib().orc_closed_edge_count(self._h))
+        xyzw = np.zeros((e, 4), np.float64)
+        ids = np.zeros((e, 4), np.uint32)
+        if e:
+            lib().orc_get_closed_edges(self._h, _p(xyzw, f64p), _p(ids, u32p))
+        return xyzw, ids
+
     def sorted_sites(self):
         s = np.zeros((self.n, 4), np.float32)
         idx = np.zeros(self.n, np.uint32)
@@ -313,3 +333,30 @@ def bruteforce_vertices(xyzr, tol: float = 1e-9, cap: int = 1 << 16):
     ids = np.zeros((cap, 4), np.uint32)
     cnt = int(lib().orc_bruteforce_vertices(_p(s, f32p), len(s), _p(ids, u32p), cap, tol))
     return ids[:min(cnt, cap)]
+
+
+def precision_counts(xyzr, xyzw, ids, arity: int = 4, threads: int = 0):
+    """Interval validator of app/precision/precision.cpp:14-163 by a scan of
+    every site: (bad, uncertain) points.  arity 4: vertices, 3: closed edges."""
+    s = _sites(xyzr)
+    x = np.ascontiguousarray(xyzw, np.float64).reshape(-1, 4)
+    i = np.ascontiguousarray(ids, np.uint32).reshape(-1, 4)
+    out = np.zeros(2, np.uint64)
+    lib().orc_precision_counts(_p(s, f32p), len(s), _p(x, f64p), _p(i, u32p), len(x), arity, threads,
+                               _p(out, u64p))
+    return int(out[0]), int(out[1])
+
+
+def round_dir(op: str, a: float, b: float = 0.0):
+    """(down, up) of a+b, a*b or sqrt(a) under the switched rounding mode."""
+    out = np.zeros(2)
+    lib().orc_round_dir({"add": 0, "mul": 1, "sqrt": 2}[op], a, b, _p(out, f64p))
+    return float(out[0]), float(out[1])
+
+
+def interval_distance(site, point):
+    s = np.ascontiguousarray(site, np.float64)
+    p = np.ascontiguousarray(point, np.float64)
+    out = np.zeros(2)
+    lib().orc_interval_distance(_p(s, f64p), _p(p, f64p), _p(out, f64p))
+    return float(out[0]), float(out[1])

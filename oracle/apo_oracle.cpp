@@ -1529,6 +1529,8 @@ struct orc_diagram {
     std::vector<uint32_t> knn;
     std::vector<double> outXyzw;
     std::vector<uint32_t> outIds;
+    std::vector<double> edgeXyzw;
+    std::vector<uint32_t> edgeIds;
     uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
@@ -1784,6 +1786,22 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
         }
     }
     d->stats[4] = d->outIds.size() / 4;
+
+    // toHostFull's closed edges: getWriteableClosedEdges + saveClosedEdges
+    // (algorithm.inl:1309-1383).  Both extreme points start at zero where the
+    // reference leaves them uninitialised.
+    for (uint32_t i = 0; i < n; i++) {
+        if (C.status[i] == kBuried) continue;
+        const D4 si = C.S[i];
+        for (const Edg& e : C.cells[i].E) {
+            if (e.vertexNb != kNone || i > e.j || e.j >= n || e.k >= n) continue;
+            D4 x[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+            trisector(si, C.S[e.j], C.S[e.k], &x[0], &x[1]);
+            if (x[1].w < x[0].w) x[0] = x[1];
+            d->edgeXyzw.insert(d->edgeXyzw.end(), {x[0].x, x[0].y, x[0].z, x[0].w});
+            d->edgeIds.insert(d->edgeIds.end(), {d->bvh.indices[i], d->bvh.indices[e.j], d->bvh.indices[e.k], 0u});
+        }
+    }
     return d;
 }
 
@@ -1805,6 +1823,13 @@ uint64_t orc_vertex_count(const orc_diagram* d) { return d->outIds.size() / 4; }
 void orc_get_vertices(const orc_diagram* d, double* xyzw, uint32_t* ids) {
     std::memcpy(xyzw, d->outXyzw.data(), d->outXyzw.size() * sizeof(double));
     std::memcpy(ids, d->outIds.data(), d->outIds.size() * sizeof(uint32_t));
+}
+
+uint64_t orc_closed_edge_count(const orc_diagram* d) { return d->edgeIds.size() / 4; }
+
+void orc_get_closed_edges(const orc_diagram* d, double* xyzw, uint32_t* ids) {
+    std::memcpy(xyzw, d->edgeXyzw.data(), d->edgeXyzw.size() * sizeof(double));
+    std::memcpy(ids, d->edgeIds.data(), d->edgeIds.size() * sizeof(uint32_t));
 }
 
 void orc_get_sorted_sites(const orc_diagram* d, float* xyzr, uint32_t* indices) {

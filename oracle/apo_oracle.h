@@ -95,6 +95,11 @@ void         orc_free(orc_diagram* d);
 uint64_t orc_vertex_count(const orc_diagram* d);
 void     orc_get_vertices(const orc_diagram* d, double* xyzw, uint32_t* ids);
 
+/* Closed trisector edges (apo toHostFull semantics, algorithm.inl:1309-1383):
+ * extreme point of smaller radius (x,y,z,w) and ids (i,j,k,0), original ids. */
+uint64_t orc_closed_edge_count(const orc_diagram* d);
+void     orc_get_closed_edges(const orc_diagram* d, double* xyzw, uint32_t* ids);
+
 /* Per-cell state after the pipeline (sorted space). */
 void orc_get_sorted_sites(const orc_diagram* d, float* xyzr, uint32_t* indices);
 void orc_get_cell_counts(const orc_diagram* d, uint32_t* vertex_nb, uint32_t* edge_nb, uint8_t* status,
@@ -122,6 +127,17 @@ void orc_certified_k(const float* xyzr, uint32_t n, uint32_t kmax, uint32_t* out
  * strictly below the vertex radius minus tol), checked by brute force.     */
 uint64_t orc_empty_sphere_violations(const float* xyzr, uint32_t n, const double* xyzw, const uint32_t* ids,
                                      uint64_t v, double tol, uint32_t threads);
+/* Interval precision validator (app/precision/precision.cpp:14-163,
+ * apo_interval.cpp): Boost.Interval semantics with the rounding mode switched
+ * per bound.  xyzw: count points (w unused); ids: count x 4 original ids
+ * (arity 3: closed-edge points, ids[3] unused).  out2 = {bad, uncertain}. */
+void orc_precision_counts(const float* xyzr, uint32_t n, const double* xyzw, const uint32_t* ids, uint64_t count,
+                          uint32_t arity, uint32_t threads, uint64_t* out2);
+/* One directed-rounding operation: op 0 a+b, 1 a*b, 2 sqrt(a); lohi = {down, up}. */
+void orc_round_dir(uint32_t op, double a, double b, double* lohi);
+/* Interval weighted distance |p - c| - r of one site (precision.cpp:14-24). */
+void orc_interval_distance(const double* site4, const double* p3, double* lohi);
+
 /* All empty tangent spheres of 4 sites, by enumeration of quadruples
  * (tiny n only).  Returns the count; writes up to cap id quadruples sorted. */
 uint64_t orc_bruteforce_vertices(const float* xyzr, uint32_t n, uint32_t* ids, uint64_t cap, double tol);

@@ -168,3 +168,15 @@ def test_cell_topology_csr(built):
         want_e = np.stack([mapped[re[:, 0]], mapped[re[:, 1]], re[:, 2], np.zeros(len(re), np.uint32)], axis=1)
         assert np.array_equal(v[vo[o]:vo[o + 1]], want_v), (name, i)
         assert np.array_equal(e[eo[o]:eo[o + 1]], want_e), (name, i)
+
+
+def test_closed_edges_exact(built):
+    # toHostFull's closed trisector edges (algorithm.inl:1309-1383): same
+    # edges, same order, same FP64 extreme points
+    name, sites, gpu, orc = built
+    g = gpu.to_host_full()
+    ox, oi = orc.closed_edges()
+    print(f"[closed-edges] {name}: {len(oi)}", flush=True)
+    assert np.array_equal(g.vertices_id, orc.vertices()[1]), name
+    assert np.array_equal(g.closed_edges_id, oi), name
+    assert np.array_equal(g.closed_edges_min.view(np.uint64), ox.view(np.uint64)), name

@@ -60,7 +60,7 @@ class AlgorithmGPU:
 
     def __init__(self, sites, k: int = 16, max_iterations: int = 10, max_restarts: int = 100,
                  stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0, proposal=None,
-                 proposal_cost: float = 0.0, offsets=None, graph: bool = False, **tuning):
+                 proposal_cost: float = 0.0, offsets=None, graph: bool = False, classes=None, **tuning):
         self.sites = np.ascontiguousarray(np.asarray(sites, dtype=np.float32).reshape(-1, 4))
         self.offsets = None if offsets is None else np.ascontiguousarray(offsets, np.uint32)
         self.k = int(k)
@@ -85,6 +85,11 @@ class AlgorithmGPU:
         if proposal is not None:
             self._proposal = np.ascontiguousarray(proposal, np.float32)
             _lib.check(self._ctx, _lib.lib().apo_proposal_set(self._ctx, _lib.fptr(self._proposal)))
+        if classes is not None:
+            # per-site proposal classes (input order) instead of the MLP's
+            self._classes = np.ascontiguousarray(classes, np.uint8)
+            _lib.check(self._ctx, _lib.lib().apo_proposal_set_classes(self._ctx, _lib.u8ptr(self._classes),
+                                                                      len(self._classes)))
 
     def close(self):
         if getattr(self, "_ctx", None):

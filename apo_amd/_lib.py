@@ -26,7 +26,8 @@ class Config(ctypes.Structure):
                 ("stage_mask", ctypes.c_uint32), ("waves_per_cu", ctypes.c_uint32), ("device", ctypes.c_uint32),
                 ("proposal_cost", ctypes.c_float), ("graph", ctypes.c_uint32), ("cover_ratio", ctypes.c_float),
                 ("waves_cells", ctypes.c_uint32), ("waves_vertex", ctypes.c_uint32), ("waves_edge", ctypes.c_uint32),
-                ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32)]
+                ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32),
+                ("proposal_max_k", ctypes.c_uint32)]
 
 
 class Precision(ctypes.Structure):
@@ -96,6 +97,7 @@ SIGNATURES = {
     "apo_proposal_set": (ctypes.c_int, [_vp, _f32p]),
     "apo_proposal_infer": (ctypes.c_int, [_vp, _f32p, _u32, _u8p, _f32p]),
     "apo_proposal_classes": (ctypes.c_int, [_vp, _u8p]),
+    "apo_proposal_set_classes": (ctypes.c_int, [_vp, _u8p, _u32]),
 }
 
 

@@ -11,9 +11,10 @@ time to produce it does:
 
 The learned path adds the neighbours per pass of its fallback class
 (proposal_fallback_k: the neighbour count of sites predicted not to
-certify within 64) and the validation cost of its class choice
-(proposal_cost); these change which cells reach validation, never the
-certification test, so its output stays the certified diagram.
+certify), the most neighbours any other class may run (proposal_max_k,
+which also sets the length of its kNN lists) and the validation cost of its
+class choice (proposal_cost); these change which cells reach validation,
+never the certification test, so its output stays the certified diagram.
 
 Fixed on purpose: the plain path's 16 neighbours (the reference's default,
 part of its output) and the traversal stack depth of 32 entries (a
@@ -21,7 +22,8 @@ shallower stack would change results where it overflows).
 
 Search: coordinate descent.  Each knob in turn is swept with the others
 held, scored by the device time of the kernel it drives (apo_stats, median
-of `repeats` builds), and the best value kept.  The tuned plain
+of `repeats` builds), and the best value kept; the learned knobs, which
+interact (the list length bounds every class), get two passes.  The tuned plain
 configuration is then checked to emit exactly the default configuration's
 vertices.  Results are cached as JSON keyed by device and workload.
 """
@@ -46,29 +48,32 @@ PLAIN_KNOBS = {
     "waves_bisector": "ms_bisector",
     "cover_ratio": "ms_edge_validation_kernel",
 }
-LEARNED_KNOBS = {"proposal_fallback_k": None, "proposal_cost": None}
+LEARNED_KNOBS = {"proposal_max_k": None, "proposal_fallback_k": None, "proposal_cost": None}
 # (a negative ratio, never covering, is valid but leaves large edge searches
 # unpruned: minutes per build at 1M sites, so it is not a candidate)
 COVER_RATIOS = (0.0, 0.5, 1.0, 2.0, 4.0, 8.0, 16.0, 32.0)
-FALLBACK_K = (16, 32, 48, 64)
+FALLBACK_K = (16, 24, 32, 48, 64)
+MAX_K = (24, 32, 48, 64)
 PROPOSAL_COSTS = (0.0, 100.0, 200.0, 400.0, 800.0)
 
 
 def coordinate_descent(knobs: Sequence[str], candidates: Dict[str, Sequence], start: Dict[str, object],
-                       score: Callable[[Dict[str, object], str], float]) -> (Dict[str, object], List[dict]):
-    """One pass over `knobs`: for each, try every candidate value with the
-    other knobs at their current best and keep the lowest score.  `score`
+                       score: Callable[[Dict[str, object], str], float],
+                       passes: int = 1) -> (Dict[str, object], List[dict]):
+    """`passes` passes over `knobs`: for each, try every candidate value with
+    the other knobs at their current best and keep the lowest score.  `score`
     receives the full configuration and the knob being tuned."""
     best = dict(start)
     log = []
-    for knob in knobs:
-        trial = {}
-        for value in candidates[knob]:
-            cfg = dict(best)
-            cfg[knob] = value
-            trial[value] = score(cfg, knob)
-            log.append({"knob": knob, "value": value, "score": trial[value]})
-        best[knob] = min(trial, key=lambda v: (trial[v], str(v)))
+    for _ in range(passes):
+        for knob in knobs:
+            trial = {}
+            for value in candidates[knob]:
+                cfg = dict(best)
+                cfg[knob] = value
+                trial[value] = score(cfg, knob)
+                log.append({"knob": knob, "value": value, "score": trial[value]})
+            best[knob] = min(trial, key=lambda v: (trial[v], str(v)))
     return best, log
 
 
@@ -211,10 +216,11 @@ def tune(sites, workload: str, offsets=None, proposal=None, repeats: int = 2,
             say(f"tune {knob}={cfg[knob]}: {t:.2f} ms")
             return t
 
-        lstart = dict(best, proposal_fallback_k=16, proposal_cost=400.0)
-        lbest, llog = coordinate_descent(list(LEARNED_KNOBS), {"proposal_fallback_k": FALLBACK_K,
+        lstart = dict(best, proposal_max_k=64, proposal_fallback_k=16, proposal_cost=400.0)
+        lbest, llog = coordinate_descent(list(LEARNED_KNOBS), {"proposal_max_k": MAX_K,
+                                                               "proposal_fallback_k": FALLBACK_K,
                                                                "proposal_cost": PROPOSAL_COSTS}, lstart,
-                                         learned_score)
+                                         learned_score, passes=2)
         result["learned"] = lbest
         result["learned_log"] = llog
     if cache:

@@ -30,11 +30,11 @@ namespace {
 struct GraphKey {
     const void* sites = nullptr;
     uint32_t n = 0, molecules = 0;
-    bool learned = false;
+    bool learned = false, given = false;
     uint64_t outCap = 0;
     apo_config cfg{};
     bool operator==(const GraphKey& o) const {
-        return sites == o.sites && n == o.n && molecules == o.molecules && learned == o.learned &&
+        return sites == o.sites && n == o.n && molecules == o.molecules && learned == o.learned && given == o.given &&
                outCap == o.outCap && std::memcmp(&cfg, &o.cfg, sizeof(cfg)) == 0;
     }
 };
@@ -116,6 +116,9 @@ struct apo_ctx {
     uint8_t* cls = nullptr;     // predicted class per sorted site
     uint8_t* labels = nullptr;  // label pass output per sorted site
     uint32_t capP = 0;          // site capacity of feat / cls / labels
+    uint8_t* givenCls = nullptr;  // classes given by the caller (input order), used instead of the MLP
+    uint32_t givenN = 0;          // their site count (0: none given)
+    uint32_t givenCap = 0;
     hipEvent_t ev[8] = {};
     ValidationTiming vt{};
     apo_stats stats{};
@@ -223,6 +226,9 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     ctx->capK = K;
     return APO_OK;
 }
+
+// The learned path runs with MLP weights or with classes given by the caller.
+bool learnedPath(const apo_ctx* ctx) { return ctx->proposalOn || ctx->givenN != 0u; }
 
 apo_status ensureProposalBuffers(apo_ctx* ctx, uint32_t n) {
     if (n <= ctx->capP) return APO_OK;
@@ -333,7 +339,8 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     APO_HIP(lbvhBuild(lbvhBuffers(ctx), sitesDev, n, segments(ctx), st));
     APO_TIMED_STAGE("lbvh");
     APO_HIP(mark(1));
-    const uint32_t Kk = learned ? 64u : K;
+    // the learned path's lists hold the most neighbours any class may ask for
+    const uint32_t Kk = learned ? proposalListK(ctx->cfg.proposal_max_k, ctx->cfg.proposal_fallback_k) : K;
     APO_HIP(knnLaunch(cc, Kk, ctx->knn, st));
     APO_TIMED_STAGE("knn");
     APO_HIP(mark(2));
@@ -341,9 +348,13 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     if (learned && (ctx->cfg.stage_mask & 1u)) {
         // features of the 16 nearest, proposal classes, then each cell with its
         // proposed neighbour count and the plain certification test
-        APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st));
-        APO_HIP(mlpInferLaunch(ctx->feat, n, ctx->pw, ctx->cls, nullptr, ctx->cfg.proposal_cost,
-                               ctx->cfg.proposal_fallback_k, st));
+        if (ctx->givenN) {
+            APO_HIP(gatherClassesLaunch(ctx->idxA, ctx->givenCls, n, ctx->cls, st));
+        } else {
+            APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st));
+            APO_HIP(mlpInferLaunch(ctx->feat, n, ctx->pw, ctx->cls, nullptr, ctx->cfg.proposal_cost,
+                                   ctx->cfg.proposal_fallback_k, ctx->cfg.proposal_max_k, st));
+        }
         APO_HIP(mark(7));
         APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, ctx->cls, nullptr, kProposal, ctx->cfg.proposal_fallback_k, st));
     } else {
@@ -467,7 +478,7 @@ apo_status finishBuild(apo_ctx* ctx, uint32_t n, bool learned, bool timed) {
 apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     hipStream_t st = ctx->stream;
     ctx->n = n;
-    const bool learned = ctx->proposalOn;
+    const bool learned = learnedPath(ctx);
     const bool graph = ctx->cfg.graph != 0u && !(ctx->cfg.stage_mask & 8u);
     if (!graph) {
         apo_status s = enqueueBuild(ctx, sitesDev, n, learned, true);
@@ -480,6 +491,7 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     key.molecules = ctx->molecules;
     key.cfg = ctx->cfg;
     key.learned = learned;
+    key.given = ctx->givenN != 0u;
     key.outCap = ctx->outCap;
     if (!ctx->graphExec || !(key == ctx->graphKey)) {
         dropGraph(ctx);
@@ -538,6 +550,7 @@ void apo_default_config(apo_config* cfg) {
     cfg->cover_ratio = 2.f;
     cfg->waves_cells = cfg->waves_vertex = cfg->waves_edge = cfg->waves_bisector = 0;
     cfg->proposal_fallback_k = 16;
+    cfg->proposal_max_k = 64;
 }
 
 apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
@@ -546,9 +559,9 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     apo_ctx* ctx = new apo_ctx();
     if (cfg) ctx->cfg = *cfg;
     else apo_default_config(&ctx->cfg);
-    const uint32_t k = ctx->cfg.k, fk = ctx->cfg.proposal_fallback_k;
-    if (!(k == 16 || k == 24 || k == 32 || k == 48 || k == 64) || ctx->cfg.max_iterations > kMaxRounds ||
-        !(fk == 16 || fk == 24 || fk == 32 || fk == 48 || fk == 64)) {
+    const uint32_t k = ctx->cfg.k, fk = ctx->cfg.proposal_fallback_k, mk = ctx->cfg.proposal_max_k;
+    auto listK = [](uint32_t v) { return v == 16 || v == 24 || v == 32 || v == 48 || v == 64; };
+    if (!listK(k) || ctx->cfg.max_iterations > kMaxRounds || !listK(fk) || !listK(mk)) {
         delete ctx;
         return APO_ERR_INVALID;
     }
@@ -579,7 +592,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
-                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -619,9 +632,11 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     if (n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count must be below 2^28");
     APO_HIP(hipSetDevice(ctx->device));
     ctx->built = false;
-    apo_status s = ensureCapacity(ctx, n, ctx->proposalOn ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
+    if (ctx->givenN && ctx->givenN != n)
+        return fail(ctx, APO_ERR_INVALID, "proposal classes were given for another site count");
+    apo_status s = ensureCapacity(ctx, n, learnedPath(ctx) ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
     if (s != APO_OK) return s;
-    if (ctx->proposalOn) {
+    if (learnedPath(ctx)) {
         s = ensureProposalBuffers(ctx, n);
         if (s != APO_OK) return s;
     }
@@ -1268,10 +1283,35 @@ apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t coun
     if (logits) APO_HIP(dL.alloc(size_t(count) * kProposalClasses));
     APO_HIP(hipMemcpyAsync(dF.p, features, size_t(count) * kFeatures * 4, hipMemcpyHostToDevice, st));
     APO_HIP(mlpInferLaunch(dF.p, count, ctx->pw, dC.p, logits ? dL.p : nullptr, ctx->cfg.proposal_cost,
-                           ctx->cfg.proposal_fallback_k, st));
+                           ctx->cfg.proposal_fallback_k, ctx->cfg.proposal_max_k, st));
     APO_HIP(hipMemcpyAsync(classes, dC.p, count, hipMemcpyDeviceToHost, st));
     if (logits) APO_HIP(hipMemcpyAsync(logits, dL.p, size_t(count) * kProposalClasses * 4, hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+
+apo_status apo_proposal_set_classes(apo_ctx* ctx, const uint8_t* classes, uint32_t n) {
+    if (!ctx) return APO_ERR_INVALID;
+    APO_HIP(hipSetDevice(ctx->device));
+    if (!classes) {
+        if (ctx->givenN) dropGraph(ctx);
+        ctx->givenN = 0;
+        return APO_OK;
+    }
+    if (n < 2 || n >= (1u << 28)) return fail(ctx, APO_ERR_INVALID, "site count out of range");
+    for (uint32_t i = 0; i < n; i++)
+        if (classes[i] >= kProposalClasses) return fail(ctx, APO_ERR_INVALID, "proposal class above 7");
+    if (n > ctx->givenCap) {
+        dropGraph(ctx);
+        if (ctx->givenCls) APO_HIP(hipFree(ctx->givenCls));
+        ctx->givenCls = nullptr;
+        ctx->givenCap = 0;
+        APO_HIP(alloc(ctx->givenCls, n));
+        ctx->givenCap = n;
+    }
+    APO_HIP(hipMemcpyAsync(ctx->givenCls, classes, n, hipMemcpyHostToDevice, ctx->stream));
+    APO_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->givenN = n;
     return APO_OK;
 }
 

@@ -269,6 +269,8 @@ __host__ __device__ constexpr uint32_t proposalK(uint32_t c, uint32_t fallback =
     return c == 0 ? 20u : c == 1 ? 24u : c == 2 ? 28u : c == 3 ? 32u : c == 4 ? 40u : c == 5 ? 48u : c == 6 ? 64u
                                                                                                        : fallback;
 }
+// Neighbour-list length of the learned path: the most neighbours a class may run.
+inline uint32_t proposalListK(uint32_t maxK, uint32_t fallbackK) { return maxK > fallbackK ? maxK : fallbackK; }
 // Class of a label (neighbours used at certification, 0 = never within 64).
 __host__ __device__ constexpr uint32_t proposalClass(uint32_t label) {
     if (label == 0) return kProposalClasses - 1;
@@ -289,8 +291,13 @@ struct TrainBuffers {
 hipError_t cellsKLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, const uint8_t* cls, uint8_t* labels,
                         uint32_t mode, uint32_t fallbackK, hipStream_t st);
 hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st);
+// cls[i] = given[indices[i]]: caller-given classes (input order) in sorted order.
+hipError_t gatherClassesLaunch(const uint32_t* indices, const uint8_t* given, uint32_t n, uint8_t* cls,
+                               hipStream_t st);
+// Classes of n sites; classes asking for more than maxK neighbours are
+// replaced by the last class (not certified within maxK).
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
-                          float cost, uint32_t fallbackK, hipStream_t st);
+                          float cost, uint32_t fallbackK, uint32_t maxK, hipStream_t st);
 hipError_t packWeightsLaunch(const float* params, const ProposalWeights& w, hipStream_t st);
 hipError_t mlpTrainStep(const TrainBuffers& t, uint32_t rows, float lr, uint32_t step, hipStream_t st);
 hipError_t gatherLaunch(const float* feat, const uint8_t* labels, const uint32_t* order, uint32_t rows, float* x,

@@ -200,7 +200,8 @@ __device__ __forceinline__ void mlpLayer(const uint16_t (*in)[kHidden], const ui
 __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat, uint32_t n, const uint16_t* w1,
                                                   const float* b1, const uint16_t* w2, const float* b2,
                                                   const uint16_t* w3, const float* b3, uint8_t* cls,
-                                                  float* logitsOut, float cost, uint32_t fallbackK) {
+                                                  float* logitsOut, float cost, uint32_t fallbackK,
+                                                  uint32_t maxK) {
     __shared__ uint16_t X[16][kHidden];
     __shared__ uint16_t H[16][kHidden];
     __shared__ float Lg[16][16];
@@ -229,10 +230,14 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
                     bv = Lg[lane][q];
                     best = q;
                 }
+            // a class asking for more than maxK neighbours predicts a cell
+            // that maxK neighbours do not certify: the last class
+            if (proposalK(best) > maxK) best = kProposalClasses - 1;
             if (cost > 0.f) {
                 // expected work: k_c updates, plus a validation pass when the
                 // cell is not certified at k_c (classes above c); the last
-                // class runs fallbackK neighbours and always validates
+                // class runs fallbackK neighbours and always validates; only
+                // classes of at most maxK neighbours are candidates
                 float p[kProposalClasses], z = 0.f;
                 for (uint32_t q = 0; q < kProposalClasses; q++) {
                     p[q] = __expf(Lg[lane][q] - bv);
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
                 }
                 float bestCost = float(proposalK(kProposalClasses - 1, fallbackK)) + cost, cum = 0.f;
                 best = kProposalClasses - 1;
-                for (uint32_t q = 0; q + 1 < kProposalClasses; q++) {
+                for (uint32_t q = 0; q + 1 < kProposalClasses && proposalK(q) <= maxK; q++) {
                     cum += p[q] / z;
                     const float e = float(proposalK(q)) + cost * (1.f - cum);
                     if (e < bestCost) {
@@ -382,6 +387,20 @@ hipError_t cellsKLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, const
     return hipGetLastError();
 }
 
+// Classes given by the caller in input order, moved to sorted order.
+__global__ __launch_bounds__(256) void k_gather_classes(const uint32_t* __restrict__ indices,
+                                                        const uint8_t* __restrict__ given, uint32_t n,
+                                                        uint8_t* cls) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) cls[i] = given[indices[i]];
+}
+
+hipError_t gatherClassesLaunch(const uint32_t* indices, const uint8_t* given, uint32_t n, uint8_t* cls,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(k_gather_classes, dim3((n + 255u) / 256u), dim3(256), 0, st, indices, given, n, cls);
+    return hipGetLastError();
+}
+
 hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st) {
     if (K < 16) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_features, dim3((c.n + 255u) / 256u), dim3(256), 0, st, c, knn, K, feat);
@@ -389,11 +408,11 @@ hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, flo
 }
 
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
-                          float cost, uint32_t fallbackK, hipStream_t st) {
+                          float cost, uint32_t fallbackK, uint32_t maxK, hipStream_t st) {
     const uint32_t tiles = (n + 15u) / 16u;
     const uint32_t grid = tiles < 8192u ? tiles : 8192u;
     hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64), 0, st, feat, n, w.w1, w.b1, w.w2, w.b2, w.w3, w.b3, cls,
-                       logits, cost, fallbackK);
+                       logits, cost, fallbackK, maxK);
     return hipGetLastError();
 }
 

@@ -205,12 +205,25 @@ def train_proposal(say):
                     "loss_last_epoch": float(loss[-1]), "label_s": t_labels, "train_s": t_train}
 
 
-def run_learned(params, dev, n, tuning, steps, warmup, plain_keys):
+def _precision(ctx):
+    """Interval validator (app/precision/precision.cpp:14-163) on the build's
+    vertices and closed edges."""
+    from apo_amd import _lib
+    r = _lib.Precision()
+    _lib.check(ctx, _lib.lib().apo_precision_check(ctx, _lib.byref(r)))
+    return {name: int(getattr(r, name)) for name, _ in r._fields_}
+
+
+def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None):
     """Time the workload with the proposal.  Cells the plain test does not
-    certify after the proposed count go through the plain validation."""
+    certify after the proposed count go through the plain validation.
+    classes: per-site classes given instead of the MLP's."""
     import torch
     from apo_amd import _lib
     ctx = _ctx(tuning, proposal=params)
+    if classes is not None:
+        cls = np.ascontiguousarray(classes, np.uint8)
+        _lib.check(ctx, _lib.lib().apo_proposal_set_classes(ctx, _lib.u8ptr(cls), len(cls)))
     L = _lib.lib()
     ptr = _lib.ctypes.c_void_p(dev.data_ptr())
     for _ in range(warmup):
@@ -228,6 +241,7 @@ def run_learned(params, dev, n, tuning, steps, warmup, plain_keys):
     _, ids = _vertices(ctx, L, _lib, v)
     cnt = _lib.c_uint64()
     _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
+    prec = _precision(ctx)
     L.apo_destroy(ctx)
     keys = _row_keys(ids)
     ms = 1000.0 * wall / steps
@@ -239,7 +253,7 @@ def run_learned(params, dev, n, tuning, steps, warmup, plain_keys):
                          "validation": s.ms_validation, "bisector": s.ms_bisector},
             "vertices": v, "vertices_only_in_plain": int(len(np.setdiff1d(plain_keys, keys))),
             "vertices_only_in_learned": int(len(np.setdiff1d(keys, plain_keys))),
-            "empty_sphere_violations": int(cnt.value)}
+            "empty_sphere_violations": int(cnt.value), "precision": prec}
 
 
 def c5_evaluation(params, plain_tuning, learned_tuning, say, n=5_000_000):
@@ -390,12 +404,14 @@ def main():
             traffic = float(json.load(f)["traffic_bytes_per_launch"])
         traffic_src = os.path.relpath(tp, ROOT)
 
-    violations = None
+    violations, precision = None, None
     if not args.no_check:
         say("FP64 empty-sphere check of every vertex through the LBVH")
         cnt = _lib.c_uint64()
         _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
         violations = int(cnt.value)
+        say("interval precision check of the vertices and closed edges")
+        precision = _precision(ctx)
 
     say("graph replay of the same build (the timed steps of value)")
     gr = graph_replay(dev, n, plain_tuning, args.steps, args.warmup, say)
@@ -424,6 +440,7 @@ def main():
         "buried_sites": int(s.buried),
         "flagged_cells": int(s.flagged_cells),
         "empty_sphere_violations": violations,
+        "precision": precision,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "launches": int(launches), "avg_launch_ms": ms_k / max(int(launches), 1),
@@ -451,9 +468,41 @@ def main():
         lc["training"] = training
         lc["speedup_vs_plain"] = lc["cells_per_s"] / out["eager"]["cells_per_s"]
         out["learned"] = lc
+        # the ceiling of any proposal: the classes of the exact labels of
+        # these very sites (the label pass, untimed), same knobs
+        from apo_amd import proposal as prop
+        _, lab = prop.labels(sites)
+        ex = run_learned(params, dev, n, ltune, args.steps, args.warmup, plain_keys, classes=prop.label_class(lab))
+        ex["speedup_vs_plain"] = ex["cells_per_s"] / out["eager"]["cells_per_s"]
+        ex["note"] = "classes of the exact certified-k labels of the same sites instead of the MLP's"
+        out["learned_exact_labels"] = ex
+        say(f"learned {lc['ms_per_step']:.1f} ms, exact-label classes {ex['ms_per_step']:.1f} ms")
         if not args.no_c5:
             say("C5: the proposal on a 5M-site set")
             out["learned_5m"] = c5_evaluation(params, plain_tuning, ltune, say)
+    if not args.no_learned:
+        # plain builds at other neighbour counts (one eager build each): the
+        # cost split that bounds what a per-cell neighbour count can gain
+        sweep = {}
+        for kk in (24, 32, 48, 64):
+            c = _ctx(dict(plain_tuning, k=kk))
+            _lib.check(c, L.apo_build_device(c, ptr, n))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _lib.check(c, L.apo_build_device(c, ptr, n))
+            torch.cuda.synchronize()
+            ms = 1000.0 * (time.perf_counter() - t0)
+            s2 = _lib.Stats()
+            _lib.check(c, L.apo_get_stats(c, _lib.byref(s2)))
+            L.apo_destroy(c)
+            sweep[str(kk)] = {"ms": ms, "knn": s2.ms_knn, "cells": s2.ms_cells, "validation": s2.ms_validation,
+                              "bisector": s2.ms_bisector, "certified": int(s2.cells_certified),
+                              "vertices": int(s2.vertices)}
+        sweep["16"] = {"ms": ms_per_step, "knn": stage["ms_knn"], "cells": stage["ms_cells"],
+                       "validation": stage["ms_validation"], "bisector": stage["ms_bisector"],
+                       "certified": int(s.cells_certified), "vertices": vertices}
+        out["plain_k_sweep"] = sweep
+        say("plain k sweep: " + ", ".join(f"k={k} {v['ms']:.0f} ms" for k, v in sorted(sweep.items(), key=lambda t: int(t[0]))))
     if not args.no_batch:
         out["batch"] = batch_workload(args.batch_molecules, min(args.steps, 3), 1, say)
     if not args.no_cpu_baseline:

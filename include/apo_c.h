@@ -56,6 +56,9 @@ typedef struct apo_config {
     uint32_t waves_bisector;
     uint32_t proposal_fallback_k; /* learned proposal: neighbours for sites of the last class (not certified
                                      within 64): 16, 24, 32, 48 or 64; default 16 */
+    uint32_t proposal_max_k;  /* learned proposal: the most neighbours any other class may run (16, 24, 32, 48
+                                 or 64; default 64).  Classes asking for more are not chosen; the kNN list
+                                 holds max(proposal_max_k, proposal_fallback_k) neighbours */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,
@@ -234,6 +237,11 @@ apo_status apo_proposal_set(apo_ctx* ctx, const float* params);
 apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t count, uint8_t* classes, float* logits);
 /* Classes of the last build's sites, in input order. */
 apo_status apo_proposal_classes(apo_ctx* ctx, uint8_t* classes);
+/* Per-site classes (input order, values 0..7) for the following builds of n
+ * sites, used instead of the MLP's: e.g. the classes of the labels of an
+ * earlier build of the same sites (apo_proposal_labels).  Certification and
+ * validation are unchanged.  NULL: back to the MLP (or the plain path). */
+apo_status apo_proposal_set_classes(apo_ctx* ctx, const uint8_t* classes, uint32_t n);
 
 /* ---- introspection of the device state (the public members dSites,
  * bvh, dStatus, dVertices, dEdges of AlgorithmGPU, algorithm.cuh:147-155) ---- */

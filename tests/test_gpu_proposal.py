@@ -8,7 +8,9 @@
   top two logits differ by more than that.
 * Training lowers the loss and beats the majority class.
 * A build with the proposal equals, record for record, the oracle run with
-  the same per-site classes, and has no empty-sphere violation.
+  the same per-site classes, and has no empty-sphere violation; so does a
+  build with classes given by the caller (the exact labels' classes) and a
+  capped neighbour list (proposal_max_k).
 """
 import numpy as np
 import pytest
@@ -111,3 +113,28 @@ def test_learned_build_matches_oracle(training):
     assert np.array_equal(gx.vertices.view(np.uint64), ox.view(np.uint64))
     assert g.empty_sphere_violations() == 0
     g.close()
+
+
+def test_given_classes_and_capped_lists_match_oracle(training):
+    sites, feat, lab = training
+    _say("[case] learned build with the labels' classes, lists capped at 32")
+    cls = proposal.label_class(lab)
+    cls[cls > 3] = 7  # classes of more than 32 neighbours become the last class
+    g = AlgorithmGPU(sites, classes=cls, proposal_max_k=32, proposal_fallback_k=32).build()
+    try:
+        assert np.array_equal(g.proposal_classes(), cls)
+        st = g.stats()
+        # every cell the labels certify within 32 neighbours is certified by the cell pass
+        assert st["cells_certified"] == int(np.sum((lab >= 1) & (lab <= 32)))
+        orc = oracle.Diagram(sites, classes=cls, edge_bound=0, fallback_k=32)
+        gc, gV, gE = g.all_cells()
+        oc, oV, oE = orc.all_cells()
+        for key in ("status", "vertex_nb", "edge_nb", "flags"):
+            assert np.array_equal(gc[key], oc[key]), key
+        assert np.array_equal(gV, oV) and np.array_equal(gE, oE)
+        gx = g.to_host()
+        ox, oi = orc.vertices()
+        assert np.array_equal(gx.vertices_id, oi)
+        assert np.array_equal(gx.vertices.view(np.uint64), ox.view(np.uint64))
+    finally:
+        g.close()

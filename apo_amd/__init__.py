@@ -183,6 +183,27 @@ class AlgorithmGPU:
                                                                     arity, _lib.byref(bad), _lib.byref(unc)))
         return int(bad.value), int(unc.value)
 
+    # -- cavity detection (app/cavity/utils.cu) -------------------------------
+    def hit_ratios(self, samples: int = 64) -> np.ndarray:
+        """Fraction of `samples` rays from each output vertex that hit a site."""
+        cnt = _lib.c_uint64()
+        _lib.check(self._ctx, _lib.lib().apo_vertex_count(self._ctx, _lib.byref(cnt)))
+        out = np.zeros(int(cnt.value), np.float32)
+        _lib.check(self._ctx, _lib.lib().apo_cavity_hit_ratios(self._ctx, samples, _lib.fptr(out)))
+        return out
+
+    def cavity_vertices(self, hit_ratio_threshold: float = 0.5, radius_threshold: float = 5.0) -> np.ndarray:
+        """Vertices (FP32 x, y, z, w) passing both thresholds, after hit_ratios()."""
+        L = _lib.lib()
+        cnt = _lib.c_uint64()
+        _lib.check(self._ctx, L.apo_cavity_vertices(self._ctx, hit_ratio_threshold, radius_threshold, None, 0,
+                                                    _lib.byref(cnt)))
+        out = np.zeros((int(cnt.value), 4), np.float32)
+        if len(out):
+            _lib.check(self._ctx, L.apo_cavity_vertices(self._ctx, hit_ratio_threshold, radius_threshold,
+                                                        _lib.fptr(out), len(out), _lib.byref(cnt)))
+        return out
+
     # -- device state (public members of AlgorithmGPU) ------------------------
     def sorted_sites(self):
         n = len(self.sites)

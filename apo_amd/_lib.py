@@ -78,6 +78,8 @@ SIGNATURES = {
     "apo_get_device_vertices": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "apo_closed_edge_count": (ctypes.c_int, [_vp, _u64p]),
     "apo_precision_check": (ctypes.c_int, [_vp, ctypes.POINTER(Precision)]),
+    "apo_cavity_hit_ratios": (ctypes.c_int, [_vp, _u32, _f32p]),
+    "apo_cavity_vertices": (ctypes.c_int, [_vp, ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_uint64, _u64p]),
     "apo_precision_check_points": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64, _u32, _u64p, _u64p]),
     "apo_get_closed_edges": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64]),
     "apo_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),

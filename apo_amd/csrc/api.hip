@@ -119,6 +119,10 @@ struct apo_ctx {
     uint8_t* givenCls = nullptr;  // classes given by the caller (input order), used instead of the MLP
     uint32_t givenN = 0;          // their site count (0: none given)
     uint32_t givenCap = 0;
+    // cavity detection: hit ratios of the current build's vertices
+    float* cavRatios = nullptr;
+    uint64_t cavCap = 0;
+    bool cavValid = false;
     hipEvent_t ev[8] = {};
     ValidationTiming vt{};
     apo_stats stats{};
@@ -533,6 +537,32 @@ apo_status runLabelPass(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     return APO_OK;
 }
 
+// Ray directions of the cavity app: Halton points on the sphere (Wong et
+// al., "Sampling with Hammersley and Halton points"; app/cavity/utils.cu:
+// 201-242 with p2 = 3 and its truncated pi), each normalised.
+std::vector<F3> cavityDirections(uint32_t samples) {
+    std::vector<F3> out(samples);
+    for (uint32_t k = 0; k < samples; k++) {
+        float t = 0.f;
+        float h = 0.5f;
+        for (uint32_t b = k; b; h *= .5f, b >>= 1)
+            if (b & 1u) t += h;
+        t = 2.f * t - 1.f;
+        const float st = std::sqrt(1.f - t * t);
+        float phi = 0.f;
+        const float ip = 1.f / 3.f;
+        float p = ip;
+        for (uint32_t b = k; b; p *= ip, b /= 3u) {
+            const uint32_t a = b % 3u;
+            if (a) phi += float(a) * p;
+        }
+        const float phirad = float(double(phi * 4.f) * 3.1415926535);
+        F3 d{st * std::cos(phirad), st * std::sin(phirad), t};
+        const float inv = 1.f / std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+        out[k] = {d.x * inv, d.y * inv, d.z * inv};
+    }
+    return out;
+}
 }  // namespace
 
 extern "C" {
@@ -592,7 +622,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
-                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -634,6 +664,7 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     ctx->built = false;
     if (ctx->givenN && ctx->givenN != n)
         return fail(ctx, APO_ERR_INVALID, "proposal classes were given for another site count");
+    ctx->cavValid = false;
     apo_status s = ensureCapacity(ctx, n, learnedPath(ctx) ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
     if (s != APO_OK) return s;
     if (learnedPath(ctx)) {
@@ -915,6 +946,66 @@ apo_status apo_precision_check(apo_ctx* ctx, apo_precision* out) {
         if (s != APO_OK) return s;
     }
     *out = r;
+    return APO_OK;
+}
+
+
+apo_status apo_cavity_hit_ratios(apo_ctx* ctx, uint32_t samples, float* hit_ratios) {
+    if (!ctx || samples == 0 || samples > 4096u) return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const uint64_t V = ctx->vertexCount;
+    if (V > ctx->cavCap) {
+        if (ctx->cavRatios) APO_HIP(hipFree(ctx->cavRatios));
+        ctx->cavRatios = nullptr;
+        ctx->cavCap = 0;
+        APO_HIP(alloc(ctx->cavRatios, V));
+        ctx->cavCap = V;
+    }
+    const std::vector<F3> dirs = cavityDirections(samples);
+    DeviceTemp<F3> dd;
+    APO_HIP(dd.alloc(samples));
+    APO_HIP(hipMemcpyAsync(dd.p, dirs.data(), samples * sizeof(F3), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
+    APO_HIP(hitRatiosLaunch(cellCtx(ctx), ctx->idxA, ctx->cellCount, ctx->outXyzw, ctx->outIds, V, dd.p, samples,
+                            ctx->cavRatios, st));
+    uint32_t guard = 0;
+    APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(guard), hipMemcpyDeviceToHost, st));
+    if (hit_ratios && V) APO_HIP(hipMemcpyAsync(hit_ratios, ctx->cavRatios, V * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped in the hit ratios: " + guardText(guard));
+    ctx->cavValid = true;
+    return APO_OK;
+}
+
+apo_status apo_cavity_vertices(apo_ctx* ctx, float hit_ratio_threshold, float radius_threshold, float* xyzw,
+                               uint64_t capacity, uint64_t* count) {
+    if (!ctx || !count) return APO_ERR_INVALID;
+    if (!ctx->built || !ctx->cavValid) return fail(ctx, APO_ERR_STATE, "no hit ratios for the current build");
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const uint64_t V = ctx->vertexCount;
+    const uint32_t blocks = uint32_t((V + 255u) / 256u);
+    DeviceTemp<uint32_t> sums, total;
+    APO_HIP(sums.alloc(blocks));
+    APO_HIP(total.alloc(1));
+    uint32_t kept = 0;
+    if (V) {
+        APO_HIP(cavityCountLaunch(ctx->outXyzw, ctx->cavRatios, V, hit_ratio_threshold, radius_threshold, sums.p,
+                                  total.p, st));
+        APO_HIP(hipMemcpyAsync(&kept, total.p, 4, hipMemcpyDeviceToHost, st));
+        APO_HIP(hipStreamSynchronize(st));
+    }
+    *count = kept;
+    if (!xyzw || kept == 0) return APO_OK;
+    if (capacity < kept) return fail(ctx, APO_ERR_CAPACITY, "output buffer too small");
+    DeviceTemp<float4> out;
+    APO_HIP(out.alloc(kept));
+    APO_HIP(cavityWriteLaunch(ctx->outXyzw, ctx->cavRatios, V, hit_ratio_threshold, radius_threshold, sums.p, out.p,
+                              kept, st));
+    APO_HIP(hipMemcpyAsync(xyzw, out.p, size_t(kept) * sizeof(float4), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
     return APO_OK;
 }
 

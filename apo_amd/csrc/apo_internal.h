@@ -313,6 +313,23 @@ hipError_t cellRecordsLaunch(const CellCtx& c, const uint32_t* indices, const ui
 hipError_t checkEmptyLaunch(const CellCtx& c, const uint32_t* indices, uint32_t* rank, const D4* xyzw,
                             const uint4* ids, uint64_t count, double tol, unsigned long long* violations,
                             hipStream_t st);
+// rank[indices[j]] = j: original site id -> sorted position.
+hipError_t rankLaunch(const uint32_t* indices, uint32_t n, uint32_t* rank, hipStream_t st);
+// Exclusive scan of `blocks` block sums in place; *total = their sum.
+hipError_t scanBlockSumsLaunch(uint32_t* blockSums, uint32_t blocks, uint32_t* total, hipStream_t st);
+
+// Cavity detection (cavity.hip, app/cavity/utils.cu): hit ratio of each
+// vertex over `samples` rays (directions dirs), then the vertices whose
+// ratio and radius pass the thresholds, compacted in order.
+hipError_t hitRatiosLaunch(const CellCtx& c, const uint32_t* indices, uint32_t* rank, const D4* xyzw,
+                           const uint4* ids, uint64_t count, const F3* dirs, uint32_t samples, float* ratios,
+                           hipStream_t st);
+hipError_t cavityCountLaunch(const D4* xyzw, const float* ratios, uint64_t count, float hitThreshold,
+                             float radiusThreshold, uint32_t* blockSums, uint32_t* total, hipStream_t st);
+hipError_t cavityWriteLaunch(const D4* xyzw, const float* ratios, uint64_t count, float hitThreshold,
+                             float radiusThreshold, const uint32_t* blockOffsets, float4* out, uint64_t capacity,
+                             hipStream_t st);
+
 // Interval validator (app/precision/precision.cpp:14-163): counts2[0] +=
 // points some other site is certainly closer to, counts2[1] += the others
 // some site may be closer to.  arity 4: vertices, 3: closed-edge points.

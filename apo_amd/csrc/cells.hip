@@ -174,6 +174,11 @@ __global__ __launch_bounds__(256) void k_out_write(CellCtx c, const uint32_t* ce
     }
 }
 
+hipError_t scanBlockSumsLaunch(uint32_t* blockSums, uint32_t blocks, uint32_t* total, hipStream_t st) {
+    hipLaunchKernelGGL(k_out_scan, dim3(1), dim3(1024), 0, st, blockSums, blocks, total);
+    return hipGetLastError();
+}
+
 hipError_t outputCountLaunch(const CellCtx& c, const OutputBuffers& o, hipStream_t st) {
     const uint32_t blocks = (c.n + 255u) / 256u;
     hipLaunchKernelGGL(k_out_count, dim3(blocks), dim3(256), 0, st, c, o.cellCount, o.blockSums);

@@ -247,6 +247,11 @@ __global__ void k_rank(const uint32_t* __restrict__ indices, uint32_t n, uint32_
     if (j < n) rank[indices[j]] = j;
 }
 
+hipError_t rankLaunch(const uint32_t* indices, uint32_t n, uint32_t* rank, hipStream_t st) {
+    hipLaunchKernelGGL(k_rank, dim3((n + 255) / 256), dim3(256), 0, st, indices, n, rank);
+    return hipGetLastError();
+}
+
 hipError_t checkEmptyLaunch(const CellCtx& c, const uint32_t* indices, uint32_t* rank, const D4* xyzw,
                             const uint4* ids, uint64_t count, double tol, unsigned long long* violations,
                             hipStream_t st) {

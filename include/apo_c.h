@@ -187,6 +187,21 @@ apo_status apo_precision_check(apo_ctx* ctx, apo_precision* out);
 apo_status apo_precision_check_points(apo_ctx* ctx, const double* xyzw, const uint32_t* ids, uint64_t count,
                                       uint32_t arity, uint64_t* bad, uint64_t* uncertain);
 
+/* Cavity detection of the reference's cavity app (app/cavity/utils.cu;
+ * the ray tracing there uses OptiX, here the LBVH).  Hit ratios
+ * (getHitRatios, :17-65): for every vertex of the last build, the fraction of
+ * `samples` rays (at most 4096) cast from its centre (as FP32) along Halton
+ * directions on the sphere (sphereHalton, :201-242, normalised) that hit a
+ * site sphere at a distance in [0.01, 1e16]; written to hit_ratios (one
+ * float per output vertex, may be NULL) and kept for apo_cavity_vertices. */
+apo_status apo_cavity_hit_ratios(apo_ctx* ctx, uint32_t samples, float* hit_ratios);
+/* filterVertices (:244-264, :527-558): the vertices whose hit ratio is above
+ * hit_ratio_threshold and whose radius (as FP32) is above radius_threshold,
+ * in output order, as FP32 (x, y, z, w).  Two-call pattern: xyzw NULL
+ * returns the count. */
+apo_status apo_cavity_vertices(apo_ctx* ctx, float hit_ratio_threshold, float radius_threshold, float* xyzw,
+                               uint64_t capacity, uint64_t* count);
+
 /* Topology sets of AlgorithmGPU::toTopology (algorithm.inl:1732-1821): for
  * every cell in sorted order, its distinct bisectors {i, j}, trisectors
  * {i, j, k} and quadrisectors {i, j, k, l} with i below the others and no

@@ -105,6 +105,11 @@ def lib():
                                                   ctypes.c_double, ctypes.c_uint32]
         L.orc_bruteforce_vertices.restype = ctypes.c_uint64
         L.orc_bruteforce_vertices.argtypes = [f32p, ctypes.c_uint32, u32p, ctypes.c_uint64, ctypes.c_double]
+        L.orc_cavity_directions.restype = None
+        L.orc_cavity_directions.argtypes = [ctypes.c_uint32, f32p]
+        L.orc_cavity_hit_ratios.restype = None
+        L.orc_cavity_hit_ratios.argtypes = [f32p, ctypes.c_uint32, f64p, ctypes.c_uint64, ctypes.c_uint32,
+                                            ctypes.c_uint32, f32p]
         L.orc_precision_counts.restype = None
         L.orc_precision_counts.argtypes = [f32p, ctypes.c_uint32, f64p, u32p, ctypes.c_uint64, ctypes.c_uint32,
                                            ctypes.c_uint32, u64p]
@@ -360,3 +365,19 @@ def interval_distance(site, point):
     out = np.zeros(2)
     lib().orc_interval_distance(_p(s, f64p), _p(p, f64p), _p(out, f64p))
     return float(out[0]), float(out[1])
+
+
+def cavity_directions(samples: int = 64):
+    out = np.zeros((samples, 3), np.float32)
+    lib().orc_cavity_directions(samples, _p(out, f32p))
+    return out
+
+
+def cavity_hit_ratios(xyzr, xyzw, samples: int = 64, threads: int = 0):
+    """Hit ratio of every vertex (app/cavity/utils.cu:17-65), every ray
+    against every site."""
+    s = _sites(xyzr)
+    x = np.ascontiguousarray(xyzw, np.float64).reshape(-1, 4)
+    out = np.zeros(len(x), np.float32)
+    lib().orc_cavity_hit_ratios(_p(s, f32p), len(s), _p(x, f64p), len(x), samples, threads, _p(out, f32p))
+    return out

@@ -138,6 +138,13 @@ void orc_round_dir(uint32_t op, double a, double b, double* lohi);
 /* Interval weighted distance |p - c| - r of one site (precision.cpp:14-24). */
 void orc_interval_distance(const double* site4, const double* p3, double* lohi);
 
+/* Cavity hit ratios (app/cavity/utils.cu:17-65, 201-242; apo_cavity.cpp):
+ * the normalised Halton directions, and per vertex the fraction of the
+ * rays from its FP32 centre that hit any site in [0.01, 1e16]. */
+void orc_cavity_directions(uint32_t samples, float* xyz);
+void orc_cavity_hit_ratios(const float* xyzr, uint32_t n, const double* xyzw, uint64_t count, uint32_t samples,
+                           uint32_t threads, float* ratios);
+
 /* All empty tangent spheres of 4 sites, by enumeration of quadruples
  * (tiny n only).  Returns the count; writes up to cap id quadruples sorted. */
 uint64_t orc_bruteforce_vertices(const float* xyzr, uint32_t n, uint32_t* ids, uint64_t cap, double tol);

@@ -28,8 +28,6 @@ namespace apo_hip {
 struct VertexLds {
     CellLds cell;
     uint32_t stNode[kStackDepth][64];
-    float stDist[kStackDepth][64];
-    uint32_t share[64];   // donor lane of each receiving rank (work sharing)
     float resD[64];       // per owner lane: smallest gap found by shared items
     uint32_t resN[64];    // ... and its site (earliest in traversal order on ties)
 };
@@ -40,17 +38,13 @@ struct EdgeLds {
         CoverLds cover;  // edge covers are built while no traversal runs
     };
     CoverOut coverOut;
-    uint32_t share[64];   // donor lane of each receiving rank (work sharing)
     uint32_t res[64];     // per owner lane: earliest contributor found by a helper
-    uint32_t settled[64]; // per owner lane: its own first item found the contributor
 };
 struct BisectorLds {
     CellLds cell;
     uint32_t stNode[kStackDepth][64];
     uint32_t bis[kBisectorCap];
-    uint32_t share[64];   // donor lane of each receiving rank (work sharing)
     uint32_t res[64];     // per owner lane: earliest contributor found by a helper
-    uint32_t settled[64]; // per owner lane: its own first item found the contributor
 };
 static_assert(sizeof(CoverLds) <= sizeof(uint32_t) * kStackDepth * 64, "cover buffers overlay the stack");
 
@@ -65,6 +59,18 @@ struct Trav {
     bool live;
     uint32_t sb = 0, base = 0;
 };
+
+// Lane of the k-th (0-based) set bit of a wave mask: the donor a receiving
+// lane of rank k takes work from (binary search on prefix popcounts).
+__device__ __forceinline__ uint32_t laneOfRank(uint64_t mask, uint32_t k) {
+    uint32_t lo = 0, hi = 63;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (popc64(mask & ((uint64_t(2) << mid) - 1u)) > k) hi = mid;
+        else lo = mid + 1u;
+    }
+    return lo;
+}
 
 // Pop the next node of a lane's depth-first traversal, or end it.
 __device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
@@ -118,9 +124,12 @@ __device__ __forceinline__ bool knownLane(const CellLds& L, uint32_t F, uint32_t
 // One step of a lane's nearest-site search (algorithm.inl:305-414): visit
 // one internal node (descend to the closer child under the best gap so far,
 // stack the other with its gap) or test one leaf; stacked nodes whose gap
-// exceeds the best are skipped when popped.
+// exceeds the best are skipped when popped.  The stack holds the parent of
+// the stacked child and which child it is (bit 31); the gap is recomputed
+// from the parent's copy of the child's box when the entry is popped, the
+// same FP32 value the reference stacks, so no gap needs LDS.
 __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, uint32_t (*stNode)[64],
-                                           float (*stDist)[64], uint32_t F, uint32_t i, uint4 vx, F4 cur, Trav& t,
+                                           uint32_t F, uint32_t i, uint4 vx, F4 cur, Trav& t,
                                            float& nd, uint32_t& nb, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
@@ -149,8 +158,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode[t.sp][lane] = farChild;
-                    stDist[t.sp][lane] = minFar;
+                    stNode[t.sp][lane] = node | (farChild == R0 ? 0x80000000u : 0u);
                     t.sp++;
                 } else {
                     flags |= kFlagStackOverflow;
@@ -171,8 +179,13 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
     bool more = false;
     while (t.sp > t.sb) {
         t.sp--;
-        if (stDist[t.sp][lane] > nd) continue;
-        t.node = stNode[t.sp][lane];
+        const uint32_t e = stNode[t.sp][lane];
+        const Node pn = c.nodes[e & 0x7fffffffu];
+        Box lb, rb;
+        nodeBoxes(pn, lb, rb);
+        const bool right = (e >> 31) != 0u;
+        if (boxDistance(selBox(right, rb, lb), cur) > nd) continue;
+        t.node = uint32_t(right ? pn.d.y : pn.d.x);
         more = true;
         break;
     }
@@ -365,9 +378,7 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                 const uint32_t dr = countBelow(donors), ir = countBelow(idle);
                 const bool give = ((donors >> lane) & 1u) && dr < m;
                 const bool take = ((idle >> lane) & 1u) && ir < m;
-                if (give) S.share[dr] = lane;
-                waveSync();
-                const uint32_t src = take ? S.share[ir] : lane;
+                const uint32_t src = take ? laneOfRank(donors, ir) : lane;
                 const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
                 const uint32_t dOwner = __shfl(owner, int(src), 64);
                 const float dNd = __shfl(nd, int(src), 64);
@@ -436,7 +447,7 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
         Trav t2{root, 0u, exact};
         float nd2 = FLT_MAX;
         while (ballot(t2.live) != 0) {
-            if (t2.live) vertexStep(c, L, S.stNode, S.stDist, w.F, i, vx0, cur0, t2, nd2, nb2, flags);
+            if (t2.live) vertexStep(c, L, S.stNode, w.F, i, vx0, cur0, t2, nd2, nb2, flags);
         }
     }
     return exact ? nb2 : site;
@@ -644,7 +655,7 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
                                               uint32_t l, const EdgeQuery& q, uint32_t& flags) {
     const uint32_t lane = laneId();
     S.res[lane] = kNone;
-    S.settled[lane] = 0u;
+    uint64_t settled = 0;  // owner lanes whose own item found the contributor
     // the item this lane works on: the query it serves and its owner
     EdgeQuery wq = q;
     uint32_t wk = k, wl = l, owner = need ? lane : kNone;
@@ -660,7 +671,7 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
     waveSync();
     while (true) {
         // a shared item of a settled search has nothing left to find
-        if (tr.live && !own && owner != kNone && S.settled[owner]) tr.live = false;
+        if (tr.live && !own && owner != kNone && ((settled >> owner) & 1u)) tr.live = false;
         const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
         if (stepping == 0 && waiting == 0) break;
         if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
@@ -679,16 +690,13 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
                     ok = valid != 0u;
                 }
                 if (ok) {
-                    if (own) {
-                        found = cand;
-                        S.settled[lane] = 1u;
-                    } else {
-                        shared = cand;
-                    }
+                    if (own) found = cand;
+                    else shared = cand;
                     tr.live = false;
                 }
                 cand = kNone;
             }
+            settled |= ballot(own && found != kNone);
             // keep the earliest contributor of the shared items, per owner
             uint64_t hm = ballot(shared != kNone);
             while (hm) {
@@ -711,9 +719,7 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
             const uint32_t dr = countBelow(donors), ir = countBelow(idle);
             const bool give = ((donors >> lane) & 1u) && dr < m;
             const bool take = ((idle >> lane) & 1u) && ir < m;
-            if (give) S.share[dr] = lane;
-            waveSync();
-            const uint32_t src = take ? S.share[ir] : lane;
+            const uint32_t src = take ? laneOfRank(donors, ir) : lane;
             const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
             const uint32_t dOwner = __shfl(owner, int(src), 64);
             const uint32_t node = take ? S.stNode[dSb][src] : 0u;
@@ -934,14 +940,14 @@ __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorL
                                                      float cylR, uint32_t& flags) {
     const uint32_t lane = laneId();
     S.res[lane] = kNone;
-    S.settled[lane] = 0u;
+    uint64_t settled = 0;  // owner lanes whose own item found the contributor
     uint32_t owner = need ? lane : kNone;
     bool own = need;
     Trav tr{root, 0u, need};
     uint32_t found = kNone, cand = kNone;
     waveSync();
     while (true) {
-        if (tr.live && !own && owner != kNone && S.settled[owner]) tr.live = false;
+        if (tr.live && !own && owner != kNone && ((settled >> owner) & 1u)) tr.live = false;
         const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
         if (stepping == 0 && waiting == 0) break;
         if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
@@ -954,16 +960,13 @@ __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorL
                     ok = cnt == 2u && clearOfNeighbours(c, L, w.F, x1, bk, bk);
                 }
                 if (ok) {
-                    if (own) {
-                        found = cand;
-                        S.settled[lane] = 1u;
-                    } else {
-                        shared = cand;
-                    }
+                    if (own) found = cand;
+                    else shared = cand;
                     tr.live = false;
                 }
                 cand = kNone;
             }
+            settled |= ballot(own && found != kNone);
             uint64_t hm = ballot(shared != kNone);
             while (hm) {
                 const uint32_t t = uint32_t(__builtin_ctzll(hm));
@@ -984,9 +987,7 @@ __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorL
             const uint32_t dr = countBelow(donors), ir = countBelow(idle);
             const bool give = ((donors >> lane) & 1u) && dr < m;
             const bool take = ((idle >> lane) & 1u) && ir < m;
-            if (give) S.share[dr] = lane;
-            waveSync();
-            const uint32_t src = take ? S.share[ir] : lane;
+            const uint32_t src = take ? laneOfRank(donors, ir) : lane;
             const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
             const uint32_t dOwner = __shfl(owner, int(src), 64);
             const uint32_t node = take ? S.stNode[dSb][src] : 0u;

@@ -289,6 +289,55 @@ def c5_evaluation(params, plain_tuning, learned_tuning, say, n=5_000_000):
     return lr
 
 
+def cavity_section(ctx, vertices, say):
+    """Hit ratios of every vertex of the 1M build (64 rays each through the
+    LBVH, app/cavity/utils.cu:17-65) and the reference app's default filter
+    (hit ratio above 0.5, radius above 5: nothing at this scale) and one at
+    this workload's scale."""
+    import torch
+    from apo_amd import _lib
+    L = _lib.lib()
+    _lib.check(ctx, L.apo_cavity_hit_ratios(ctx, 64, None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(ctx, L.apo_cavity_hit_ratios(ctx, 64, None))
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0)
+    kept = {}
+    for hit, rad in ((0.5, 5.0), (0.9, 0.004)):
+        c = _lib.c_uint64()
+        _lib.check(ctx, L.apo_cavity_vertices(ctx, hit, rad, None, 0, _lib.byref(c)))
+        kept[f"hit>{hit},r>{rad}"] = int(c.value)
+    say(f"cavity hit ratios: {ms:.1f} ms for {vertices} vertices x 64 rays")
+    return {"ms": ms, "vertices": vertices, "rays_per_s": vertices * 64 / (ms / 1000.0), "kept": kept,
+            "note": "one call of apo_cavity_hit_ratios (64 Halton directions, any-hit through the LBVH), "
+                    "wall time between device synchronisations"}
+
+
+def protein_section(say):
+    """The reference's five smallest sample proteins (MMTF, joggled by 1e-3 as
+    its apps do): one eager build each after a warm-up."""
+    import torch
+    from apo_amd import _lib, molecules
+    L = _lib.lib()
+    out = {}
+    for name in ("5ZCK", "1AGA", "3DIK", "101M", "1A3F"):
+        s = molecules.joggle(molecules.load_mmtf(os.path.join(ROOT, "tests", "golden", "mmtf", f"{name}.mmtf")), 1e-3)
+        ctx = _ctx({})
+        _lib.check(ctx, L.apo_build(ctx, _lib.fptr(s), len(s)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(ctx, L.apo_build(ctx, _lib.fptr(s), len(s)))
+        torch.cuda.synchronize()
+        ms = 1000.0 * (time.perf_counter() - t0)
+        st = _lib.Stats()
+        _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(st)))
+        L.apo_destroy(ctx)
+        out[name] = {"atoms": int(len(s)), "ms": ms, "device_ms": st.ms_total, "vertices": int(st.vertices)}
+    say("proteins: " + ", ".join(f"{k} {v['atoms']} atoms {v['ms']:.2f} ms" for k, v in out.items()))
+    return out
+
+
 def cpu_baseline(sites: np.ndarray, target: int):
     """Time the oracle (host C++/OpenMP restatement of the reference's method,
     reference edge searches) on a sub-cube of the workload holding about
@@ -413,6 +462,8 @@ def main():
         say("interval precision check of the vertices and closed edges")
         precision = _precision(ctx)
 
+    if not args.no_check:
+        out_cavity = cavity_section(ctx, vertices, say)
     say("graph replay of the same build (the timed steps of value)")
     gr = graph_replay(dev, n, plain_tuning, args.steps, args.warmup, say)
     out = {
@@ -441,6 +492,7 @@ def main():
         "flagged_cells": int(s.flagged_cells),
         "empty_sphere_violations": violations,
         "precision": precision,
+        "cavity": out_cavity if not args.no_check else None,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "launches": int(launches), "avg_launch_ms": ms_k / max(int(launches), 1),
@@ -505,6 +557,7 @@ def main():
         say("plain k sweep: " + ", ".join(f"k={k} {v['ms']:.0f} ms" for k, v in sorted(sweep.items(), key=lambda t: int(t[0]))))
     if not args.no_batch:
         out["batch"] = batch_workload(args.batch_molecules, min(args.steps, 3), 1, say)
+        out["proteins"] = protein_section(say)
     if not args.no_cpu_baseline:
         say(f"CPU baseline: oracle on a sub-cube of about {args.cpu_sample} sites")
         out["cpu_baseline"] = cpu_baseline(sites, args.cpu_sample)

@@ -96,6 +96,8 @@ struct apo_ctx {
     uint32_t* enb = nullptr;
     uint32_t* knn = nullptr;
     uint32_t* counters = nullptr;   // [0] cell queue, [1..15] validation
+    uint32_t* spill = nullptr;      // deep stack slots of the vertex-validation waves
+    uint32_t spillBlocks = 0;
     uint32_t* err = nullptr;        // device guard bits
     uint32_t* tasks = nullptr;
     // output
@@ -374,7 +376,7 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     APO_HIP(treeHeightLaunch(ctx->range, n, n - ctx->molecules, reinterpret_cast<int32_t*>(ctx->tasks),
                              reinterpret_cast<int32_t*>(ctx->cellCount), reinterpret_cast<int2*>(ctx->blockSums),
                              ctx->counters + kTreeHeight, st));
-    const ValidationBuffers vb{ctx->tasks, ctx->counters};
+    const ValidationBuffers vb{ctx->tasks, ctx->counters, ctx->spill, ctx->spillBlocks};
     const ValidationGrid vg{persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
                             persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge),
                             persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector)};
@@ -622,7 +624,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
-                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -665,6 +667,16 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     if (ctx->givenN && ctx->givenN != n)
         return fail(ctx, APO_ERR_INVALID, "proposal classes were given for another site count");
     ctx->cavValid = false;
+    // deep stack slots of every wave of the vertex-validation grid
+    const uint32_t vblocks = persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex);
+    if (vblocks > ctx->spillBlocks) {
+        dropGraph(ctx);
+        if (ctx->spill) APO_HIP(hipFree(ctx->spill));
+        ctx->spill = nullptr;
+        ctx->spillBlocks = 0;
+        APO_HIP(alloc(ctx->spill, size_t(vblocks) * vertexSpillWords()));
+        ctx->spillBlocks = vblocks;
+    }
     apo_status s = ensureCapacity(ctx, n, learnedPath(ctx) ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
     if (s != APO_OK) return s;
     if (learnedPath(ctx)) {

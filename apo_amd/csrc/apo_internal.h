@@ -221,7 +221,11 @@ __device__ __forceinline__ uint32_t popWork(uint32_t* heads, uint32_t count) {
 struct ValidationBuffers {
     uint32_t* tasks;       // n
     uint32_t* counters;    // kCounterWords
+    uint32_t* spill;       // spillBlocks * vertexSpillWords(): deep stack slots of the vertex searches
+    uint32_t spillBlocks;  // the most blocks a vertex validation grid may have
 };
+// Words of device memory per vertex-validation wave for its deep stack slots.
+uint32_t vertexSpillWords();
 // Persistent grid sizes (blocks of one wave).
 struct ValidationGrid {
     uint32_t vertex, edge, bisector;

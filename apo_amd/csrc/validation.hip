@@ -24,10 +24,28 @@
 
 namespace apo_hip {
 
+// Traversal stack of the vertex searches: the first kVertexLdsDepth slots
+// of every lane in LDS, the rest of the reference's kStackDepth in a
+// per-wave region of device memory (reached only by deep traversals), so
+// the kernel's LDS admits more waves per CU.
+constexpr uint32_t kVertexLdsDepth = 16;
+constexpr uint32_t kVertexSpillWords = (kStackDepth - kVertexLdsDepth) * 64;
+struct VStack {
+    uint32_t (*lds)[64];
+    uint32_t* spill;  // kVertexSpillWords words of this wave
+    __device__ __forceinline__ uint32_t get(uint32_t slot, uint32_t lane) const {
+        return slot < kVertexLdsDepth ? lds[slot][lane] : spill[(slot - kVertexLdsDepth) * 64u + lane];
+    }
+    __device__ __forceinline__ void set(uint32_t slot, uint32_t lane, uint32_t v) const {
+        if (slot < kVertexLdsDepth) lds[slot][lane] = v;
+        else spill[(slot - kVertexLdsDepth) * 64u + lane] = v;
+    }
+};
+
 // LDS of each kernel (one wave per block)
 struct VertexLds {
     CellLds cell;
-    uint32_t stNode[kStackDepth][64];
+    uint32_t stNode[kVertexLdsDepth][64];
     float resD[64];       // per owner lane: smallest gap found by shared items
     uint32_t resN[64];    // ... and its site (earliest in traversal order on ties)
 };
@@ -81,6 +99,14 @@ __device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
         t.node = stNode[t.sp][laneId()];
     }
 }
+__device__ __forceinline__ void popNode(const VStack& st, Trav& t) {
+    if (t.sp == t.sb) {
+        t.live = false;
+    } else {
+        t.sp--;
+        t.node = st.get(t.sp, laneId());
+    }
+}
 
 __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
     const uint32_t lane = laneId();
@@ -128,7 +154,7 @@ __device__ __forceinline__ bool knownLane(const CellLds& L, uint32_t F, uint32_t
 // the stacked child and which child it is (bit 31); the gap is recomputed
 // from the parent's copy of the child's box when the entry is popped, the
 // same FP32 value the reference stacks, so no gap needs LDS.
-__device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, uint32_t (*stNode)[64],
+__device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, const VStack& stNode,
                                            uint32_t F, uint32_t i, uint4 vx, F4 cur, Trav& t,
                                            float& nd, uint32_t& nb, uint32_t& flags) {
     const uint32_t lane = laneId();
@@ -158,7 +184,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode[t.sp][lane] = node | (farChild == R0 ? 0x80000000u : 0u);
+                    stNode.set(t.sp, lane, node | (farChild == R0 ? 0x80000000u : 0u));
                     t.sp++;
                 } else {
                     flags |= kFlagStackOverflow;
@@ -179,7 +205,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, u
     bool more = false;
     while (t.sp > t.sb) {
         t.sp--;
-        const uint32_t e = stNode[t.sp][lane];
+        const uint32_t e = stNode.get(t.sp, lane);
         const Node pn = c.nodes[e & 0x7fffffffu];
         Box lb, rb;
         nodeBoxes(pn, lb, rb);
@@ -254,7 +280,7 @@ constexpr uint32_t kShareIdle = 4;
 // below when it lies inside (a centre inside a site gives a negative gap),
 // so a box within `slack` of the centre is never pruned.  Leaves improve
 // the bound strictly, so the first of equal gaps in this order is kept.
-__device__ __forceinline__ void vertexNearStep(const CellCtx& c, const CellLds& L, uint32_t (*stNode)[64],
+__device__ __forceinline__ void vertexNearStep(const CellCtx& c, const CellLds& L, const VStack& stNode,
                                                uint32_t F, uint32_t i, uint4 vx, F4 cur, float slack, Trav& t,
                                                float& nd, uint32_t& nb, uint32_t& flags) {
     const uint32_t lane = laneId();
@@ -283,7 +309,7 @@ __device__ __forceinline__ void vertexNearStep(const CellCtx& c, const CellLds& 
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode[t.sp][lane] = farChild;
+                    stNode.set(t.sp, lane, farChild);
                     t.sp++;
                 } else {
                     flags |= kFlagStackOverflow;
@@ -350,8 +376,9 @@ __device__ __forceinline__ bool searchKeeps(const CellCtx& c, uint32_t j, float 
 // else kNone.
 __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& S, const CellLds& L,
                                                    const WaveCell& w, uint32_t i, uint32_t root, bool need,
-                                                   const uint4 vx0, const F4 cur0, uint32_t& flags) {
+                                                   const uint4 vx0, const F4 cur0, uint32_t* spill, uint32_t& flags) {
     const uint32_t lane = laneId();
+    const VStack st{S.stNode, spill};
     const float reach = 1e-5f * (__builtin_fabsf(cur0.x) + __builtin_fabsf(cur0.y) + __builtin_fabsf(cur0.z) +
                                  __builtin_fabsf(cur0.w)) + 1e-6f;
     bool exact = need;
@@ -378,11 +405,12 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                 const uint32_t dr = countBelow(donors), ir = countBelow(idle);
                 const bool give = ((donors >> lane) & 1u) && dr < m;
                 const bool take = ((idle >> lane) & 1u) && ir < m;
+                waveSync();  // the donors' stack slots are visible to the takers
                 const uint32_t src = take ? laneOfRank(donors, ir) : lane;
                 const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
                 const uint32_t dOwner = __shfl(owner, int(src), 64);
                 const float dNd = __shfl(nd, int(src), 64);
-                const uint32_t node = take ? S.stNode[dSb][src] : 0u;
+                const uint32_t node = take ? st.get(dSb, src) : 0u;
                 cur = shflF4(cur, src);
                 slack = __shfl(slack, int(src), 64);
                 vx.x = __shfl(vx.x, int(src), 64);
@@ -407,7 +435,7 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                 waveSync();
             }
             const bool wasLive = tr.live;
-            if (tr.live) vertexNearStep(c, L, S.stNode, w.F, i, vx, cur, slack, tr, nd, nb, rflags);
+            if (tr.live) vertexNearStep(c, L, st, w.F, i, vx, cur, slack, tr, nd, nb, rflags);
             // a shared item that ended hands its nearest site to its owner
             const bool report = wasLive && !tr.live && !own && nb != kNone;
             uint64_t hm = ballot(report);
@@ -447,13 +475,14 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
         Trav t2{root, 0u, exact};
         float nd2 = FLT_MAX;
         while (ballot(t2.live) != 0) {
-            if (t2.live) vertexStep(c, L, S.stNode, w.F, i, vx0, cur0, t2, nd2, nb2, flags);
+            if (t2.live) vertexStep(c, L, st, w.F, i, vx0, cur0, t2, nd2, nb2, flags);
         }
     }
     return exact ? nb2 : site;
 }
 
-__device__ __forceinline__ void vertexTask(const CellCtx& c, VertexLds& S, uint32_t i, uint32_t maxRestarts) {
+__device__ __forceinline__ void vertexTask(const CellCtx& c, VertexLds& S, uint32_t i, uint32_t maxRestarts,
+                                           uint32_t* spill) {
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
     const uint8_t st = c.status[i];
@@ -480,7 +509,8 @@ __device__ __forceinline__ void vertexTask(const CellCtx& c, VertexLds& S, uint3
             L.vtx[v].w = (r.w & 0xffu) | (1u << 8);
         }
         // nearest non-neighbour site to the vertex sphere, all lanes' searches in step
-        nb = vertexSearches(c, S, L, w, i, root, need, v < w.V ? L.vtx[v] : uint4{0, 0, 0, 0}, toF(cur), flags);
+        nb = vertexSearches(c, S, L, w, i, root, need, v < w.V ? L.vtx[v] : uint4{0, 0, 0, 0}, toF(cur), spill,
+                            flags);
         bool invalid = false;
         if (nb != kNone) invalid = intersectD(cur, loadSite(c, nb));
         const uint64_t m = ballot(invalid);
@@ -719,6 +749,7 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
             const uint32_t dr = countBelow(donors), ir = countBelow(idle);
             const bool give = ((donors >> lane) & 1u) && dr < m;
             const bool take = ((idle >> lane) & 1u) && ir < m;
+            waveSync();  // the donors' stack slots are visible to the takers
             const uint32_t src = take ? laneOfRank(donors, ir) : lane;
             const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
             const uint32_t dOwner = __shfl(owner, int(src), 64);
@@ -987,6 +1018,7 @@ __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorL
             const uint32_t dr = countBelow(donors), ir = countBelow(idle);
             const bool give = ((donors >> lane) & 1u) && dr < m;
             const bool take = ((idle >> lane) & 1u) && ir < m;
+            waveSync();  // the donors' stack slots are visible to the takers
             const uint32_t src = take ? laneOfRank(donors, ir) : lane;
             const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
             const uint32_t dOwner = __shfl(owner, int(src), 64);
@@ -1119,14 +1151,17 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint32_t* __restrict__ tasks,
                                                           uint32_t* counters, uint32_t* roundTasks,
-                                                          uint32_t maxRestarts) {
+                                                          uint32_t maxRestarts, uint32_t* spill) {
     __shared__ VertexLds S;
+    // the deep stack slots of this wave (the grid has at most one block per
+    // spill region: validationLaunch checks it)
+    uint32_t* mySpill = spill + size_t(blockIdx.x) * kVertexSpillWords;
     const uint32_t count = __builtin_amdgcn_readfirstlane(counters[kTaskCount]);
     if (blockIdx.x == 0 && laneId() == 0) *roundTasks = count;
     for (;;) {
         const uint32_t t = popWork(counters + kVertexQueue, count);
         if (t == kNone) break;
-        vertexTask(c, S, tasks[t], maxRestarts);
+        vertexTask(c, S, tasks[t], maxRestarts, mySpill);
         waveSync();
     }
 }
@@ -1171,6 +1206,8 @@ __global__ void k_round_reset(uint32_t* counters) {
     if (t < kRoundWords) counters[kTaskCount + t] = 0u;
 }
 
+uint32_t vertexSpillWords() { return kVertexSpillWords; }
+
 hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks) {
     int v = 0, e = 0, b = 0;
     hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_vertex_validation, 64, 0);
@@ -1190,7 +1227,7 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const 
         hipLaunchKernelGGL(k_bisector_validation, dim3(g.bisector), dim3(64), 0, st, c, v.counters, maxRestarts);
         return hipGetLastError();
     }
-    if (iterations > kMaxRounds) return hipErrorInvalidValue;
+    if (iterations > kMaxRounds || g.vertex > v.spillBlocks) return hipErrorInvalidValue;
     hipError_t e = hipSuccess;
     for (uint32_t it = 0; it < iterations && e == hipSuccess; it++) {
         hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, st, v.counters);
@@ -1199,7 +1236,7 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const 
         const bool timed = tm && it < kTimedRounds;
         if (timed) e = hipEventRecord(tm->ev[it][0], st);
         hipLaunchKernelGGL(k_vertex_validation, dim3(g.vertex), dim3(64), 0, st, c, v.tasks, v.counters,
-                           v.counters + kRoundTasks + it, maxRestarts);
+                           v.counters + kRoundTasks + it, maxRestarts, v.spill);
         if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][1], st);
         hipLaunchKernelGGL(k_edge_validation, dim3(g.edge), dim3(64), 0, st, c, v.tasks, v.counters, maxRestarts);
         if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][2], st);

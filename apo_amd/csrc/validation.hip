@@ -28,7 +28,7 @@ namespace apo_hip {
 // of every lane in LDS, the rest of the reference's kStackDepth in a
 // per-wave region of device memory (reached only by deep traversals), so
 // the kernel's LDS admits more waves per CU.
-constexpr uint32_t kVertexLdsDepth = 16;
+constexpr uint32_t kVertexLdsDepth = 4;
 constexpr uint32_t kVertexSpillWords = (kStackDepth - kVertexLdsDepth) * 64;
 struct VStack {
     uint32_t (*lds)[64];

@@ -362,7 +362,9 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
                                    ctx->cfg.proposal_fallback_k, ctx->cfg.proposal_max_k, st));
         }
         APO_HIP(mark(7));
-        APO_HIP(cellsKLaunch(cc, ctx->knn, Kk, ctx->cls, nullptr, kProposal, ctx->cfg.proposal_fallback_k, st));
+        APO_HIP(cellsLaunch(cc, ctx->knn, Kk, ctx->counters + kCellQueue,
+                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st, ctx->cls,
+                            ctx->cfg.proposal_fallback_k));
     } else {
         APO_HIP(mark(7));
         APO_HIP(cellsLaunch(cc, ctx->knn, (ctx->cfg.stage_mask & 1u) ? K : 0u, ctx->counters + kCellQueue,

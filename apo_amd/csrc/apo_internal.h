@@ -168,8 +168,10 @@ __device__ __forceinline__ void countBytes(const CellCtx& c, uint32_t family, ui
 hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st);
 hipError_t cellsOccupancy(uint32_t* blocksPerCu);
 // Persistent cell pass over all sites; queue = kXcdQueueWords heads.
+// cls (may be null): learned-proposal classes in sorted order; each cell then
+// runs min(proposalK(class, fallbackK), k) neighbours.
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
-                       hipStream_t st);
+                       hipStream_t st, const uint8_t* cls = nullptr, uint32_t fallbackK = 16u);
 
 // Device counters of a build (words of one buffer): work-queue heads of the
 // persistent kernels (8 per queue, one range per XCD), the task count of

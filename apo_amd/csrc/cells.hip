@@ -21,10 +21,10 @@
 
 namespace apo_hip {
 
-// getCells (algorithm.inl:101-206) for cell i: K updates in kNN order, then
-// the security-radius certification.
+// getCells (algorithm.inl:101-206) for cell i: K updates in kNN order (the
+// first `use` of the K-long list), then the security-radius certification.
 __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uint32_t* __restrict__ knn, uint32_t K,
-                                         uint32_t i) {
+                                         uint32_t use, uint32_t i) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
     WaveCell w;
@@ -50,7 +50,7 @@ __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uin
         const uint64_t noneMask = ballot(myJ == kNone);
         // neighbours are used up to the first empty slot
         const uint32_t count = noneMask ? uint32_t(__builtin_ctzll(noneMask)) : 64u;
-        const uint32_t limit = count < K ? count : K;
+        const uint32_t limit = count < use ? count : use;
         double knownRadius = -DBL_MAX;
         for (uint32_t t = 0; t < limit; t++) {
             if ((buriedMask >> t) & 1u) continue;
@@ -76,14 +76,22 @@ __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uin
 }
 
 // Persistent cell pass: each wave takes cells from the XCD-ranged queue
-// (Morton order within each range) until every cell is done.
+// (Morton order within each range) until every cell is done.  With cls
+// (the learned proposal), cell i runs the neighbour count of its class,
+// at most the list length K.
 __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
-                                              uint32_t* queue) {
+                                              uint32_t* queue, const uint8_t* __restrict__ cls, uint32_t fallbackK) {
     __shared__ CellLds L;
     for (;;) {
         const uint32_t i = popWork(queue, c.n);
         if (i == kNone) break;
-        cellTask(c, L, knn, K, i);
+        uint32_t use = K;
+        if (cls) {
+            const uint32_t q = cls[i];
+            use = q < kProposalClasses ? proposalK(q, fallbackK) : K;
+            if (use > K) use = K;
+        }
+        cellTask(c, L, knn, K, use, i);
         waveSync();
     }
 }
@@ -96,10 +104,10 @@ hipError_t cellsOccupancy(uint32_t* blocksPerCu) {
 }
 
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
-                       hipStream_t st) {
+                       hipStream_t st, const uint8_t* cls, uint32_t fallbackK) {
     if (k > 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, queue, kXcdQueueWords, 0u);
-    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue);
+    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue, cls, fallbackK);
     return hipGetLastError();
 }
 

@@ -414,7 +414,19 @@ apo_status finishBuild(apo_ctx* ctx, uint32_t n, bool learned, bool timed) {
     APO_HIP(hipStreamSynchronize(st));
     if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped: " + guardText(guard));
     if (ctx->cfg.stage_mask & 8u) {
+        // LBVH and kNN only: their stage times, no cells
         ctx->vertexCount = 0;
+        apo_stats& S = ctx->stats;
+        std::memset(&S, 0, sizeof(S));
+        if (timed) {
+            float a = 0.f, b = 0.f;
+            APO_HIP(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+            APO_HIP(hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]));
+            S.ms_lbvh = a;
+            S.ms_knn = b;
+            S.ms_total = a + b;
+        }
+        S.sites = n;
         ctx->built = true;
         return APO_OK;
     }

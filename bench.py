@@ -338,6 +338,37 @@ def protein_section(say):
     return out
 
 
+def lbvh_sizes(say, sizes=(1_000_000, 4_000_000, 16_000_000)):
+    """The LBVH stage alone (scene box, Morton codes, sort, Karras, refit) at
+    growing site counts of the same distribution: eager builds stopped after
+    the kNN pass, LBVH time from the stage events, 440 algorithmic bytes per
+    site (the Morton, sort, leaf, hierarchy and refit traffic)."""
+    import torch
+    from apo_amd import _lib, datasets
+    L = _lib.lib()
+    out = {}
+    for n in sizes:
+        sites = datasets.loguniform_sites(n, seed=3)
+        dev = torch.from_numpy(sites).to("cuda")
+        torch.cuda.synchronize()
+        ctx = _ctx({"stage_mask": 8})
+        ptr = _lib.ctypes.c_void_p(dev.data_ptr())
+        times = []
+        for _ in range(4):
+            _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+            s = _lib.Stats()
+            _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+            times.append(s.ms_lbvh)
+        L.apo_destroy(ctx)
+        del dev
+        torch.cuda.empty_cache()
+        ms = float(np.median(times[1:]))
+        gbs = 440.0 * n / (ms * 1e-3) / 1e9
+        out[str(n)] = {"ms": ms, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        say(f"LBVH at {n} sites: {ms:.3f} ms, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+    return out
+
+
 def cpu_baseline(sites: np.ndarray, target: int):
     """Time the oracle (host C++/OpenMP restatement of the reference's method,
     reference edge searches) on a sub-cube of the workload holding about
@@ -558,6 +589,7 @@ def main():
     if not args.no_batch:
         out["batch"] = batch_workload(args.batch_molecules, min(args.steps, 3), 1, say)
         out["proteins"] = protein_section(say)
+        out["lbvh_sizes"] = lbvh_sizes(say)
     if not args.no_cpu_baseline:
         say(f"CPU baseline: oracle on a sub-cube of about {args.cpu_sample} sites")
         out["cpu_baseline"] = cpu_baseline(sites, args.cpu_sample)

@@ -681,14 +681,16 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     if (ctx->givenN && ctx->givenN != n)
         return fail(ctx, APO_ERR_INVALID, "proposal classes were given for another site count");
     ctx->cavValid = false;
-    // deep stack slots of every wave of the vertex-validation grid
-    const uint32_t vblocks = persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex);
+    // per-wave device regions of the vertex and edge validation grids (the
+    // two kernels run one after the other on the stream and share them)
+    const uint32_t vblocks = std::max(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
+                                      persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge));
     if (vblocks > ctx->spillBlocks) {
         dropGraph(ctx);
         if (ctx->spill) APO_HIP(hipFree(ctx->spill));
         ctx->spill = nullptr;
         ctx->spillBlocks = 0;
-        APO_HIP(alloc(ctx->spill, size_t(vblocks) * vertexSpillWords()));
+        APO_HIP(alloc(ctx->spill, size_t(vblocks) * waveScratchWords()));
         ctx->spillBlocks = vblocks;
     }
     apo_status s = ensureCapacity(ctx, n, learnedPath(ctx) ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);

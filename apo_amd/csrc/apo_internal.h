@@ -223,11 +223,12 @@ __device__ __forceinline__ uint32_t popWork(uint32_t* heads, uint32_t count) {
 struct ValidationBuffers {
     uint32_t* tasks;       // n
     uint32_t* counters;    // kCounterWords
-    uint32_t* spill;       // spillBlocks * vertexSpillWords(): deep stack slots of the vertex searches
-    uint32_t spillBlocks;  // the most blocks a vertex validation grid may have
+    uint32_t* spill;       // spillBlocks * waveScratchWords(): per-wave deep stack slots and edge cover buffers
+    uint32_t spillBlocks;  // the most blocks a vertex or edge validation grid may have
 };
-// Words of device memory per vertex-validation wave for its deep stack slots.
-uint32_t vertexSpillWords();
+// Words of device memory per validation wave (deep stack slots of the vertex
+// and edge searches, then the edge cover buffers).
+uint32_t waveScratchWords();
 // Persistent grid sizes (blocks of one wave).
 struct ValidationGrid {
     uint32_t vertex, edge, bisector;

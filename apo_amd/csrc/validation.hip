@@ -24,12 +24,15 @@
 
 namespace apo_hip {
 
-// Traversal stack of the vertex searches: the first kVertexLdsDepth slots
-// of every lane in LDS, the rest of the reference's kStackDepth in a
-// per-wave region of device memory (reached only by deep traversals), so
-// the kernel's LDS admits more waves per CU.
+// Traversal stack of the vertex and edge searches: the first
+// kVertexLdsDepth slots of every lane in LDS, the rest of the reference's
+// kStackDepth in a per-wave region of device memory (reached only by deeper
+// traversals), so the kernels' LDS admits more waves per CU.  The edge
+// kernel keeps its cover buffers in the same region, after the stack.
 constexpr uint32_t kVertexLdsDepth = 4;
 constexpr uint32_t kVertexSpillWords = (kStackDepth - kVertexLdsDepth) * 64;
+constexpr uint32_t kWaveScratchWords = kVertexSpillWords + uint32_t(sizeof(CoverLds) / sizeof(uint32_t));
+static_assert(kVertexSpillWords % 2u == 0u, "the cover buffers of a wave region are 8-byte aligned");
 struct VStack {
     uint32_t (*lds)[64];
     uint32_t* spill;  // kVertexSpillWords words of this wave
@@ -51,10 +54,7 @@ struct VertexLds {
 };
 struct EdgeLds {
     CellLds cell;
-    union {
-        uint32_t stNode[kStackDepth][64];
-        CoverLds cover;  // edge covers are built while no traversal runs
-    };
+    uint32_t stNode[kVertexLdsDepth][64];
     CoverOut coverOut;
     uint32_t res[64];     // per owner lane: earliest contributor found by a helper
 };
@@ -64,7 +64,6 @@ struct BisectorLds {
     uint32_t bis[kBisectorCap];
     uint32_t res[64];     // per owner lane: earliest contributor found by a helper
 };
-static_assert(sizeof(CoverLds) <= sizeof(uint32_t) * kStackDepth * 64, "cover buffers overlay the stack");
 
 // Traversal state of one lane: next node and stack height.  With work
 // sharing (edge searches) the stack occupies slots [sb, sp): the slots
@@ -556,7 +555,7 @@ struct EdgeQuery {
 // hyperbolic edge, on the right side of both tangent planes.  A lane walks
 // its traversal in the reference's order; the wave never waits for lanes
 // to reach leaves together.
-__device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, uint32_t (*stNode)[64], uint32_t i, uint32_t k,
+__device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, const VStack& stNode, uint32_t i, uint32_t k,
                                              uint32_t l, const EdgeQuery& q, Trav& t, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
@@ -606,7 +605,7 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, uint32_t (*stNode
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode[t.sp][lane] = farChild;
+                    stNode.set(t.sp, lane, farChild);
                     t.sp++;
                 } else {
                     flags |= kFlagStackOverflow;
@@ -682,8 +681,9 @@ __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L
 // the answer if the owner's own item finds none.
 __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, const CellLds& L, const WaveCell& w,
                                               uint32_t i, const D4& si, uint32_t root, bool need, uint32_t k,
-                                              uint32_t l, const EdgeQuery& q, uint32_t& flags) {
+                                              uint32_t l, const EdgeQuery& q, uint32_t* spill, uint32_t& flags) {
     const uint32_t lane = laneId();
+    const VStack st{S.stNode, spill};
     S.res[lane] = kNone;
     uint64_t settled = 0;  // owner lanes whose own item found the contributor
     // the item this lane works on: the query it serves and its owner
@@ -753,7 +753,7 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
             const uint32_t src = take ? laneOfRank(donors, ir) : lane;
             const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
             const uint32_t dOwner = __shfl(owner, int(src), 64);
-            const uint32_t node = take ? S.stNode[dSb][src] : 0u;
+            const uint32_t node = take ? st.get(dSb, src) : 0u;
             wq.bs = shflF4(wq.bs, src);
             wq.hyperbola = __shfl(wq.hyperbola ? 1 : 0, int(src), 64) != 0;
             wq.n1 = shflF3(wq.n1, src);
@@ -779,13 +779,15 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
             if (give) tr.sb++;
             waveSync();
         }
-        if (tr.live && cand == kNone) cand = edgeStep(c, S.stNode, i, wk, wl, wq, tr, flags);
+        if (tr.live && cand == kNone) cand = edgeStep(c, st, i, wk, wl, wq, tr, flags);
     }
     waveSync();
     return need && found == kNone ? S.res[lane] : found;
 }
 
-__device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t i, uint32_t maxRestarts) {
+__device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t i, uint32_t maxRestarts,
+                                         uint32_t* scratch) {
+    CoverLds& cover = *reinterpret_cast<CoverLds*>(scratch + kVertexSpillWords);
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
     const uint8_t st = c.status[i];
@@ -859,7 +861,7 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
             const uint32_t t = uint32_t(__builtin_ctzll(bm));
             bm &= bm - 1;
             const int cnt =
-                waveEdgeCover(c, L, S.cover, S.coverOut, w.F, si, readLane(k, t), readLane(l, t));
+                waveEdgeCover(c, L, cover, S.coverOut, w.F, si, readLane(k, t), readLane(l, t));
             const bool mine = cnt > 0 && lane == t;
             q.cover = mine ? cnt : q.cover;
 #pragma unroll
@@ -868,7 +870,7 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
         }
         // the first contributor of each lane's traversal: a candidate with two
         // vertex solutions of which one is clear of the cell's neighbours
-        const uint32_t found = edgeSearches(c, S, L, w, i, si, root, need, k, l, q, flags);
+        const uint32_t found = edgeSearches(c, S, L, w, i, si, root, need, k, l, q, scratch, flags);
         if (need && found == kNone) L.edg[e].w = 1u;
         const uint64_t m = ballot(found != kNone);
         waveSync();
@@ -1154,8 +1156,8 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
                                                           uint32_t maxRestarts, uint32_t* spill) {
     __shared__ VertexLds S;
     // the deep stack slots of this wave (the grid has at most one block per
-    // spill region: validationLaunch checks it)
-    uint32_t* mySpill = spill + size_t(blockIdx.x) * kVertexSpillWords;
+    // wave region: validationLaunch checks it)
+    uint32_t* mySpill = spill + size_t(blockIdx.x) * kWaveScratchWords;
     const uint32_t count = __builtin_amdgcn_readfirstlane(counters[kTaskCount]);
     if (blockIdx.x == 0 && laneId() == 0) *roundTasks = count;
     for (;;) {
@@ -1166,14 +1168,15 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
     }
 }
 
-__global__ __launch_bounds__(64) void k_edge_validation(CellCtx c, const uint32_t* __restrict__ tasks,
-                                                        uint32_t* counters, uint32_t maxRestarts) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_edge_validation(CellCtx c, const uint32_t* __restrict__ tasks,
+                                                        uint32_t* counters, uint32_t maxRestarts, uint32_t* scratch) {
     __shared__ EdgeLds S;
+    uint32_t* myScratch = scratch + size_t(blockIdx.x) * kWaveScratchWords;
     const uint32_t count = __builtin_amdgcn_readfirstlane(counters[kTaskCount]);
     for (;;) {
         const uint32_t t = popWork(counters + kEdgeQueue, count);
         if (t == kNone) break;
-        edgeTask(c, S, tasks[t], maxRestarts);
+        edgeTask(c, S, tasks[t], maxRestarts, myScratch);
         waveSync();
     }
 }
@@ -1206,7 +1209,7 @@ __global__ void k_round_reset(uint32_t* counters) {
     if (t < kRoundWords) counters[kTaskCount + t] = 0u;
 }
 
-uint32_t vertexSpillWords() { return kVertexSpillWords; }
+uint32_t waveScratchWords() { return kWaveScratchWords; }
 
 hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks) {
     int v = 0, e = 0, b = 0;
@@ -1227,7 +1230,7 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const 
         hipLaunchKernelGGL(k_bisector_validation, dim3(g.bisector), dim3(64), 0, st, c, v.counters, maxRestarts);
         return hipGetLastError();
     }
-    if (iterations > kMaxRounds || g.vertex > v.spillBlocks) return hipErrorInvalidValue;
+    if (iterations > kMaxRounds || g.vertex > v.spillBlocks || g.edge > v.spillBlocks) return hipErrorInvalidValue;
     hipError_t e = hipSuccess;
     for (uint32_t it = 0; it < iterations && e == hipSuccess; it++) {
         hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, st, v.counters);
@@ -1238,7 +1241,8 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const 
         hipLaunchKernelGGL(k_vertex_validation, dim3(g.vertex), dim3(64), 0, st, c, v.tasks, v.counters,
                            v.counters + kRoundTasks + it, maxRestarts, v.spill);
         if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][1], st);
-        hipLaunchKernelGGL(k_edge_validation, dim3(g.edge), dim3(64), 0, st, c, v.tasks, v.counters, maxRestarts);
+        hipLaunchKernelGGL(k_edge_validation, dim3(g.edge), dim3(64), 0, st, c, v.tasks, v.counters, maxRestarts,
+                           v.spill);
         if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][2], st);
         if (e == hipSuccess) e = hipGetLastError();
     }

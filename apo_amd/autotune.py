@@ -209,8 +209,8 @@ def tune(sites, workload: str, offsets=None, proposal=None, repeats: int = 2,
     result = {"workload": workload, "sites": n, "occupancy": occ, "plain": best, "log": log,
               "ms_total_default": default_total, "ms_total_tuned": tuned_total, "same_vertices_as_default": same}
     if proposal is not None:
-        # whole-build scores differ by a few ms between runs: one more repeat
-        lrun = _Runner(sites, offsets=offsets, proposal=proposal, repeats=max(repeats, 3))
+        # whole-build scores differ by several ms between runs: more repeats
+        lrun = _Runner(sites, offsets=offsets, proposal=proposal, repeats=max(repeats, 5))
 
         def learned_score(cfg, knob):
             t = statistics.median(s["ms_total"] for s in lrun.stats(cfg))

@@ -58,7 +58,20 @@ __device__ inline void buildNeighbours(const CellCtx& c, CellLds& L, WaveCell& w
         }
         bool f0 = has, f1 = has && x1 != x0;
         const uint32_t lim = (w.E - b) < 64u ? w.E : b + 64u;
-        for (uint32_t e2 = 0; e2 < lim; e2++) {
+        // earlier edges, eight per step: independent LDS reads in flight
+        uint32_t e2 = 0;
+        for (; e2 + 8u <= lim; e2 += 8u) {
+            uint4 y[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8u; u++) y[u] = L.edg[e2 + u];
+#pragma unroll
+            for (uint32_t u = 0; u < 8u; u++) {
+                const bool before = e2 + u < e;
+                if (before && (y[u].x == x0 || y[u].y == x0)) f0 = false;
+                if (before && (y[u].x == x1 || y[u].y == x1)) f1 = false;
+            }
+        }
+        for (; e2 < lim; e2++) {
             const uint4 y = L.edg[e2];
             if (e2 < e) {
                 if (y.x == x0 || y.y == x0) f0 = false;

@@ -22,7 +22,6 @@ struct CellLds {
     uint32_t nb[kNeighbourCap];
     uint32_t ne[kNewEdgeCap];
     uint32_t list[kListCap];
-    uint32_t knn[64];
 };
 
 struct WaveCell {
@@ -99,7 +98,21 @@ __device__ __forceinline__ D4 nbSite(const CellCtx& c, const CellLds& L, uint32_
 // (topology_update.inl:578-592).  Per lane, serial over the neighbour set.
 __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLds& L, uint32_t F, const D4& cand,
                                                   uint32_t skip1, uint32_t skip2) {
-    for (uint32_t q = 0; q < F; q++) {
+    uint32_t q = 0;
+    // four neighbours per step: their site loads are in flight together
+    for (; q + 4u <= F; q += 4u) {
+        uint32_t id[4];
+        D4 s[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; u++) id[u] = __builtin_amdgcn_readfirstlane(L.nb[q + u]);
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; u++) s[u] = nbSite(c, L, q + u, id[u]);
+        bool hit = false;
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; u++) hit |= id[u] != skip1 && id[u] != skip2 && intersectD(s[u], cand);
+        if (hit) return false;
+    }
+    for (; q < F; q++) {
         const uint32_t id = __builtin_amdgcn_readfirstlane(L.nb[q]);
         if (id == skip1 || id == skip2) continue;
         if (intersectD(nbSite(c, L, q, id), cand)) return false;

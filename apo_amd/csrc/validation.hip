@@ -138,7 +138,17 @@ __device__ __forceinline__ bool knownSite(const CellLds& L, const WaveCell& w, u
 
 // same test from one lane, over the neighbour set
 __device__ __forceinline__ bool knownLane(const CellLds& L, uint32_t F, uint32_t j) {
-    for (uint32_t q = 0; q < F; q++)
+    uint32_t q = 0;
+    for (; q + 8u <= F; q += 8u) {
+        uint32_t o[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; u++) o[u] = L.nb[q + u];
+        bool hit = false;
+#pragma unroll
+        for (uint32_t u = 0; u < 8u; u++) hit |= o[u] == j;
+        if (hit) return true;
+    }
+    for (; q < F; q++)
         if (L.nb[q] == j) return true;
     return false;
 }

@@ -95,22 +95,26 @@ __device__ __forceinline__ D4 nbSite(const CellCtx& c, const CellLds& L, uint32_
 }
 
 // Candidate sphere is clear of every neighbour except skip1/skip2
-// (topology_update.inl:578-592).  Per lane, serial over the neighbour set.
+// (topology_update.inl:578-592).  Per lane, serial over the neighbour set,
+// U neighbours per step: their site loads are in flight together (callers
+// with many live registers take fewer).
+template <uint32_t U = 4>
 __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLds& L, uint32_t F, const D4& cand,
                                                   uint32_t skip1, uint32_t skip2) {
     uint32_t q = 0;
-    // four neighbours per step: their site loads are in flight together
-    for (; q + 4u <= F; q += 4u) {
-        uint32_t id[4];
-        D4 s[4];
+    if constexpr (U > 1u) {
+        for (; q + U <= F; q += U) {
+            uint32_t id[U];
+            D4 s[U];
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) id[u] = __builtin_amdgcn_readfirstlane(L.nb[q + u]);
+            for (uint32_t u = 0; u < U; u++) id[u] = __builtin_amdgcn_readfirstlane(L.nb[q + u]);
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) s[u] = nbSite(c, L, q + u, id[u]);
-        bool hit = false;
+            for (uint32_t u = 0; u < U; u++) s[u] = nbSite(c, L, q + u, id[u]);
+            bool hit = false;
 #pragma unroll
-        for (uint32_t u = 0; u < 4u; u++) hit |= id[u] != skip1 && id[u] != skip2 && intersectD(s[u], cand);
-        if (hit) return false;
+            for (uint32_t u = 0; u < U; u++) hit |= id[u] != skip1 && id[u] != skip2 && intersectD(s[u], cand);
+            if (hit) return false;
+        }
     }
     for (; q < F; q++) {
         const uint32_t id = __builtin_amdgcn_readfirstlane(L.nb[q]);

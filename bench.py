@@ -483,6 +483,21 @@ def main():
         with open(tp) as f:
             traffic = float(json.load(f)["traffic_bytes_per_launch"])
         traffic_src = os.path.relpath(tp, ROOT)
+    # VALU instructions per launch of that kernel from the SQ counter passes
+    # committed under profiles/ (same workload): the issue rate it reaches
+    # with this run's launch time, against 4 SIMDs per CU issuing one wave64
+    # VALU instruction every 4 cycles at the 2.4 GHz peak clock
+    valu_issue = None
+    vp = os.path.join(ROOT, "profiles", "pmc_valu_validation_kernels_1M_loguniform.json")
+    if n == 1_000_000 and args.k == 16 and os.path.exists(vp):
+        with open(vp) as f:
+            vk = json.load(f)["kernels"].get(dominant)
+        if vk:
+            cus = torch.cuda.get_device_properties(0).multi_processor_count
+            peak = cus * 4 * 2.4e9 / 4.0
+            rate = vk["valu_insts_per_launch"] / (ms_k / max(int(launches), 1) * 1e-3)
+            valu_issue = {"achieved": rate, "peak": peak, "unit": "wave64 VALU insts/s", "frac": rate / peak,
+                          "lanes_per_inst": vk["lanes_per_valu_inst"], "source": os.path.relpath(vp, ROOT)}
 
     violations, precision = None, None
     if not args.no_check:
@@ -528,7 +543,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "launches": int(launches), "avg_launch_ms": ms_k / max(int(launches), 1),
                      "bytes_per_launch": bytes_k / max(int(launches), 1),
-                     "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS},
+                     "lbvh_achieved": lbvh_gbs, "lbvh_frac": lbvh_gbs / HBM_PEAK_GBS,
+                     "valu_issue": valu_issue},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
         # every persistent kernel's algorithmic bytes and launch time (the
         # counter passes of each one are compared against these)

@@ -118,3 +118,23 @@ def batched_molecules(count: int = 512, seed: int = 4, lo: int = 500, hi: int = 
     for m in range(count):
         sites[offsets[m]:offsets[m + 1]] = protein_like_sites(int(sizes[m]), seed=seed, density=density, seed_tag=m)
     return sites, offsets
+
+
+def crowded_shell_sites(shell: int, seed: int = 7) -> np.ndarray:
+    """One large site (r = 0.05 at the cube's centre) ringed by `shell` small
+    sites (r in [0.001, 0.003]) on a thin shell at distance 0.062-0.07, plus
+    200 scattered sites.  The large site's cell has one edge per pair of
+    adjacent shell sites: about 200 edges at shell = 60, and more than the
+    per-cell edge capacity at shell = 150 (the cell is flagged and
+    truncated).  A stress case for the per-cell record capacities and for
+    scans that take a cell's edges 64 at a time."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(shell, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ring = np.zeros((shell, 4), np.float32)
+    ring[:, :3] = 0.5 + d * rng.uniform(0.062, 0.07, (shell, 1))
+    ring[:, 3] = rng.uniform(0.001, 0.003, shell)
+    bg = np.zeros((200, 4), np.float32)
+    bg[:, :3] = rng.random((200, 3))
+    bg[:, 3] = rng.uniform(0.01, 0.03, 200)
+    return np.concatenate([[[0.5, 0.5, 0.5, 0.05]], ring, bg]).astype(np.float32)

@@ -71,6 +71,8 @@ CASES = {
     "zero_radii": _zero_radii,
     "far_negative": _far_negative,
     "protein_like_small": lambda: datasets.protein_like_sites(600, seed=21),
+    "crowded_shell_60": lambda: datasets.crowded_shell_sites(60),
+    "crowded_shell_150": lambda: datasets.crowded_shell_sites(150),
 }
 
 
@@ -90,6 +92,27 @@ def test_edge_case_matches_oracle(name):
     assert np.array_equal(g.vertices.view(np.uint64), ox.view(np.uint64)), name
     assert np.array_equal(g.closed_edges_id, ei), name
     assert np.array_equal(g.closed_edges_min.view(np.uint64), ex.view(np.uint64)), name
+
+
+@pytest.mark.parametrize("shell", [60, 150])
+def test_crowded_cell_records_match_oracle(shell):
+    """A cell of about 200 edges (four 64-edge steps of every per-edge scan)
+    and one past the edge capacity (flagged and truncated): every cell
+    record, count and flag equals the oracle's."""
+    sites = datasets.crowded_shell_sites(shell)
+    print(f"[case] crowded shell {shell} n={len(sites)} cell records", flush=True)
+    gpu = AlgorithmGPU(sites).build()
+    try:
+        gc, gV, gE = gpu.all_cells()
+    finally:
+        gpu.close()
+    oc, oV, oE = oracle.Diagram(sites).all_cells()
+    for key in ("status", "vertex_nb", "edge_nb", "flags"):
+        assert np.array_equal(gc[key], oc[key]), key
+    assert np.array_equal(gc["known_radius"].view(np.uint32), oc["known_radius"].view(np.uint32))
+    assert np.array_equal(gV, oV)
+    assert np.array_equal(gE, oE)
+    assert int(gc["edge_nb"].max()) > 192
 
 
 def test_invalid_arguments_fail_with_a_status():

@@ -124,6 +124,45 @@ __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLd
     return true;
 }
 
+// Both candidate spheres of an edge against the neighbour set in one pass:
+// the bits of `mask` (bit 0: x0, bit 1: x1) whose candidate is clear of
+// every neighbour except skip1/skip2.  Equal to two clearOfNeighbours calls;
+// each neighbour's site is loaded once for both.
+template <uint32_t U = 4>
+__device__ __forceinline__ uint32_t clearPairOfNeighbours(const CellCtx& c, const CellLds& L, uint32_t F,
+                                                          const D4& x0, const D4& x1, uint32_t mask,
+                                                          uint32_t skip1, uint32_t skip2) {
+    uint32_t ok = mask;
+    uint32_t q = 0;
+    if constexpr (U > 1u) {
+        for (; q + U <= F && ok; q += U) {
+            uint32_t id[U];
+            D4 s[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) id[u] = __builtin_amdgcn_readfirstlane(L.nb[q + u]);
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) s[u] = nbSite(c, L, q + u, id[u]);
+            bool h0 = false, h1 = false;
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const bool use = id[u] != skip1 && id[u] != skip2;
+                h0 |= use && (ok & 1u) && intersectD(s[u], x0);
+                h1 |= use && (ok & 2u) && intersectD(s[u], x1);
+            }
+            if (h0) ok &= ~1u;
+            if (h1) ok &= ~2u;
+        }
+    }
+    for (; q < F && ok; q++) {
+        const uint32_t id = __builtin_amdgcn_readfirstlane(L.nb[q]);
+        if (id == skip1 || id == skip2) continue;
+        const D4 s = nbSite(c, L, q, id);
+        if ((ok & 1u) && intersectD(s, x0)) ok &= ~1u;
+        if ((ok & 2u) && intersectD(s, x1)) ok &= ~2u;
+    }
+    return ok;
+}
+
 // Stable in-place removal of records flagged by `drop`, step by step.
 template <typename Drop>
 __device__ __forceinline__ uint32_t compact(uint4* rec, uint32_t count, Drop drop) {
@@ -227,11 +266,7 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
             trisectorAliased(si, sk, sl, t);
             if (intersectD(t, sj)) vnb = 0;
         }
-        uint32_t valid = mask;
-        if (mask & 1u)
-            if (!clearOfNeighbours(c, L, w.F, q0, k, l)) valid &= ~1u;
-        if (mask & 2u)
-            if (!clearOfNeighbours(c, L, w.F, q1, k, l)) valid &= ~2u;
+        const uint32_t valid = clearPairOfNeighbours(c, L, w.F, q0, q1, mask, k, l);
         const uint32_t nNew = uint32_t(__builtin_popcount(valid));
         if (has && mask != 0 && nNew > 0) {
             if (vnb == kNone) vnb = 0;

@@ -724,10 +724,8 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
                     D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
                     uint32_t mask = quadrisector(si, loadSite(c, cand), wsk, wsl, x0, x1);
                     if (mask != 3u) mask = 0u;
-                    uint32_t valid = mask;
                     // two neighbours per step here: the search holds many live registers
-                    if ((mask & 1u) && !clearOfNeighbours<2>(c, L, w.F, x0, wk, wl)) valid &= ~1u;
-                    if ((mask & 2u) && !clearOfNeighbours<2>(c, L, w.F, x1, wk, wl)) valid &= ~2u;
+                    const uint32_t valid = clearPairOfNeighbours<2>(c, L, w.F, x0, x1, mask, wk, wl);
                     ok = valid != 0u;
                 }
                 if (ok) {

@@ -3,7 +3,8 @@
 This mirrors the construction API of PlathC/apo
 (apo/include/apo/gpu/algorithm.hpp:9-59, algorithm.cuh:12-46):
 
-    AlgorithmGPU(sites, k=16).build()           -> AlgorithmGPU::build
+    AlgorithmGPU(sites, k=16, knn_validation_step=1).build()
+                                                -> AlgorithmGPU<K>(sites, knnValidationStep).build
     AlgorithmGPU.to_host()                      -> AlgorithmGPU::toHost -> VertexDiagram
     AlgorithmGPU.to_host_full()                 -> AlgorithmGPU::toHostFull -> FullDiagram
     AlgorithmGPU.topology()                     -> AlgorithmGPU::toTopology -> Topology sets
@@ -51,6 +52,10 @@ class FullDiagram:
 class AlgorithmGPU:
     """One construction context on one GPU (AlgorithmGPU<K>, algorithm.cuh:12-46).
 
+    k: neighbours per kNN pass (the template argument K, 1..64).
+    knn_validation_step: kNN + cell passes (the constructor's second
+    argument, algorithm.cuh:16); each later pass continues the neighbour
+    lists after the previous pass's last neighbour.
     offsets: for a batch, the molecule boundaries (m + 1 values, molecule i is
     sites[offsets[i]:offsets[i+1]]), built in one pass of offset-indexed
     LBVHs and persistent work-queue kernels.  graph: capture the device build
@@ -60,7 +65,8 @@ class AlgorithmGPU:
 
     def __init__(self, sites, k: int = 16, max_iterations: int = 10, max_restarts: int = 100,
                  stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0, proposal=None,
-                 proposal_cost: float = 0.0, offsets=None, graph: bool = False, classes=None, **tuning):
+                 proposal_cost: float = 0.0, offsets=None, graph: bool = False, classes=None,
+                 knn_validation_step: int = 1, **tuning):
         self.sites = np.ascontiguousarray(np.asarray(sites, dtype=np.float32).reshape(-1, 4))
         self.offsets = None if offsets is None else np.ascontiguousarray(offsets, np.uint32)
         self.k = int(k)
@@ -74,6 +80,7 @@ class AlgorithmGPU:
         cfg.device = device
         cfg.proposal_cost = proposal_cost
         cfg.graph = 1 if graph else 0
+        cfg.knn_passes = int(knn_validation_step)
         # further apo_config fields (cover_ratio, waves_edge, proposal_fallback_k ...)
         names = {f for f, _ in cfg._fields_}
         for key, val in tuning.items():

@@ -27,7 +27,7 @@ class Config(ctypes.Structure):
                 ("proposal_cost", ctypes.c_float), ("graph", ctypes.c_uint32), ("cover_ratio", ctypes.c_float),
                 ("waves_cells", ctypes.c_uint32), ("waves_vertex", ctypes.c_uint32), ("waves_edge", ctypes.c_uint32),
                 ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32),
-                ("proposal_max_k", ctypes.c_uint32)]
+                ("proposal_max_k", ctypes.c_uint32), ("knn_passes", ctypes.c_uint32)]
 
 
 class Precision(ctypes.Structure):
@@ -64,6 +64,8 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 # symbol -> (restype, argtypes); the list is exactly what include/apo_c.h declares
 SIGNATURES = {
     "apo_default_config": (None, [ctypes.POINTER(Config)]),
+    "apo_joggle": (None, [_f64p, ctypes.c_uint64, ctypes.c_double]),
+    "apo_joggle_f32": (None, [_f32p, ctypes.c_uint64, ctypes.c_double]),
     "apo_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(Config)]),
     "apo_destroy": (None, [_vp]),
     "apo_last_error": (ctypes.c_char_p, [_vp]),

@@ -60,6 +60,7 @@ def build(verbose: bool = True) -> str:
 
 
 EXAMPLE = os.path.join(LIBDIR, "apo_diagram")
+SHIM_CALLER = os.path.join(LIBDIR, "apo_reference_caller")
 SAMPLES = os.path.join(LIBDIR, "libapo_samples.so")
 
 
@@ -76,15 +77,22 @@ def build_samples() -> str:
 
 
 def _build_example() -> None:
-    """The C++ caller of include/apo/gpu/algorithm.hpp, linked to the library."""
+    """The C++ callers of include/apo/gpu/algorithm.hpp, linked to the library:
+    examples/diagram.cpp (FP32 sites), and examples/reference_caller.cpp with
+    the reference-side shim examples/apo_gpu_shim.cpp (Real sites, the
+    header in its declarations-only form)."""
     include = os.path.join(os.path.dirname(HERE), "include")
-    src = os.path.join(HERE, "examples", "diagram.cpp")
-    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", include, src, "-o", EXAMPLE + ".tmp",
-           "-L", LIBDIR, "-lapo_hip", "-Wl,-rpath,$ORIGIN"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError("example build failed:\n" + res.stderr)
-    os.replace(EXAMPLE + ".tmp", EXAMPLE)
+    ex = os.path.join(HERE, "examples")
+    cxx = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Werror", "-I", include]
+    link = ["-L", LIBDIR, "-lapo_hip", "-Wl,-rpath,$ORIGIN"]
+    for out, srcs, extra in ((EXAMPLE, ["diagram.cpp"], []),
+                             (SHIM_CALLER, ["reference_caller.cpp", "apo_gpu_shim.cpp"],
+                              ["-DAPO_GPU_DECLARATIONS_ONLY"])):
+        cmd = [*cxx, *extra, *[os.path.join(ex, s) for s in srcs], "-o", out + ".tmp", *link]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"example build failed ({os.path.basename(out)}):\n" + res.stderr)
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

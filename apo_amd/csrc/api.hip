@@ -25,6 +25,9 @@ using namespace apo_hip;
 
 namespace {
 
+// kNN passes after the first whose kNN launch is timed (the rest count as cell time)
+constexpr uint32_t kTimedPasses = 32;
+
 // What a captured build graph depends on besides buffer addresses (a change
 // of any buffer drops the graph).
 struct GraphKey {
@@ -126,6 +129,7 @@ struct apo_ctx {
     uint64_t cavCap = 0;
     bool cavValid = false;
     hipEvent_t ev[8] = {};
+    hipEvent_t passEv[kTimedPasses][2] = {};  // around the kNN launch of passes 1..kTimedPasses
     ValidationTiming vt{};
     apo_stats stats{};
 };
@@ -347,7 +351,14 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     APO_HIP(mark(1));
     // the learned path's lists hold the most neighbours any class may ask for
     const uint32_t Kk = learned ? proposalListK(ctx->cfg.proposal_max_k, ctx->cfg.proposal_fallback_k) : K;
-    APO_HIP(knnLaunch(cc, Kk, ctx->knn, st));
+    const uint32_t passes = learned ? 1u : ctx->cfg.knn_passes;
+    if (passes == 0u) {
+        // no kNN or cell pass (knnValidationStep = 0): empty lists, every cell
+        // in validation; the cell launch below writes the octahedra
+        APO_HIP(cellsResetLaunch(cc, ctx->knn, Kk, st));
+    } else {
+        APO_HIP(knnLaunch(cc, Kk, ctx->knn, st, 0u));
+    }
     APO_TIMED_STAGE("knn");
     APO_HIP(mark(2));
     if (ctx->cfg.stage_mask & 8u) return APO_OK;  // stop after the kNN pass
@@ -367,8 +378,19 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
                             ctx->cfg.proposal_fallback_k));
     } else {
         APO_HIP(mark(7));
-        APO_HIP(cellsLaunch(cc, ctx->knn, (ctx->cfg.stage_mask & 1u) ? K : 0u, ctx->counters + kCellQueue,
-                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st));
+        const uint32_t cellBlocks = persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells);
+        APO_HIP(cellsLaunch(cc, ctx->knn, (ctx->cfg.stage_mask & 1u) && passes > 0u ? K : 0u,
+                            ctx->counters + kCellQueue, cellBlocks, st));
+        // knnValidationStep > 1 (algorithm.inl:1535-1549): the next K neighbours
+        // of every site, then the cells still in validation continue with them
+        for (uint32_t s = 1; s < passes; s++) {
+            const bool tp = timed && s <= kTimedPasses;
+            if (tp) APO_HIP(hipEventRecord(ctx->passEv[s - 1][0], st));
+            APO_HIP(knnLaunch(cc, K, ctx->knn, st, s));
+            if (tp) APO_HIP(hipEventRecord(ctx->passEv[s - 1][1], st));
+            if (ctx->cfg.stage_mask & 1u)
+                APO_HIP(cellsLaunch(cc, ctx->knn, K, ctx->counters + kCellQueue, cellBlocks, st, nullptr, 16u, true));
+        }
     }
     APO_TIMED_STAGE("cells");
     APO_HIP(mark(3));
@@ -452,9 +474,17 @@ apo_status finishBuild(apo_ctx* ctx, uint32_t n, bool learned, bool timed) {
         float msProposal = 0.f, msCellsOnly = 0.f;
         APO_HIP(hipEventElapsedTime(&msProposal, ctx->ev[2], ctx->ev[7]));
         APO_HIP(hipEventElapsedTime(&msCellsOnly, ctx->ev[7], ctx->ev[3]));
+        // kNN launches of the later passes sit inside the cell interval
+        float msLaterKnn = 0.f;
+        const uint32_t passes = learned ? 1u : ctx->cfg.knn_passes;
+        for (uint32_t s = 1; s < passes && s <= kTimedPasses; s++) {
+            float a = 0.f;
+            APO_HIP(hipEventElapsedTime(&a, ctx->passEv[s - 1][0], ctx->passEv[s - 1][1]));
+            msLaterKnn += a;
+        }
         S.ms_lbvh = ms[0];
-        S.ms_knn = ms[1];
-        S.ms_cells = msCellsOnly;
+        S.ms_knn = ms[1] + msLaterKnn;
+        S.ms_cells = msCellsOnly - msLaterKnn;
         S.ms_proposal = msProposal;
         S.ms_validation = ms[3];
         S.ms_bisector = ms[4];
@@ -597,6 +627,33 @@ void apo_default_config(apo_config* cfg) {
     cfg->waves_cells = cfg->waves_vertex = cfg->waves_edge = cfg->waves_bisector = 0;
     cfg->proposal_fallback_k = 16;
     cfg->proposal_max_k = 64;
+    cfg->knn_passes = 1;
+}
+
+// pcg hash of apo/src/core/utils.cpp:6-12, scaled by 2^-32
+static double joggleNoise(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return std::ldexp(double((word >> 22u) ^ word), -32);
+}
+
+void apo_joggle(double* xyzr, uint64_t values, double range) {
+    if (!xyzr) return;
+    const double a = 2.0 * range, b = -range;
+    const uint64_t count = values / 4;
+    for (uint32_t i = 0; i < count; i++)
+        for (uint32_t j = 0; j < 4; j++) xyzr[size_t(i) * 4 + j] += joggleNoise(i * 4u + j) * a + b;
+}
+
+void apo_joggle_f32(float* xyzr, uint64_t values, double range) {
+    if (!xyzr) return;
+    const double a = 2.0 * range, b = -range;
+    const uint64_t count = values / 4;
+    for (uint32_t i = 0; i < count; i++)
+        for (uint32_t j = 0; j < 4; j++) {
+            float& v = xyzr[size_t(i) * 4 + j];
+            v = float(double(v) + (joggleNoise(i * 4u + j) * a + b));
+        }
 }
 
 apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
@@ -607,7 +664,7 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     else apo_default_config(&ctx->cfg);
     const uint32_t k = ctx->cfg.k, fk = ctx->cfg.proposal_fallback_k, mk = ctx->cfg.proposal_max_k;
     auto listK = [](uint32_t v) { return v == 16 || v == 24 || v == 32 || v == 48 || v == 64; };
-    if (!listK(k) || ctx->cfg.max_iterations > kMaxRounds || !listK(fk) || !listK(mk)) {
+    if (k == 0 || k > 64 || ctx->cfg.max_iterations > kMaxRounds || !listK(fk) || !listK(mk)) {
         delete ctx;
         return APO_ERR_INVALID;
     }
@@ -620,6 +677,8 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     for (int s = 0; s < 8 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
     for (uint32_t r = 0; r < kTimedRounds && e == hipSuccess; r++)
         for (int q = 0; q < 3 && e == hipSuccess; q++) e = hipEventCreate(&ctx->vt.ev[r][q]);
+    for (uint32_t p = 0; p < kTimedPasses && e == hipSuccess; p++)
+        for (int q = 0; q < 2 && e == hipSuccess; q++) e = hipEventCreate(&ctx->passEv[p][q]);
     if (e != hipSuccess) {
         apo_destroy(ctx);
         return e == hipErrorOutOfMemory ? APO_ERR_OUT_OF_MEMORY : APO_ERR_HIP;
@@ -646,6 +705,9 @@ void apo_destroy(apo_ctx* ctx) {
         if (e) (void)hipEventDestroy(e);
     for (auto& round : ctx->vt.ev)
         for (auto& e : round)
+            if (e) (void)hipEventDestroy(e);
+    for (auto& pass : ctx->passEv)
+        for (auto& e : pass)
             if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -680,6 +742,8 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     ctx->built = false;
     if (ctx->givenN && ctx->givenN != n)
         return fail(ctx, APO_ERR_INVALID, "proposal classes were given for another site count");
+    if (learnedPath(ctx) && ctx->cfg.knn_passes != 1u)
+        return fail(ctx, APO_ERR_INVALID, "the learned proposal runs one kNN pass (knn_passes = 1)");
     ctx->cavValid = false;
     // per-wave device regions of the vertex and edge validation grids (the
     // two kernels run one after the other on the stream and share them)
@@ -869,34 +933,32 @@ apo_status apo_check_empty_spheres(apo_ctx* ctx, const double* xyzw, const uint3
                                    uint64_t* violations) {
     if (!ctx || !violations || (xyzw == nullptr) != (ids == nullptr)) return APO_ERR_INVALID;
     if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    APO_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const D4* dx = ctx->outXyzw;
     const uint4* di = ctx->outIds;
-    D4* ux = nullptr;
-    uint4* ui = nullptr;
+    DeviceTemp<D4> ux;
+    DeviceTemp<uint4> ui;
     if (xyzw) {
-        APO_HIP(hipMalloc(reinterpret_cast<void**>(&ux), (count + 1) * sizeof(D4)));
-        APO_HIP(hipMalloc(reinterpret_cast<void**>(&ui), (count + 1) * sizeof(uint4)));
-        APO_HIP(hipMemcpy(ux, xyzw, count * sizeof(D4), hipMemcpyHostToDevice));
-        APO_HIP(hipMemcpy(ui, ids, count * sizeof(uint4), hipMemcpyHostToDevice));
-        dx = ux;
-        di = ui;
+        APO_HIP(ux.alloc(count));
+        APO_HIP(ui.alloc(count));
+        APO_HIP(hipMemcpyAsync(ux.p, xyzw, count * sizeof(D4), hipMemcpyHostToDevice, st));
+        APO_HIP(hipMemcpyAsync(ui.p, ids, count * sizeof(uint4), hipMemcpyHostToDevice, st));
+        dx = ux.p;
+        di = ui.p;
     } else {
         count = ctx->vertexCount;
     }
-    unsigned long long* dv = nullptr;
-    APO_HIP(hipMalloc(reinterpret_cast<void**>(&dv), sizeof(unsigned long long)));
-    APO_HIP(hipMemsetAsync(dv, 0, sizeof(unsigned long long), st));
+    DeviceTemp<unsigned long long> dv;
+    APO_HIP(dv.alloc(1));
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(dv.p), 2u, 0u);
     hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
-    APO_HIP(checkEmptyLaunch(cellCtx(ctx), ctx->idxA, ctx->cellCount, dx, di, count, tol, dv, st));
+    APO_HIP(checkEmptyLaunch(cellCtx(ctx), ctx->idxA, ctx->cellCount, dx, di, count, tol, dv.p, st));
     unsigned long long bad = 0;
     uint32_t guard = 0;
-    APO_HIP(hipMemcpyAsync(&bad, dv, sizeof(bad), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(&bad, dv.p, sizeof(bad), hipMemcpyDeviceToHost, st));
     APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(guard), hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
-    APO_HIP(hipFree(dv));
-    if (ux) APO_HIP(hipFree(ux));
-    if (ui) APO_HIP(hipFree(ui));
     if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped in the empty-sphere check: " + guardText(guard));
     *violations = bad;
     return APO_OK;
@@ -909,7 +971,7 @@ apo_status precisionOnDevice(apo_ctx* ctx, const D4* dx, const uint4* di, uint64
     hipStream_t st = ctx->stream;
     DeviceTemp<unsigned long long> dc;
     APO_HIP(dc.alloc(2));
-    APO_HIP(hipMemsetAsync(dc.p, 0, 2 * sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(dc.p), 4u, 0u);
     hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, ctx->err, 1u, 0u);
     APO_HIP(precisionLaunch(cellCtx(ctx), ctx->idxA, ctx->cellCount, dx, di, count, arity, dc.p, st));
     unsigned long long h[2] = {0, 0};

@@ -165,13 +165,20 @@ __device__ __forceinline__ void countBytes(const CellCtx& c, uint32_t family, ui
     atomicAdd(&c.bytes[family * kByteSlots + (blockIdx.x & (kByteSlots - 1u))], static_cast<unsigned long long>(b));
 }
 
-hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st);
+// One kNN pass (k in 1..64 neighbours per site); pass > 0 continues the
+// lists of the previous pass in place (knnValidationStep, algorithm.inl:43-60).
+hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st, uint32_t pass = 0u);
+// No kNN pass: empty lists (n * k words), status in validation, flags clear.
+hipError_t cellsResetLaunch(const CellCtx& c, uint32_t* knn, uint32_t k, hipStream_t st);
 hipError_t cellsOccupancy(uint32_t* blocksPerCu);
 // Persistent cell pass over all sites; queue = kXcdQueueWords heads.
 // cls (may be null): learned-proposal classes in sorted order; each cell then
-// runs min(proposalK(class, fallbackK), k) neighbours.
+// runs min(proposalK(class, fallbackK), k) neighbours.  resume: the cells
+// still in validation continue from their stored records (a later kNN pass)
+// instead of the octahedron, and the others are left untouched.
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
-                       hipStream_t st, const uint8_t* cls = nullptr, uint32_t fallbackK = 16u);
+                       hipStream_t st, const uint8_t* cls = nullptr, uint32_t fallbackK = 16u,
+                       bool resume = false);
 
 // Device counters of a build (words of one buffer): work-queue heads of the
 // persistent kernels (8 per queue, one range per XCD), the task count of

@@ -444,6 +444,22 @@ __device__ __forceinline__ void initCell(CellLds& L, WaveCell& w, uint32_t n) {
     waveSync();
 }
 
+// The stored records of cell i (its vertices too when withVertices).
+__device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
+    const uint32_t lane = laneId();
+    const uint32_t E = c.enb[i];
+    const uint32_t V = withVertices ? c.vnb[i] : 0u;
+    const uint4* ge = c.edg + size_t(i) * kEdgeCap;
+    const uint4* gv = c.vtx + size_t(i) * kVertexCap;
+    for (uint32_t e = lane; e < E; e += 64) L.edg[e] = ge[e];
+    for (uint32_t v = lane; v < V; v += 64) L.vtx[v] = gv[v];
+    w.V = V;
+    w.E = E;
+    w.F = 0;
+    w.flags = 0;
+    waveSync();
+}
+
 __device__ __forceinline__ void storeCell(const CellCtx& c, const CellLds& L, const WaveCell& w, uint32_t i) {
     const uint32_t lane = laneId();
     uint4* gv = c.vtx + size_t(i) * kVertexCap;

@@ -23,14 +23,22 @@ namespace apo_hip {
 
 // getCells (algorithm.inl:101-206) for cell i: K updates in kNN order (the
 // first `use` of the K-long list), then the security-radius certification.
+// The first pass starts from the octahedron; a resumed pass (a later kNN
+// pass) continues the stored cell and leaves certified and buried cells as
+// they are.
 __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uint32_t* __restrict__ knn, uint32_t K,
-                                         uint32_t use, uint32_t i) {
+                                         uint32_t use, uint32_t i, bool resume) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
     WaveCell w;
-    initCell(L, w, artificialBase(c, cellMolecule(c, i)));
     const uint8_t st = c.status[i];
     const bool active = !(st == kStatusCertified || st == kStatusBuried || K == 0);
+    if (resume) {
+        if (!active) return;
+        loadCell(c, L, w, i, true);
+    } else {
+        initCell(L, w, artificialBase(c, cellMolecule(c, i)));
+    }
     float kr = -FLT_MAX;
     bool certified = false;
     uint64_t readBytes = 1;  // status
@@ -80,7 +88,8 @@ __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uin
 // (the learned proposal), cell i runs the neighbour count of its class,
 // at most the list length K.
 __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
-                                              uint32_t* queue, const uint8_t* __restrict__ cls, uint32_t fallbackK) {
+                                              uint32_t* queue, const uint8_t* __restrict__ cls, uint32_t fallbackK,
+                                              uint32_t resume) {
     __shared__ CellLds L;
     for (;;) {
         const uint32_t i = popWork(queue, c.n);
@@ -91,9 +100,27 @@ __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restr
             use = q < kProposalClasses ? proposalK(q, fallbackK) : K;
             if (use > K) use = K;
         }
-        cellTask(c, L, knn, K, use, i);
+        cellTask(c, L, knn, K, use, i, resume != 0u);
         waveSync();
     }
+}
+
+// No kNN pass (knnValidationStep = 0, algorithm.inl:1526-1549): every list
+// empty, every cell in validation with no flags (the cell launch that
+// follows writes the octahedra and the lowest known radius).
+__global__ void k_reset_cells(CellCtx c, uint32_t* __restrict__ knn, uint32_t K) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < c.n) {
+        c.status[t] = kStatusInValidation;
+        c.flags[t] = 0u;
+    }
+    if (t < size_t(c.n) * K) knn[t] = kNone;
+}
+
+hipError_t cellsResetLaunch(const CellCtx& c, uint32_t* knn, uint32_t k, hipStream_t st) {
+    const size_t words = size_t(c.n) * (k > 0u ? k : 1u);
+    hipLaunchKernelGGL(k_reset_cells, dim3(uint32_t((words + 255u) / 256u)), dim3(256), 0, st, c, knn, k);
+    return hipGetLastError();
 }
 
 hipError_t cellsOccupancy(uint32_t* blocksPerCu) {
@@ -104,10 +131,11 @@ hipError_t cellsOccupancy(uint32_t* blocksPerCu) {
 }
 
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
-                       hipStream_t st, const uint8_t* cls, uint32_t fallbackK) {
+                       hipStream_t st, const uint8_t* cls, uint32_t fallbackK, bool resume) {
     if (k > 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, queue, kXcdQueueWords, 0u);
-    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue, cls, fallbackK);
+    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue, cls, fallbackK,
+                       resume ? 1u : 0u);
     return hipGetLastError();
 }
 

@@ -16,10 +16,13 @@
 
 namespace apo_hip {
 
-template <uint32_t K>
-__global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ knn) {
-    __shared__ uint32_t hKey[K][64];
-    __shared__ float hVal[K][64];
+// One pass of getKnns for 64 sites.  KT bounds the list length k in LDS;
+// pass > 0 continues each list after the previous pass's last neighbour
+// (algorithm.inl:43-60): the lists are read and rewritten in place.
+template <uint32_t KT>
+__global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ knn, uint32_t K, uint32_t pass) {
+    __shared__ uint32_t hKey[KT][64];
+    __shared__ float hVal[KT][64];
     __shared__ uint32_t stNode[kStackDepth][64];
     __shared__ float stDist[kStackDepth][64];
 
@@ -27,16 +30,22 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
     const uint32_t base = blockIdx.x * 64u;
     const uint32_t i = base + lane;
     const uint32_t n = c.n;
-    const bool active = i < n;
+    bool active = i < n;
 
     for (uint32_t k = 0; k < K; k++) {
         hKey[k][lane] = kNone;
         hVal[k][lane] = FLT_MAX;
     }
     uint32_t overflow = 0;
+    float startingDistance = -FLT_MAX;
+    if (active && pass > 0) {
+        // a list the previous pass did not fill ends here (0xffffffff x K)
+        const uint32_t last = knn[size_t(i) * K + K - 1u];
+        if (last < n) startingDistance = sphereDistF(c.sorted[last], c.sorted[i]);
+        else active = false;
+    }
     if (active) {
         const F4 s = c.sorted[i];
-        const float startingDistance = -FLT_MAX;
         uint32_t sp = 0;
         uint32_t node = rootNode(c, cellMolecule(c, i));
         const uint32_t internal = c.internal, nodes = nodeCount(c);
@@ -138,19 +147,23 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
                 p = largest;
             }
         }
-        // buried test against the nearest site (algorithm.inl:85-98)
-        const uint32_t j0 = hKey[0][lane];
-        uint8_t status = kStatusInValidation;
-        if (j0 != kNone) {
-            const F4 cl = c.sorted[j0];
-            const double ri = s.w, rj = cl.w;
-            const double bigger = __builtin_fmax(ri, rj), smaller = __builtin_fmin(ri, rj);
-            const D3 dc{double(s.x) - double(cl.x), double(s.y) - double(cl.y), double(s.z) - double(cl.z)};
-            if (ltD(len(dc), bigger - smaller) && !ltD(rj, ri)) status = kStatusBuried;
+        if (pass > 0) {
+            if (overflow) c.flags[i] |= kFlagStackOverflow;
+        } else {
+            // buried test against the nearest site, first pass only (algorithm.inl:85-98)
+            const uint32_t j0 = hKey[0][lane];
+            uint8_t status = kStatusInValidation;
+            if (j0 != kNone) {
+                const F4 cl = c.sorted[j0];
+                const double ri = s.w, rj = cl.w;
+                const double bigger = __builtin_fmax(ri, rj), smaller = __builtin_fmin(ri, rj);
+                const D3 dc{double(s.x) - double(cl.x), double(s.y) - double(cl.y), double(s.z) - double(cl.z)};
+                if (ltD(len(dc), bigger - smaller) && !ltD(rj, ri)) status = kStatusBuried;
+            }
+            // the kNN pass is the first writer of status and flags
+            c.status[i] = status;
+            c.flags[i] = overflow ? kFlagStackOverflow : 0u;
         }
-        // the kNN pass is the first writer of status and flags
-        c.status[i] = status;
-        c.flags[i] = overflow ? kFlagStackOverflow : 0u;
     }
     waveSync();
     // coalesced write of the 64 x K block
@@ -161,16 +174,14 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
     }
 }
 
-hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st) {
+hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st, uint32_t pass) {
     const uint32_t blocks = (c.n + 63u) / 64u;
-    switch (k) {
-    case 16: hipLaunchKernelGGL(k_knn<16>, dim3(blocks), dim3(64), 0, st, c, knn); break;
-    case 24: hipLaunchKernelGGL(k_knn<24>, dim3(blocks), dim3(64), 0, st, c, knn); break;
-    case 32: hipLaunchKernelGGL(k_knn<32>, dim3(blocks), dim3(64), 0, st, c, knn); break;
-    case 48: hipLaunchKernelGGL(k_knn<48>, dim3(blocks), dim3(64), 0, st, c, knn); break;
-    case 64: hipLaunchKernelGGL(k_knn<64>, dim3(blocks), dim3(64), 0, st, c, knn); break;
-    default: return hipErrorInvalidValue;
-    }
+    if (k == 0 || k > 64) return hipErrorInvalidValue;
+    // the smallest LDS list that holds k entries
+    if (k <= 8) hipLaunchKernelGGL(k_knn<8>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
+    else if (k <= 16) hipLaunchKernelGGL(k_knn<16>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
+    else if (k <= 32) hipLaunchKernelGGL(k_knn<32>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
+    else hipLaunchKernelGGL(k_knn<64>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
     return hipGetLastError();
 }
 

@@ -107,21 +107,6 @@ __device__ __forceinline__ void popNode(const VStack& st, Trav& t) {
     }
 }
 
-__device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
-    const uint32_t lane = laneId();
-    const uint32_t E = c.enb[i];
-    const uint32_t V = withVertices ? c.vnb[i] : 0u;
-    const uint4* ge = c.edg + size_t(i) * kEdgeCap;
-    const uint4* gv = c.vtx + size_t(i) * kVertexCap;
-    for (uint32_t e = lane; e < E; e += 64) L.edg[e] = ge[e];
-    for (uint32_t v = lane; v < V; v += 64) L.vtx[v] = gv[v];
-    w.V = V;
-    w.E = E;
-    w.F = 0;
-    w.flags = 0;
-    waveSync();
-}
-
 // j is an endpoint of a current edge (wave-wide; algorithm.inl:368-376)
 __device__ __forceinline__ bool knownSite(const CellLds& L, const WaveCell& w, uint32_t j) {
     const uint32_t lane = laneId();

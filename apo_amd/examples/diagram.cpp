@@ -30,7 +30,7 @@ int main(int argc, char** argv) {
     while ((got = std::fread(buf, sizeof(float), 4096, in)) > 0) sites.insert(sites.end(), buf, buf + got);
     std::fclose(in);
     try {
-        apo::gpu::AlgorithmGPU algo({sites.data(), sites.size()});
+        apo::gpu::AlgorithmGPU<> algo(apo::ConstSpan<float>{sites.data(), sites.size()});
         algo.build();
         const apo::gpu::FullDiagram d = algo.toHostFull(true);
         const uint64_t v = d.vertices.size() / 4;
@@ -61,7 +61,7 @@ int main(int argc, char** argv) {
                     sites.size() / 4, static_cast<unsigned long long>(v),
                     static_cast<unsigned long long>(e), algo.sample.total,
                     algo.sample.bvhConstruction, algo.sample.knnSearch, algo.sample.knnConstruction,
-                    algo.sample.vertexEdgeValidation + algo.sample.bisectorValidation);
+                    algo.sample.vertexValidation + algo.sample.edgeValidation + algo.sample.bisectorValidation);
     } catch (const apo::gpu::Error& e) {
         std::fprintf(stderr, "apo error %d: %s\n", int(e.status), e.what());
         return 1;

@@ -33,7 +33,8 @@ typedef struct apo_ctx apo_ctx;
  * AlgorithmGPU<16>(sites, knnValidationStep = 1)
  * (apo/include/apo/gpu/algorithm.cuh:12-16, algorithm.inl:1404-1594). */
 typedef struct apo_config {
-    uint32_t k;               /* neighbours per cell (16, 24, 32, 48 or 64)           */
+    uint32_t k;               /* neighbours per kNN pass, the template argument K of AlgorithmGPU<K>
+                                 (algorithm.cuh:12): 1..64, default 16                 */
     uint32_t max_iterations;  /* vertex + edge validation rounds (10, algorithm.inl:1554) */
     uint32_t max_restarts;    /* restarts per cell per validation kernel (100, :272)  */
     uint32_t stage_mask;      /* bit0 cells, bit1 vertex/edge validation, bit2 bisector,
@@ -59,6 +60,12 @@ typedef struct apo_config {
     uint32_t proposal_max_k;  /* learned proposal: the most neighbours any other class may run (16, 24, 32, 48
                                  or 64; default 64).  Classes asking for more are not chosen; the kNN list
                                  holds max(proposal_max_k, proposal_fallback_k) neighbours */
+    uint32_t knn_passes;      /* kNN + cell passes: the knnValidationStep argument of AlgorithmGPU<K>
+                                 (algorithm.cuh:16, algorithm.inl:1535-1549).  Pass s > 0 continues each
+                                 site's list after the previous pass's k-th neighbour (a list the previous
+                                 pass did not fill ends) and updates the cells still in validation; 0 runs
+                                 no kNN or cell pass (every cell goes to validation as the octahedron).
+                                 Default 1; the learned proposal needs 1 */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,
@@ -98,6 +105,14 @@ typedef struct apo_stats {
 } apo_stats;
 
 void apo_default_config(apo_config* cfg);
+
+/* apo::joggle (apo/src/core/utils.cpp:14-25, core/utils.hpp:9): adds
+ * pcg(4i + j) * 2 range - range to value j of site i, pcg(v) being the
+ * shadertoy PCG hash scaled by 2^-32 (utils.cpp:6-12); values = 4n.  Host
+ * only, no device needed.  The _f32 variant takes FP32 sites: the noise is
+ * added in FP64 and each value rounded to FP32 once. */
+void apo_joggle(double* xyzr, uint64_t values, double range);
+void apo_joggle_f32(float* xyzr, uint64_t values, double range);
 
 /* Replaces the AlgorithmGPU<K> constructor (algorithm.inl:1404-1408). */
 apo_status apo_create(apo_ctx** ctx, const apo_config* cfg);

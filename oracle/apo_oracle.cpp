@@ -1246,13 +1246,16 @@ uint32_t bisectorSearch(const Context& C, const Cell& cell, uint32_t i, const D4
 // ---------------------------------------------------------------------------
 // Pipeline stages (algorithm.inl:101-1233)
 // ---------------------------------------------------------------------------
-void stageCells(Context& C, const std::vector<uint32_t>& knn, uint32_t K) {
+// getCells.  The first pass starts every cell from the octahedron (the
+// state initializeDataStructure leaves, topology_update.inl:5-47); later
+// passes continue the stored cells that are still in validation.
+void stageCells(Context& C, const std::vector<uint32_t>& knn, uint32_t K, bool resume) {
     const int64_t n = C.n;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t ii = 0; ii < n; ii++) {
         const uint32_t i = uint32_t(ii);
         Cell& cell = C.cells[i];
-        initCell(C, cell);
+        if (!resume) initCell(C, cell);
         if (C.status[i] == kCertified || C.status[i] == kBuried) continue;
         const D4 si = C.S[i];
         double knownRadius = -DBL_MAX;
@@ -1667,6 +1670,7 @@ void orc_default_config(orc_config* cfg) {
     cfg->stage_mask = 7;
     cfg->edge_bound = 0;
     cfg->fallback_k = 16;
+    cfg->knn_passes = 1;
 }
 
 static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {
@@ -1708,18 +1712,34 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
         C.cfg.k = 64;
     }
     const uint32_t K = cfg.k;
+    // the learned path runs one pass of its own lists
+    const uint32_t passes = classes ? 1u : cfg.knn_passes;
 
-    // getKnns, iteration 0 (algorithm.inl:32-99)
+    // getKnns + getCells, knnValidationStep times (algorithm.inl:32-206,
+    // 1535-1549).  The lists start as 0xffffffff (the memset of :1526).
     d->knn.assign(size_t(n) * K, kNone);
-    {
-        std::vector<float> dist(size_t(n) * K);
+    C.buried.assign(n, 0);
+    std::vector<float> dist(size_t(n) * K);
+    for (uint32_t pass = 0; pass < passes; pass++) {
 #pragma omp parallel for schedule(dynamic, 256)
         for (int64_t ii = 0; ii < int64_t(n); ii++) {
             const uint32_t i = uint32_t(ii);
             uint32_t* keys = &d->knn[size_t(i) * K];
-            const int depth = knnQuery(d->bvh, C.Sf, i, K, -FLT_MAX, keys, &dist[size_t(i) * K]);
+            float startingDistance = -FLT_MAX;
+            if (pass > 0) {
+                // continue after the previous pass's last neighbour, or end the
+                // list when that pass did not fill it (algorithm.inl:46-60)
+                const uint32_t last = keys[K - 1];
+                if (last >= n) {
+                    for (uint32_t kk = 0; kk < K; kk++) keys[kk] = kNone;
+                    continue;
+                }
+                startingDistance = sphereDistF(C.Sf[last], C.Sf[i]);
+            }
+            const int depth = knnQuery(d->bvh, C.Sf, i, K, startingDistance, keys, &dist[size_t(i) * K]);
             if (depth > 32) C.flags[i] |= kFlagStackOverflow;
-            if (keys[0] == kNone) continue;
+            if (pass > 0 || keys[0] == kNone) continue;
+            // buried test against the nearest site, first pass only (:85-98)
             const F4 si = C.Sf[i];
             const D4 closest = C.S[keys[0]];
             const double ri = si.w, rj = closest.w;
@@ -1730,17 +1750,19 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
                 C.status[i] = kBuried;
             }
         }
+        if (pass == 0) {
+            uint64_t buried = 0;
+            for (uint32_t i = 0; i < n; i++) {
+                C.buried[i] = C.status[i] == kBuried;
+                buried += C.buried[i];
+            }
+            d->stats[5] = buried;
+        }
+        if (cfg.stage_mask & 1u) stageCells(C, d->knn, K, pass > 0);
+        else if (pass == 0)
+            for (uint32_t i = 0; i < n; i++) initCell(C, C.cells[i]);
     }
-    C.buried.assign(n, 0);
-    uint64_t buried = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        C.buried[i] = C.status[i] == kBuried;
-        buried += C.buried[i];
-    }
-    d->stats[5] = buried;
-
-    if (cfg.stage_mask & 1u) stageCells(C, d->knn, K);
-    else
+    if (passes == 0)
         for (uint32_t i = 0; i < n; i++) initCell(C, C.cells[i]);
 
     uint64_t uncertified = 0;

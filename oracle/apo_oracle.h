@@ -78,6 +78,9 @@ typedef struct orc_config {
                                    (the reference's search), 1 when the search sphere exceeds 2x the known
                                    radius, 2 always; results are identical in every mode */
     uint32_t fallback_k;        /* learned proposal: neighbours of the last class (not certified within 64), 16 */
+    uint32_t knn_passes;        /* kNN + cell passes, the reference's knnValidationStep (algorithm.cuh:16,
+                                   algorithm.inl:1535-1549): pass s > 0 continues each list after the previous
+                                   pass's last neighbour; 0 runs none (cells stay the octahedron).  Default 1 */
 } orc_config;
 
 void orc_default_config(orc_config* cfg);

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from apo_amd import _lib
@@ -14,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "apo_c.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(apo_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(apo_[a-z_0-9]+)\s*\(", text)))
 
 
 def test_header_and_binding_agree():
@@ -35,7 +36,34 @@ def test_default_config_matches_reference_defaults():
     cfg = _lib.Config()
     _lib.lib().apo_default_config(ctypes.byref(cfg))
     # AlgorithmGPU<16>(sites, knnValidationStep = 1), 10 validation rounds, <100 restarts
-    assert (cfg.k, cfg.max_iterations, cfg.max_restarts, cfg.stage_mask) == (16, 10, 100, 7)
+    assert (cfg.k, cfg.knn_passes, cfg.max_iterations, cfg.max_restarts, cfg.stage_mask) == (16, 1, 10, 100, 7)
+
+
+def test_config_struct_mirrors_header():
+    text = open(os.path.join(ROOT, "include", "apo_c.h")).read()
+    body = re.search(r"typedef struct apo_config \{(.*?)\} apo_config;", text, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"(uint32_t|float)\s+([a-z_0-9]+);", body)
+    want = [(name, {"uint32_t": "c_uint", "float": "c_float"}[ty]) for ty, name in fields]
+    got = [(name, t.__name__) for name, t in _lib.Config._fields_]
+    assert want == got
+
+
+def test_joggle_matches_the_python_restatement():
+    # apo::joggle (utils.cpp:14-25) in the C ABI, FP64 and FP32 sites, against
+    # apo_amd.molecules.joggle (itself pinned to the oracle's pcg)
+    if not os.path.exists(_lib.library_path()):
+        pytest.skip("libapo_hip.so not built")
+    from apo_amd import datasets, molecules
+    s = datasets.uniform_sites(500, seed=4)
+    f = s.copy()
+    _lib.lib().apo_joggle_f32(_lib.fptr(f), f.size, 1e-3)
+    assert np.array_equal(f, molecules.joggle(s, 1e-3))
+    d = s.astype(np.float64)
+    _lib.lib().apo_joggle(_lib.dptr(d), d.size, 1e-3)
+    v = np.arange(s.size, dtype=np.uint32)
+    noise = np.ldexp(molecules.pcg_word(v).astype(np.float64), -32) * 2e-3 - 1e-3
+    assert np.array_equal(d.ravel(), s.astype(np.float64).ravel() + noise)
 
 
 def test_stats_struct_mirrors_header():

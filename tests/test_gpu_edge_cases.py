@@ -126,5 +126,13 @@ def test_invalid_arguments_fail_with_a_status():
         gpu.close()
     with pytest.raises(ApoError):
         AlgorithmGPU(_rand(50, 2), classes=np.full(50, 9, np.uint8))
-    with pytest.raises(ApoError):
-        AlgorithmGPU(_rand(50, 2), k=17)
+    for k in (0, 65):
+        with pytest.raises(ApoError):
+            AlgorithmGPU(_rand(50, 2), k=k)
+    # the learned proposal runs one kNN pass
+    gpu = AlgorithmGPU(_rand(50, 2), classes=np.zeros(50, np.uint8), knn_validation_step=2)
+    try:
+        with pytest.raises(ApoError):
+            gpu.build()
+    finally:
+        gpu.close()

@@ -122,21 +122,25 @@ def test_empty_sphere(built):
 
 
 def test_checker_counts_planted_violations(built):
-    # move a third of the vertex centres: the tangent-site radius is
-    # recomputed from the moved centre, so other sites fall inside; the BVH
-    # checker must report exactly the brute-force count of the oracle
+    # move vertex centres: the tangent-site radius is recomputed from the
+    # moved centre, so other sites fall inside; the BVH checker must report
+    # exactly the brute-force count of the oracle (up to 10k sites) and the
+    # independent KD-tree count (every size, on at most 20k moved vertices)
     name, sites, gpu, orc = built
-    if len(sites) > 10_000:
-        pytest.skip("brute-force reference limited to small sets")
     g = gpu.to_host()
-    x = g.vertices.copy()
     rng = np.random.default_rng(7)
-    pick = rng.random(len(x)) < 0.33
-    x[pick, :3] += rng.uniform(-0.03, 0.03, (pick.sum(), 3))
-    want = oracle.empty_sphere_violations(sites, x, g.vertices_id)
-    got = gpu.empty_sphere_violations(x, g.vertices_id)
+    pick = rng.random(len(g.vertices)) < min(0.33, 20_000 / len(g.vertices))
+    x = g.vertices[pick].copy()
+    ids = np.ascontiguousarray(g.vertices_id[pick])
+    scale = float(np.ptp(sites[:, :3], axis=0).max())
+    x[:, :3] += rng.uniform(-0.03, 0.03, (len(x), 3)) * scale
+    got = gpu.empty_sphere_violations(x, ids)
+    want = _violations_kdtree(sites, x, ids)
+    print(f"[planted] {name}: {got} of {len(x)} moved vertices", flush=True)
     assert want > 0
     assert got == want, (name, got, want)
+    if len(sites) <= 10_000:
+        assert oracle.empty_sphere_violations(sites, x, ids) == want, name
 
 
 def test_topology_sets_exact(built):

@@ -74,6 +74,7 @@ SIGNATURES = {
     "apo_build_batched": (ctypes.c_int, [_vp, _f32p, _u32p, _u32]),
     "apo_build_batched_device": (ctypes.c_int, [_vp, _vp, _u32p, _u32]),
     "apo_batch_vertex_offsets": (ctypes.c_int, [_vp, _u64p]),
+    "apo_build_device_static": (ctypes.c_int, [_vp, _vp, _u32, _vp, _vp, ctypes.c_uint64, _u64p]),
     "apo_launch_occupancy": (ctypes.c_int, [_vp, _u32p]),
     "apo_vertex_count": (ctypes.c_int, [_vp, _u64p]),
     "apo_get_vertices": (ctypes.c_int, [_vp, _f64p, _u32p, ctypes.c_uint64]),

@@ -14,6 +14,7 @@
 #include <string>
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -41,6 +42,38 @@ struct GraphKey {
                outCap == o.outCap && std::memcmp(&cfg, &o.cfg, sizeof(cfg)) == 0;
     }
 };
+
+}  // namespace
+
+// ---- working memory of apo_build_device_static: one array per device,
+// held until the process ends.  Host code never uses its name as an
+// address: the device address comes from hipGetSymbolAddress, once per
+// device, at apo_create.
+static __device__ __align__(256) unsigned char scratch[APO_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr uint32_t kStaticMaxSites = APO_STATIC_MAX_SITES;
+// vertex / edge validation waves of a static build (each uses its own
+// device region of the array)
+constexpr uint32_t kStaticWaveRegions = 1024;
+constexpr int kStaticMaxDevices = 64;
+std::mutex gScratchLock;
+void* gScratchAddr[kStaticMaxDevices] = {};
+
+// Device address of this device's copy of `scratch`, with the device current.
+hipError_t staticScratchAddress(int device, void** out) {
+    if (device < 0 || device >= kStaticMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(gScratchLock);
+    if (!gScratchAddr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        gScratchAddr[device] = p;
+    }
+    *out = gScratchAddr[device];
+    return hipSuccess;
+}
 
 }  // namespace
 
@@ -128,6 +161,7 @@ struct apo_ctx {
     float* cavRatios = nullptr;
     uint64_t cavCap = 0;
     bool cavValid = false;
+    void* staticScratch = nullptr;  // this device's copy of the static scratch array (apo_build_device_static)
     hipEvent_t ev[8] = {};
     hipEvent_t passEv[kTimedPasses][2] = {};  // around the kNN launch of passes 1..kTimedPasses
     ValidationTiming vt{};
@@ -401,8 +435,9 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
                              reinterpret_cast<int32_t*>(ctx->cellCount), reinterpret_cast<int2*>(ctx->blockSums),
                              ctx->counters + kTreeHeight, st));
     const ValidationBuffers vb{ctx->tasks, ctx->counters, ctx->spill, ctx->spillBlocks};
-    const ValidationGrid vg{persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
-                            persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge),
+    // vertex and edge validation waves each own a device region: at most spillBlocks of them
+    const ValidationGrid vg{std::min(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex), ctx->spillBlocks),
+                            std::min(persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge), ctx->spillBlocks),
                             persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector)};
     if (ctx->cfg.stage_mask & 2u)
         APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st,
@@ -672,6 +707,7 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    if (e == hipSuccess) e = staticScratchAddress(ctx->device, &ctx->staticScratch);
     if (e == hipSuccess) e = cellsOccupancy(&ctx->occCells);
     if (e == hipSuccess) e = validationOccupancy(&ctx->occVertex, &ctx->occEdge, &ctx->occBisector);
     for (int s = 0; s < 8 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
@@ -801,6 +837,162 @@ apo_status apo_build_batched(apo_ctx* ctx, const float* xyzr, const uint32_t* of
     if (s != APO_OK) return s;
     APO_HIP(hipMemcpyAsync(ctx->sitesIn, xyzr, size_t(n) * sizeof(F4), hipMemcpyHostToDevice, ctx->stream));
     return runPipeline(ctx, ctx->sitesIn, n);
+}
+
+}  // extern "C"
+
+namespace {
+
+// The device buffers of a build, by name: apo_build_device_static points
+// them into the static scratch array for one call and restores the
+// context's own afterwards.
+#define APO_BUILD_BUFFERS(X)                                                                                     \
+    X(scene) X(boxPartials) X(boxTicket) X(sortScratch) X(range) X(startLists) X(codesA) X(codesB) X(idxA) X(idxB) \
+    X(sorted) X(nodes) X(parent) X(sizes) X(refitCounter) X(rootBox) X(status) X(knownRadii) X(flags) X(vtx)       \
+    X(vnb) X(edg) X(enb) X(knn) X(counters) X(spill) X(err) X(tasks) X(cellCount) X(blockSums) X(total)            \
+    X(outXyzw) X(outIds) X(bytes)
+
+struct BuildBuffers {
+#define APO_DECLARE(f) decltype(apo_ctx::f) f;
+    APO_BUILD_BUFFERS(APO_DECLARE)
+#undef APO_DECLARE
+    uint32_t spillBlocks;
+    uint64_t outCap;
+};
+
+BuildBuffers takeBuffers(const apo_ctx* ctx) {
+    BuildBuffers b;
+#define APO_TAKE(f) b.f = ctx->f;
+    APO_BUILD_BUFFERS(APO_TAKE)
+#undef APO_TAKE
+    b.spillBlocks = ctx->spillBlocks;
+    b.outCap = ctx->outCap;
+    return b;
+}
+
+void putBuffers(apo_ctx* ctx, const BuildBuffers& b) {
+#define APO_PUT(f) ctx->f = b.f;
+    APO_BUILD_BUFFERS(APO_PUT)
+#undef APO_PUT
+    ctx->spillBlocks = b.spillBlocks;
+    ctx->outCap = b.outCap;
+}
+
+// Consecutive 256-byte aligned pieces of the static array; `fits` turns
+// false when the pieces overrun it.
+struct Carve {
+    char* base;
+    size_t used = 0;
+    bool fits = true;
+    template <typename T>
+    void take(T*& p, size_t count) {
+        used = (used + 255u) & ~size_t(255u);
+        p = reinterpret_cast<T*>(base + used);
+        used += (count ? count : 1u) * sizeof(T);
+        if (used > size_t(APO_STATIC_SCRATCH_BYTES)) fits = false;
+    }
+};
+
+// The buffers of one build of n <= kStaticMaxSites sites with K-long
+// neighbour lists in the static array (device address `base`), the vertex
+// output in the caller's buffers.
+bool carveStatic(void* base, uint32_t n, uint32_t K, double* xyzw, uint32_t* ids, uint64_t capacity,
+                 BuildBuffers& b) {
+    Carve c{static_cast<char*>(base)};
+    const size_t nodes = 2 * size_t(n) - 1;
+    c.take(b.scene, 1);
+    c.take(b.boxPartials, size_t(kBoxBlocks) * 8);
+    c.take(b.boxTicket, 1);
+    c.take(b.sortScratch, sortScratchWords(n));
+    c.take(b.range, 2 * size_t(n));
+    c.take(b.startLists, refitStartWords(n));
+    c.take(b.codesA, n);
+    c.take(b.codesB, n);
+    c.take(b.idxA, n);
+    c.take(b.idxB, n);
+    c.take(b.sorted, n);
+    c.take(b.nodes, n);
+    c.take(b.parent, nodes);
+    c.take(b.sizes, nodes);
+    c.take(b.refitCounter, n);
+    c.take(b.rootBox, 8);
+    c.take(b.status, n);
+    c.take(b.knownRadii, n);
+    c.take(b.flags, n);
+    c.take(b.vtx, size_t(n) * kVertexCap);
+    c.take(b.vnb, n);
+    c.take(b.edg, size_t(n) * kEdgeCap);
+    c.take(b.enb, n);
+    c.take(b.knn, size_t(n) * K);
+    c.take(b.counters, kCounterWords);
+    c.take(b.err, 1);
+    c.take(b.tasks, n);
+    c.take(b.cellCount, n);
+    c.take(b.blockSums, (n + 255u) / 256u + 2u);
+    c.take(b.total, 1);
+    c.take(b.bytes, size_t(kByteFamilies) * kByteSlots);
+    c.take(b.spill, size_t(kStaticWaveRegions) * waveScratchWords());
+    b.spillBlocks = kStaticWaveRegions;
+    b.outXyzw = reinterpret_cast<D4*>(xyzw);
+    b.outIds = reinterpret_cast<uint4*>(ids);
+    b.outCap = capacity;
+    return c.fits;
+}
+
+}  // namespace
+
+extern "C" {
+
+apo_status apo_build_device_static(apo_ctx* ctx, const float* xyzr_device, uint32_t n, double* xyzw_device,
+                                   uint32_t* ids_device, uint64_t capacity, uint64_t* count) {
+    if (!ctx || !xyzr_device || !count || (capacity && (!xyzw_device || !ids_device))) return APO_ERR_INVALID;
+    if (n < 2) return fail(ctx, APO_ERR_INVALID, "need at least 2 sites");
+    if (n > kStaticMaxSites)
+        return fail(ctx, APO_ERR_INVALID, "apo_build_device_static takes at most " +
+                                              std::to_string(kStaticMaxSites) + " sites (APO_STATIC_MAX_SITES)");
+    if (learnedPath(ctx)) return fail(ctx, APO_ERR_INVALID, "apo_build_device_static runs the plain path only");
+    if (ctx->cfg.stage_mask & 8u)
+        return fail(ctx, APO_ERR_INVALID, "apo_build_device_static runs the whole build (stage_mask bit 3 clear)");
+    if (!ctx->staticScratch) return fail(ctx, APO_ERR_STATE, "no static scratch address for this device");
+    APO_HIP(hipSetDevice(ctx->device));
+    BuildBuffers mine{};
+    if (!carveStatic(ctx->staticScratch, n, ctx->cfg.k, xyzw_device, ids_device, capacity, mine))
+        return fail(ctx, APO_ERR_CAPACITY, "the build does not fit the static scratch array");
+    // the context's own buffers and the state that describes them come back
+    // on every return path
+    struct Restore {
+        apo_ctx* ctx;
+        BuildBuffers own;
+        uint32_t n, molecules;
+        ~Restore() {
+            putBuffers(ctx, own);
+            ctx->n = n;
+            ctx->molecules = molecules;
+        }
+    } restore{ctx, takeBuffers(ctx), ctx->n, ctx->molecules};
+    ctx->built = false;  // the context's buffers no longer hold a complete build
+    ctx->cavValid = false;
+    putBuffers(ctx, mine);
+    ctx->n = n;
+    ctx->molecules = 1;
+    hipStream_t st = ctx->stream;
+    // the one word no kernel of the build clears
+    hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, ctx->boxTicket, 1u, 0u);
+    APO_HIP(hipGetLastError());
+    APO_HIP(hipEventRecord(ctx->ev[0], st));
+    const apo_status s = enqueueBuild(ctx, reinterpret_cast<const F4*>(xyzr_device), n, false, false);
+    if (s != APO_OK) return s;
+    APO_HIP(hipEventRecord(ctx->ev[6], st));
+    uint32_t total = 0, guard = 0;
+    APO_HIP(hipMemcpyAsync(&total, ctx->total, sizeof(total), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(&guard, ctx->err, sizeof(guard), hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    if (guard) return fail(ctx, APO_ERR_HIP, "device guard tripped: " + guardText(guard));
+    *count = total;
+    if (total > capacity)
+        return fail(ctx, APO_ERR_CAPACITY, "output buffers hold " + std::to_string(capacity) + " of " +
+                                                std::to_string(total) + " vertices");
+    return APO_OK;
 }
 
 apo_status apo_launch_occupancy(apo_ctx* ctx, uint32_t* out5) {

@@ -141,6 +141,25 @@ apo_status apo_build_device(apo_ctx* ctx, const float* xyzr_device, uint32_t n);
 apo_status apo_build_batched(apo_ctx* ctx, const float* xyzr, const uint32_t* offsets, uint32_t molecules);
 apo_status apo_build_batched_device(apo_ctx* ctx, const float* xyzr_device, const uint32_t* offsets,
                                     uint32_t molecules);
+/* ---- a build that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * apo_build_device + apo_get_device_vertices for one molecule of 2 to
+ * APO_STATIC_MAX_SITES sites on the plain path (any k and knn_passes of the
+ * context's config), with every working buffer in one static device array of
+ * APO_STATIC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends.  xyzr_device: n float4 sites in device memory;
+ * xyzw_device / ids_device: device buffers for `capacity` vertices (FP64
+ * x, y, z, w and 4 original site ids); *count receives the vertex count.  A
+ * count above capacity returns APO_ERR_CAPACITY with the first `capacity`
+ * vertices written.  Synchronous on the context's stream.  Calls that
+ * overlap on one GPU (from any context) race on the array; after the call
+ * the context holds no build (results are in the caller's buffers only). */
+#define APO_STATIC_MAX_SITES 4096u
+#define APO_STATIC_SCRATCH_BYTES (48u << 20)
+apo_status apo_build_device_static(apo_ctx* ctx, const float* xyzr_device, uint32_t n, double* xyzw_device,
+                                   uint32_t* ids_device, uint64_t capacity, uint64_t* count);
+
 /* First output vertex of each molecule of the last build (molecules + 1 values). */
 apo_status apo_batch_vertex_offsets(apo_ctx* ctx, uint64_t* offsets);
 

@@ -27,7 +27,7 @@ class Config(ctypes.Structure):
                 ("proposal_cost", ctypes.c_float), ("graph", ctypes.c_uint32), ("cover_ratio", ctypes.c_float),
                 ("waves_cells", ctypes.c_uint32), ("waves_vertex", ctypes.c_uint32), ("waves_edge", ctypes.c_uint32),
                 ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32),
-                ("proposal_max_k", ctypes.c_uint32), ("knn_passes", ctypes.c_uint32)]
+                ("proposal_max_k", ctypes.c_uint32), ("knn_passes", ctypes.c_uint32), ("complete", ctypes.c_uint32)]
 
 
 class Precision(ctypes.Structure):
@@ -49,7 +49,9 @@ class Stats(ctypes.Structure):
                 ("bytes_cells", ctypes.c_uint64), ("bytes_vertex_validation", ctypes.c_uint64),
                 ("bytes_edge_validation", ctypes.c_uint64), ("bytes_bisector_validation", ctypes.c_uint64),
                 ("ms_proposal", ctypes.c_double), ("proposal", ctypes.c_uint64), ("cells_certified", ctypes.c_uint64),
-                ("reserved", ctypes.c_uint64 * 2)]
+                ("ms_completion", ctypes.c_double), ("completion_rounds", ctypes.c_uint64),
+                ("completion_updated_cells", ctypes.c_uint64), ("completion_unrepairable", ctypes.c_uint64),
+                ("completion_validation_tasks", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 2)]
 
 
 _vp = ctypes.c_void_p

@@ -17,8 +17,8 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libapo_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["lbvh.hip", "knn.hip", "cells.hip", "validation.hip", "check.hip", "topology.hip", "learned.hip",
-           "cavity.hip", "api.hip"]
+SOURCES = ["lbvh.hip", "knn.hip", "cells.hip", "validation.hip", "complete.hip", "check.hip", "topology.hip",
+           "learned.hip", "cavity.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -154,6 +154,8 @@ struct apo_ctx {
     uint8_t* cls = nullptr;     // predicted class per sorted site
     uint8_t* labels = nullptr;  // label pass output per sorted site
     uint32_t capP = 0;          // site capacity of feat / cls / labels
+    uint32_t* want = nullptr;     // completion mode: smallest wanted site per cell (kNone: none)
+    uint32_t capW = 0;
     uint8_t* givenCls = nullptr;  // classes given by the caller (input order), used instead of the MLP
     uint32_t givenN = 0;          // their site count (0: none given)
     uint32_t givenCap = 0;
@@ -162,7 +164,7 @@ struct apo_ctx {
     uint64_t cavCap = 0;
     bool cavValid = false;
     void* staticScratch = nullptr;  // this device's copy of the static scratch array (apo_build_device_static)
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};
     hipEvent_t passEv[kTimedPasses][2] = {};  // around the kNN launch of passes 1..kTimedPasses
     ValidationTiming vt{};
     apo_stats stats{};
@@ -305,6 +307,7 @@ CellCtx cellCtx(apo_ctx* ctx) {
     c.err = ctx->err;
     c.bytes = ctx->bytes;
     c.coverRatio = ctx->cfg.cover_ratio < 0.f ? __builtin_inff() : ctx->cfg.cover_ratio;
+    c.complete = ctx->cfg.complete ? 1u : 0u;
     return c;
 }
 
@@ -448,6 +451,22 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
         APO_HIP(validationLaunch(cc, vb, vg, 1, ctx->cfg.max_restarts, true, st, nullptr));
     APO_TIMED_STAGE("bisector validation");
     APO_HIP(mark(5));
+    if (ctx->cfg.complete && (ctx->cfg.stage_mask & 2u)) {
+        // completion mode: the cells the bisector stage changed (status 0) are
+        // validated again, then the consistency rounds, each followed by
+        // validation rounds over the cells it updated
+        hipLaunchKernelGGL(k_fill_words, dim3((n + 255u) / 256u), dim3(256), 0, st, ctx->want, n, kNone);
+        APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st, nullptr,
+                                 ctx->counters + kCompletionTasks));
+        const uint32_t blocks = persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells);
+        for (uint32_t r = 0; r < ctx->cfg.complete; r++) {
+            APO_HIP(consistencyLaunch(cc, ctx->want, ctx->counters, r, blocks, st));
+            APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st, nullptr,
+                                     ctx->counters + kCompletionTasks + (r + 1u) * kMaxRounds));
+        }
+    }
+    APO_TIMED_STAGE("completion");
+    APO_HIP(mark(8));
     const OutputBuffers ob{ctx->cellCount, ctx->blockSums, ctx->total, ctx->outXyzw, ctx->outIds, ctx->outCap};
     APO_HIP(outputCountLaunch(cc, ob, st));
     APO_HIP(outputWriteLaunch(cc, ob, ctx->idxA, st));
@@ -522,8 +541,12 @@ apo_status finishBuild(apo_ctx* ctx, uint32_t n, bool learned, bool timed) {
         S.ms_cells = msCellsOnly - msLaterKnn;
         S.ms_proposal = msProposal;
         S.ms_validation = ms[3];
+        float msCompletion = 0.f, msOutput = 0.f;
+        APO_HIP(hipEventElapsedTime(&msCompletion, ctx->ev[5], ctx->ev[8]));
+        APO_HIP(hipEventElapsedTime(&msOutput, ctx->ev[8], ctx->ev[6]));
         S.ms_bisector = ms[4];
-        S.ms_output = ms[5];
+        S.ms_completion = msCompletion;
+        S.ms_output = msOutput;
         for (uint32_t r = 0; r < ctx->vt.rounds; r++) {
             float a = 0.f, b = 0.f;
             APO_HIP(hipEventElapsedTime(&a, ctx->vt.ev[r][0], ctx->vt.ev[r][1]));
@@ -545,6 +568,18 @@ apo_status finishBuild(apo_ctx* ctx, uint32_t n, bool learned, bool timed) {
         S.validation_tasks += t;
     }
     S.validation_launches = uint32_t(ctx->roundTasks.size());
+    if (ctx->cfg.complete && (ctx->cfg.stage_mask & 2u)) {
+        // round r ran when r == 0 or round r - 1 updated a cell
+        for (uint32_t r = 0; r < ctx->cfg.complete; r++) {
+            if (r > 0 && counters[kCompletionUpdated + r - 1] == 0) break;
+            S.completion_rounds++;
+            S.completion_updated_cells += counters[kCompletionUpdated + r];
+        }
+        S.completion_unrepairable = counters[kCompletionStuck];
+        for (uint32_t q = 0; q <= ctx->cfg.complete; q++)
+            for (uint32_t r = 0; r < ctx->cfg.max_iterations; r++)
+                S.completion_validation_tasks += counters[kCompletionTasks + q * kMaxRounds + r];
+    }
     unsigned long long counts[kByteFamilies * kByteSlots] = {};
     APO_HIP(hipMemcpy(counts, ctx->bytes, sizeof(counts), hipMemcpyDeviceToHost));
     uint64_t fam[kByteFamilies] = {};
@@ -663,6 +698,7 @@ void apo_default_config(apo_config* cfg) {
     cfg->proposal_fallback_k = 16;
     cfg->proposal_max_k = 64;
     cfg->knn_passes = 1;
+    cfg->complete = 0;
 }
 
 // pcg hash of apo/src/core/utils.cpp:6-12, scaled by 2^-32
@@ -699,7 +735,8 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     else apo_default_config(&ctx->cfg);
     const uint32_t k = ctx->cfg.k, fk = ctx->cfg.proposal_fallback_k, mk = ctx->cfg.proposal_max_k;
     auto listK = [](uint32_t v) { return v == 16 || v == 24 || v == 32 || v == 48 || v == 64; };
-    if (k == 0 || k > 64 || ctx->cfg.max_iterations > kMaxRounds || !listK(fk) || !listK(mk)) {
+    if (k == 0 || k > 64 || ctx->cfg.max_iterations > kMaxRounds || !listK(fk) || !listK(mk) ||
+        ctx->cfg.complete > kMaxCompletionRounds) {
         delete ctx;
         return APO_ERR_INVALID;
     }
@@ -710,7 +747,7 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     if (e == hipSuccess) e = staticScratchAddress(ctx->device, &ctx->staticScratch);
     if (e == hipSuccess) e = cellsOccupancy(&ctx->occCells);
     if (e == hipSuccess) e = validationOccupancy(&ctx->occVertex, &ctx->occEdge, &ctx->occBisector);
-    for (int s = 0; s < 8 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
+    for (int s = 0; s < 10 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
     for (uint32_t r = 0; r < kTimedRounds && e == hipSuccess; r++)
         for (int q = 0; q < 3 && e == hipSuccess; q++) e = hipEventCreate(&ctx->vt.ev[r][q]);
     for (uint32_t p = 0; p < kTimedPasses && e == hipSuccess; p++)
@@ -733,7 +770,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
-                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -795,6 +832,11 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     }
     apo_status s = ensureCapacity(ctx, n, learnedPath(ctx) ? (ctx->cfg.k > 64u ? ctx->cfg.k : 64u) : ctx->cfg.k);
     if (s != APO_OK) return s;
+    if (ctx->cfg.complete && n > ctx->capW) {
+        dropGraph(ctx);
+        APO_HIP(alloc(ctx->want, n));
+        ctx->capW = n;
+    }
     if (learnedPath(ctx)) {
         s = ensureProposalBuffers(ctx, n);
         if (s != APO_OK) return s;
@@ -850,7 +892,7 @@ namespace {
     X(scene) X(boxPartials) X(boxTicket) X(sortScratch) X(range) X(startLists) X(codesA) X(codesB) X(idxA) X(idxB) \
     X(sorted) X(nodes) X(parent) X(sizes) X(refitCounter) X(rootBox) X(status) X(knownRadii) X(flags) X(vtx)       \
     X(vnb) X(edg) X(enb) X(knn) X(counters) X(spill) X(err) X(tasks) X(cellCount) X(blockSums) X(total)            \
-    X(outXyzw) X(outIds) X(bytes)
+    X(outXyzw) X(outIds) X(bytes) X(want)
 
 struct BuildBuffers {
 #define APO_DECLARE(f) decltype(apo_ctx::f) f;
@@ -931,6 +973,7 @@ bool carveStatic(void* base, uint32_t n, uint32_t K, double* xyzw, uint32_t* ids
     c.take(b.blockSums, (n + 255u) / 256u + 2u);
     c.take(b.total, 1);
     c.take(b.bytes, size_t(kByteFamilies) * kByteSlots);
+    c.take(b.want, n);
     c.take(b.spill, size_t(kStaticWaveRegions) * waveScratchWords());
     b.spillBlocks = kStaticWaveRegions;
     b.outXyzw = reinterpret_cast<D4*>(xyzw);

@@ -128,6 +128,21 @@ __device__ __forceinline__ float boxFarthestDistance(const Box& b, F4 s) {
     return lenf(subf(c, fp)) - s.w;
 }
 
+// Can some point of box b lie strictly inside the half-space n.(x - p) < 0?
+// The box's lowest point along n is the corner taking the lower bound where
+// n is positive; the margin covers the FP32 rounding of the three-term dot
+// product (8 units in the last place of the sum of magnitudes, 2^-20), so no
+// box holding a site inside the half-space is rejected.  The
+// completion mode's hyperbolic edge searches prune with it.
+__device__ __forceinline__ bool boxReachesHalfSpace(const Box& b, F3 n, F3 p) {
+    const float dx = (n.x > 0.f ? b.mn.x : b.mx.x) - p.x;
+    const float dy = (n.y > 0.f ? b.mn.y : b.mx.y) - p.y;
+    const float dz = (n.z > 0.f ? b.mn.z : b.mx.z) - p.z;
+    const float ex = n.x * dx, ey = n.y * dy, ez = n.z * dz;
+    const float margin = 0x1p-20f * (__builtin_fabsf(ex) + __builtin_fabsf(ey) + __builtin_fabsf(ez));
+    return ex + ey + ez < margin;
+}
+
 // ---------------------------------------------------------------------------
 // LBVH node: both child boxes live in the parent so one 64-byte line per
 // visited node carries everything the traversal needs.

@@ -144,6 +144,7 @@ struct CellCtx {
     uint32_t* err;          // device guard bits
     unsigned long long* bytes;  // algorithmic HBM bytes per kernel family, kByteSlots counters each
     float coverRatio;       // edge searches with a sphere above coverRatio x known radius use the edge cover
+    uint32_t complete;      // completion mode: hyperbolic edge searches prune only boxes wholly outside a half-space
 };
 
 __device__ __forceinline__ uint32_t nodeCount(const CellCtx& c) { return c.n + c.internal; }
@@ -193,7 +194,14 @@ constexpr uint32_t kBisectorQueue = 32;
 constexpr uint32_t kTreeHeight = 40;   // deepest leaf of the build's trees (root = 0)
 constexpr uint32_t kRoundTasks = 64;
 constexpr uint32_t kMaxRounds = 256;
-constexpr uint32_t kCounterWords = kRoundTasks + kMaxRounds;
+// completion mode: cells updated by each consistency round, the records no
+// update can repair, then the task counts of its validation rounds (after
+// the bisector stage, then after each consistency round: kMaxRounds each)
+constexpr uint32_t kMaxCompletionRounds = 8;
+constexpr uint32_t kCompletionUpdated = kRoundTasks + kMaxRounds;
+constexpr uint32_t kCompletionStuck = kCompletionUpdated + kMaxCompletionRounds;
+constexpr uint32_t kCompletionTasks = kCompletionStuck + 8;
+constexpr uint32_t kCounterWords = kCompletionTasks + (1 + kMaxCompletionRounds) * kMaxRounds;
 
 static __global__ void k_fill_words(uint32_t* p, uint32_t count, uint32_t value) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -248,9 +256,14 @@ struct ValidationTiming {
     uint32_t rounds;                 // rounds with events
 };
 hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks);
+// roundTasks: one word per round for its task count (vertex + edge rounds)
 hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const ValidationGrid& g,
                             uint32_t iterations, uint32_t maxRestarts, bool bisector, hipStream_t st,
-                            ValidationTiming* tm);
+                            ValidationTiming* tm, uint32_t* roundTasks = nullptr);
+// One consistency round of the completion mode (complete.hip): want holds n
+// words, kNone between rounds; round < kMaxCompletionRounds.
+hipError_t consistencyLaunch(const CellCtx& c, uint32_t* want, uint32_t* counters, uint32_t round, uint32_t blocks,
+                             hipStream_t st);
 
 // Output (algorithm.inl:1235-1307): count, scan, write.
 struct OutputBuffers {

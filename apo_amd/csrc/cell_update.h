@@ -444,6 +444,20 @@ __device__ __forceinline__ void initCell(CellLds& L, WaveCell& w, uint32_t n) {
     waveSync();
 }
 
+// j is an endpoint of a current edge (wave-wide; algorithm.inl:368-376)
+__device__ __forceinline__ bool knownSite(const CellLds& L, const WaveCell& w, uint32_t j) {
+    const uint32_t lane = laneId();
+    bool hit = false;
+    for (uint32_t b = 0; b < w.E; b += 64) {
+        const uint32_t e = b + lane;
+        if (e < w.E) {
+            const uint4 ed = L.edg[e];
+            hit = hit || ed.x == j || ed.y == j;
+        }
+    }
+    return ballot(hit) != 0;
+}
+
 // The stored records of cell i (its vertices too when withVertices).
 __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
     const uint32_t lane = laneId();

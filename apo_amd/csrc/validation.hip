@@ -107,20 +107,6 @@ __device__ __forceinline__ void popNode(const VStack& st, Trav& t) {
     }
 }
 
-// j is an endpoint of a current edge (wave-wide; algorithm.inl:368-376)
-__device__ __forceinline__ bool knownSite(const CellLds& L, const WaveCell& w, uint32_t j) {
-    const uint32_t lane = laneId();
-    bool hit = false;
-    for (uint32_t b = 0; b < w.E; b += 64) {
-        const uint32_t e = b + lane;
-        if (e < w.E) {
-            const uint4 ed = L.edg[e];
-            hit = hit || ed.x == j || ed.y == j;
-        }
-    }
-    return ballot(hit) != 0;
-}
-
 // same test from one lane, over the neighbour set
 __device__ __forceinline__ bool knownLane(const CellLds& L, uint32_t F, uint32_t j) {
     uint32_t q = 0;
@@ -575,11 +561,18 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, const VStack& stN
             const F3 closePoint = selF3(lc, cpL, cpR);
             closeChild = lc ? L0 : R0;
             ef = minFar < 0.f;
-            if (q.hyperbola && ef)
-                ef = dotf(q.n1, subf(farPoint, q.pp1)) < 0.f && dotf(q.n2, subf(farPoint, q.pp2)) < 0.f;
             ec = minClose < 0.f;
-            if (q.hyperbola && ec)
-                ec = dotf(q.n1, subf(closePoint, q.pp1)) < 0.f && dotf(q.n2, subf(closePoint, q.pp2)) < 0.f;
+            if (q.hyperbola && c.complete) {
+                // a subtree may hold a site inside both half-spaces wherever its
+                // box reaches into them (the reference tests only the box point
+                // closest to the search sphere's centre, algorithm.inl:660-670)
+                const Box fb = selBox(lc, rb, lb), cb = selBox(lc, lb, rb);
+                ef = ef && boxReachesHalfSpace(fb, q.n1, q.pp1) && boxReachesHalfSpace(fb, q.n2, q.pp2);
+                ec = ec && boxReachesHalfSpace(cb, q.n1, q.pp1) && boxReachesHalfSpace(cb, q.n2, q.pp2);
+            } else if (q.hyperbola) {
+                if (ef) ef = dotf(q.n1, subf(farPoint, q.pp1)) < 0.f && dotf(q.n2, subf(farPoint, q.pp2)) < 0.f;
+                if (ec) ec = dotf(q.n1, subf(closePoint, q.pp1)) < 0.f && dotf(q.n2, subf(closePoint, q.pp2)) < 0.f;
+            }
             if (q.cover) {
                 const float lr = __builtin_bit_cast(float, nn.d.z), rr = __builtin_bit_cast(float, nn.d.w);
                 const Box fb = selBox(lc, rb, lb), cb = selBox(lc, lb, rb);
@@ -1218,7 +1211,8 @@ hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uin
 
 hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const ValidationGrid& g,
                             uint32_t iterations, uint32_t maxRestarts, bool bisector, hipStream_t st,
-                            ValidationTiming* tm) {
+                            ValidationTiming* tm, uint32_t* roundTasks) {
+    if (!roundTasks) roundTasks = v.counters + kRoundTasks;
     if (bisector) {
         hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, v.counters + kBisectorQueue, kXcdQueueWords, 0u);
         hipLaunchKernelGGL(k_bisector_validation, dim3(g.bisector), dim3(64), 0, st, c, v.counters, maxRestarts);
@@ -1233,7 +1227,7 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const 
         const bool timed = tm && it < kTimedRounds;
         if (timed) e = hipEventRecord(tm->ev[it][0], st);
         hipLaunchKernelGGL(k_vertex_validation, dim3(g.vertex), dim3(64), 0, st, c, v.tasks, v.counters,
-                           v.counters + kRoundTasks + it, maxRestarts, v.spill);
+                           roundTasks + it, maxRestarts, v.spill);
         if (timed && e == hipSuccess) e = hipEventRecord(tm->ev[it][1], st);
         hipLaunchKernelGGL(k_edge_validation, dim3(g.edge), dim3(64), 0, st, c, v.tasks, v.counters, maxRestarts,
                            v.spill);

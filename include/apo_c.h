@@ -66,6 +66,17 @@ typedef struct apo_config {
                                  pass did not fill ends) and updates the cells still in validation; 0 runs
                                  no kNN or cell pass (every cell goes to validation as the octahedron).
                                  Default 1; the learned proposal needs 1 */
+    uint32_t complete;        /* completion mode, no counterpart in the reference; 0 = off (default, the
+                                 reference's output).  1..8: (a) the hyperbolic edge searches prune a subtree
+                                 only when no point of its box lies inside both tangent-plane half-spaces (the
+                                 reference tests the box point closest to the search sphere's centre,
+                                 algorithm.inl:660-670, and skips contributors); (b) the cells the bisector
+                                 stage changed go through vertex and edge validation again (the reference
+                                 leaves them unvalidated, :1551-1594); (c) up to `complete` consistency rounds:
+                                 a cell holding fewer records of a quadruple (i, j, k, l) than another of its
+                                 four cells is updated with the smallest quadruple site it does not know, and
+                                 validated again.  The output then no longer depends on k or the learned
+                                 proposal where the reference's does */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,
@@ -101,6 +112,15 @@ typedef struct apo_stats {
     double ms_proposal;
     uint64_t proposal;
     uint64_t cells_certified;
+    /* completion mode (config.complete): its time, the consistency rounds that
+     * ran, the cells they updated, the missing records whose sites the cell
+     * already knows (no update repairs them), and the tasks of its validation
+     * rounds */
+    double ms_completion;
+    uint64_t completion_rounds;
+    uint64_t completion_updated_cells;
+    uint64_t completion_unrepairable;
+    uint64_t completion_validation_tasks;
     uint64_t reserved[2];
 } apo_stats;
 

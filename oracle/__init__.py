@@ -27,7 +27,8 @@ u8p = ctypes.POINTER(ctypes.c_uint8)
 class OrcConfig(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint32) for name in (
         "k", "vertex_capacity", "edge_capacity", "new_edge_capacity", "bisector_capacity",
-        "max_iterations", "max_restarts", "threads", "stage_mask", "edge_bound", "fallback_k", "knn_passes")]
+        "max_iterations", "max_restarts", "threads", "stage_mask", "edge_bound", "fallback_k", "knn_passes",
+        "complete")]
 
 
 def build() -> str:
@@ -311,10 +312,11 @@ class Diagram:
         return out
 
     def stats(self):
-        s = np.zeros(8, np.uint64)
+        s = np.zeros(12, np.uint64)
         lib().orc_get_stats(self._h, _p(s, u64p))
         keys = ["uncertified_after_cells", "validation_tasks", "validation_iterations", "flagged_cells",
-                "vertices", "buried", "bisector_changed", "reserved"]
+                "vertices", "buried", "bisector_changed", "completion_rounds", "completion_updated_cells",
+                "completion_unrepairable", "completion_validation_tasks", "reserved"]
         return dict(zip(keys, (int(x) for x in s)))
 
 

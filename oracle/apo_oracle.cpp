@@ -1113,6 +1113,22 @@ bool edgeCover(const Context& C, const Cell& cell, const D4& si, uint32_t k, uin
     return true;
 }
 
+// Completion mode: can some point of box b lie strictly inside the
+// half-space n.(x - p) < 0?  The box's lowest point along n is the corner
+// taking the lower bound where n is positive, compared with a margin above
+// the FP32 rounding of the three-term dot product, so no box that holds a
+// site the leaf test accepts is skipped.  (The reference tests the box's
+// point closest to the search sphere's centre, algorithm.inl:660-670, which
+// can lie outside the half-space while part of the box lies inside.)
+inline bool boxReachesHalfSpace(const Box& b, F3 n, F3 p) {
+    const float dx = (n.x > 0.f ? b.mn.x : b.mx.x) - p.x;
+    const float dy = (n.y > 0.f ? b.mn.y : b.mx.y) - p.y;
+    const float dz = (n.z > 0.f ? b.mn.z : b.mx.z) - p.z;
+    const float ex = n.x * dx, ey = n.y * dy, ez = n.z * dz;
+    const float margin = 0x1p-20f * (std::fabs(ex) + std::fabs(ey) + std::fabs(ez));
+    return ex + ey + ez < margin;
+}
+
 // Contributor search around an edge (algorithm.inl:616-793)
 uint32_t edgeSearch(const Context& C, const Cell& cell, uint32_t i, const D4& si, uint32_t k, uint32_t l,
                     const EdgeQuery& q, uint32_t& flags) {
@@ -1135,11 +1151,16 @@ uint32_t edgeSearch(const Context& C, const Cell& cell, uint32_t i, const D4& si
             const F3 closePoint = lc ? cpL : cpR;
             const uint32_t closeChild = lc ? L : R;
             bool ef = minFar < 0.f;
-            if (q.hyperbola && ef)
-                ef = ef && dotf(q.n1, subf(farPoint, q.pp1)) < 0.f && dotf(q.n2, subf(farPoint, q.pp2)) < 0.f;
             bool ec = minClose < 0.f;
-            if (q.hyperbola && ec)
+            if (q.hyperbola && C.cfg.complete) {
+                const Box& fb = lc ? rb : lb;
+                const Box& cb = lc ? lb : rb;
+                ef = ef && boxReachesHalfSpace(fb, q.n1, q.pp1) && boxReachesHalfSpace(fb, q.n2, q.pp2);
+                ec = ec && boxReachesHalfSpace(cb, q.n1, q.pp1) && boxReachesHalfSpace(cb, q.n2, q.pp2);
+            } else if (q.hyperbola) {
+                ef = ef && dotf(q.n1, subf(farPoint, q.pp1)) < 0.f && dotf(q.n2, subf(farPoint, q.pp2)) < 0.f;
                 ec = ec && dotf(q.n1, subf(closePoint, q.pp1)) < 0.f && dotf(q.n2, subf(closePoint, q.pp2)) < 0.f;
+            }
             if (q.cover) {
                 const Box fb = lc ? rb : lb, cb = lc ? lb : rb;
                 const float fr = b.maxr[farChild], cr = b.maxr[closeChild];
@@ -1497,6 +1518,99 @@ void stageBisectorValidation(Context& C, uint64_t& changedCells) {
     changedCells = changedCount.load();
 }
 
+// The vertex + edge validation loop of AlgorithmGPU::build (algorithm.inl:
+// 1551-1593): rounds over the cells in validation (status 0) while there are
+// any, at most max_iterations.  Adds the round task counts to *tasks;
+// returns the rounds run.
+uint32_t validationRounds(Context& C, uint64_t& tasksTotal) {
+    const uint32_t n = C.n;
+    uint32_t it = 0;
+    for (; it < C.cfg.max_iterations; it++) {
+        std::vector<uint32_t> tasks;
+        for (uint32_t i = 0; i < n; i++)
+            if (C.status[i] == 0) tasks.push_back(i);
+        if (tasks.empty()) break;
+        tasksTotal += tasks.size();
+        stageVertexValidation(C, tasks);
+        stageEdgeValidation(C, tasks);
+    }
+    return it;
+}
+
+// Records of cell c whose three sites are {a, b, d} (ascending).
+inline uint32_t recordCount(const Cell& c, uint32_t a, uint32_t b, uint32_t d) {
+    uint32_t m = 0;
+    for (const Vtx& v : c.V) m += v.j == a && v.k == b && v.l == d;
+    return m;
+}
+
+// One consistency round of the completion mode (no counterpart in the
+// reference).  A vertex of the diagram is tangent to 4 sites and belongs to
+// the 4 cells; a record (i; j, k, l) of cell i asks cells j, k and l for as
+// many records of the same quadruple.  A cell holding fewer wants the
+// quadruple sites it does not know; each cell is then updated with the
+// smallest site it wants (one per round, so the result does not depend on
+// the order of the requests) and goes back to validation when it changes.
+// Buried cells, artificial sites and cells with a capacity flag take no
+// part.  A cell that knows all three sites yet holds fewer records cannot be
+// repaired by an update and is counted in *unrepairable.  Returns the cells
+// the round changed.
+uint64_t consistencyRound(Context& C, uint64_t& unrepairable) {
+    const uint32_t n = C.n;
+    std::vector<std::atomic<uint32_t>> want(n);
+    for (auto& w : want) w.store(kNone, std::memory_order_relaxed);
+    std::atomic<uint64_t> stuck{0};
+    auto part = [&](uint32_t m) { return C.status[m] != kBuried && C.flags[m] == 0u; };
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t ii = 0; ii < int64_t(n); ii++) {
+        const uint32_t i = uint32_t(ii);
+        if (!part(i)) continue;
+        const Cell& ci = C.cells[i];
+        uint64_t myStuck = 0;
+        for (const Vtx& v : ci.V) {
+            if (v.l >= n) continue;  // j < k < l: an artificial site is last
+            const uint32_t own = recordCount(ci, v.j, v.k, v.l);
+            const uint32_t q[4] = {i, v.j, v.k, v.l};
+            for (int t = 1; t < 4; t++) {
+                const uint32_t m = q[t];
+                if (!part(m)) continue;
+                uint32_t o[3], c = 0;
+                for (int u = 0; u < 4; u++)
+                    if (u != t) o[c++] = q[u];
+                std::sort(o, o + 3);
+                if (recordCount(C.cells[m], o[0], o[1], o[2]) >= own) continue;
+                bool any = false;
+                for (int u = 0; u < 3; u++)
+                    if (!known(C.cells[m], o[u])) {
+                        uint32_t cur = want[m].load(std::memory_order_relaxed);
+                        while (o[u] < cur && !want[m].compare_exchange_weak(cur, o[u])) {
+                        }
+                        any = true;
+                    }
+                if (!any) myStuck++;
+            }
+        }
+        if (myStuck) stuck += myStuck;
+    }
+    unrepairable += stuck.load();
+    std::atomic<uint64_t> updated{0};
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t mm = 0; mm < int64_t(n); mm++) {
+        const uint32_t m = uint32_t(mm);
+        const uint32_t j = want[m].load(std::memory_order_relaxed);
+        if (j == kNone) continue;
+        Cell& cell = C.cells[m];
+        uint32_t flags = 0;
+        const bool changed = updateCell(C, m, C.S[m], cell.V, cell, j, C.S[j], flags);
+        C.flags[m] |= flags;
+        if (changed) {
+            C.status[m] = 0;
+            updated++;
+        }
+    }
+    return updated.load();
+}
+
 // pcg hash of the reference (apo/src/core/utils.cpp:6-12)
 inline uint32_t pcgWord(uint32_t v) {
     const uint32_t state = v * 747796405u + 2891336453u;
@@ -1534,7 +1648,7 @@ struct orc_diagram {
     std::vector<uint32_t> outIds;
     std::vector<double> edgeXyzw;
     std::vector<uint32_t> edgeIds;
-    uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t stats[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 extern "C" {
@@ -1671,6 +1785,7 @@ void orc_default_config(orc_config* cfg) {
     cfg->edge_bound = 0;
     cfg->fallback_k = 16;
     cfg->knn_passes = 1;
+    cfg->complete = 0;
 }
 
 static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {
@@ -1769,25 +1884,27 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
     for (uint32_t i = 0; i < n; i++) uncertified += C.status[i] == 0;
     d->stats[0] = uncertified;
 
-    if (cfg.stage_mask & 2u) {
-        uint32_t it = 0;
-        uint64_t taskNb = uncertified;
-        for (; it < cfg.max_iterations && taskNb != 0; it++) {
-            std::vector<uint32_t> tasks;
-            for (uint32_t i = 0; i < n; i++)
-                if (C.status[i] == 0) tasks.push_back(i);
-            d->stats[1] += tasks.size();
-            stageVertexValidation(C, tasks);
-            stageEdgeValidation(C, tasks);
-            taskNb = 0;
-            for (uint32_t i = 0; i < n; i++) taskNb += C.status[i] == 0;
-        }
-        d->stats[2] = it;
-    }
+    if (cfg.stage_mask & 2u) d->stats[2] = validationRounds(C, d->stats[1]);
     if (cfg.stage_mask & 4u) {
         uint64_t changed = 0;
         stageBisectorValidation(C, changed);
         d->stats[6] = changed;
+    }
+    if (cfg.complete && (cfg.stage_mask & 2u)) {
+        // the cells the bisector stage changed (status 0) are validated again,
+        // then the consistency rounds
+        validationRounds(C, d->stats[10]);
+        // every round's validation rounds run (over the cells still in
+        // validation); a round after one that updated no cell checks nothing
+        uint64_t updated = 1;
+        for (uint32_t round = 0; round < cfg.complete; round++) {
+            if (updated) {
+                d->stats[7]++;
+                updated = consistencyRound(C, d->stats[9]);
+                d->stats[8] += updated;
+            }
+            validationRounds(C, d->stats[10]);
+        }
     }
     uint64_t overflowed = 0;
     for (uint32_t i = 0; i < n; i++) overflowed += C.flags[i] != 0;
@@ -1952,7 +2069,7 @@ void orc_topology(const orc_diagram* d, uint64_t* counts3, uint32_t* bis, uint32
 
 void orc_get_knns(const orc_diagram* d, uint32_t* knn) { std::memcpy(knn, d->knn.data(), d->knn.size() * 4); }
 
-void orc_get_stats(const orc_diagram* d, uint64_t* stats8) { std::memcpy(stats8, d->stats, sizeof(d->stats)); }
+void orc_get_stats(const orc_diagram* d, uint64_t* stats12) { std::memcpy(stats12, d->stats, sizeof(d->stats)); }
 
 void orc_certified_k(const float* xyzr, uint32_t n, uint32_t kmax, uint32_t* out_k, uint32_t threads) {
 #ifdef _OPENMP

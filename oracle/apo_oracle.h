@@ -81,6 +81,13 @@ typedef struct orc_config {
     uint32_t knn_passes;        /* kNN + cell passes, the reference's knnValidationStep (algorithm.cuh:16,
                                    algorithm.inl:1535-1549): pass s > 0 continues each list after the previous
                                    pass's last neighbour; 0 runs none (cells stay the octahedron).  Default 1 */
+    uint32_t complete;          /* completion mode (no counterpart in the reference), 0 = off (default): the
+                                   hyperbolic edge searches prune a subtree only when no point of its box lies
+                                   inside both tangent-plane half-spaces; the cells the bisector stage changed
+                                   go through vertex and edge validation again; then up to `complete`
+                                   consistency rounds: a cell holding fewer records of a quadruple (i,j,k,l)
+                                   than one of its other three cells is updated with the quadruple's sites it
+                                   does not know, and the cells that changed are validated again */
 } orc_config;
 
 void orc_default_config(orc_config* cfg);
@@ -118,8 +125,12 @@ void orc_topology(const orc_diagram* d, uint64_t* counts3, uint32_t* bisectors, 
 void orc_get_knns(const orc_diagram* d, uint32_t* knn);
 /* Stage statistics: [0] uncertified after cells, [1] tasks per validation
  * iteration summed, [2] iterations run, [3] overflowed cells, [4] vertices
- * emitted, [5] buried cells, [6] cells changed by bisector validation.   */
-void orc_get_stats(const orc_diagram* d, uint64_t* stats8);
+ * emitted, [5] buried cells, [6] cells changed by bisector validation,
+ * [7] completion: consistency rounds run, [8] cells updated by them,
+ * [9] missing records whose sites the cell already knows (not repairable
+ * by an update), [10] tasks of the validation rounds after the bisector
+ * stage and after each consistency round, [11] reserved.                 */
+void orc_get_stats(const orc_diagram* d, uint64_t* stats12);
 /* Certified k per cell (first neighbour count at which the security-radius
  * test passes, 0xffffffff when the K neighbours never certify), computed
  * with the per-neighbour test of app/benchmarks/benchmark_contributing_nearest.cu:75-113. */

@@ -1,0 +1,115 @@
+"""Completion mode (config.complete) on the GPU.
+
+* Parity: every cell record, status, flag and output vertex equals the
+  oracle's completion mode bit for bit (uniform 10k, log-uniform 100k,
+  protein-like 100k).
+* The reference method's output depends on K (its hyperbolic edge searches
+  skip contributors that a larger K inserts directly); with completion the
+  vertex set no longer does: K = 16, 24 and 32 give the same rows.
+* At full size (1M log-uniform sites, the learned-proposal workload): the
+  plain build with completion has 0 empty-sphere violations and 0 bad
+  interval vertices, and the learned path (per-site neighbour counts, here
+  the label classes of the same sites) emits exactly the plain path's
+  vertex rows.  The difference without completion is printed.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from apo_amd import AlgorithmGPU, datasets, proposal
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "uniform_10k": lambda: datasets.uniform_sites(10_000, seed=1),
+    "loguniform_100k": lambda: datasets.loguniform_sites(100_000, seed=3),
+    "protein_100k": lambda: datasets.protein_like_sites(100_000, seed=2),
+}
+
+
+def _rows(x, ids):
+    """Vertex rows (ids, FP64 bits) in a canonical order: the multiset of
+    emitted vertices."""
+    a = np.concatenate([ids.astype(np.uint64), x.view(np.uint64)], axis=1)
+    return a[np.lexsort(a.T[::-1])]
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_complete_matches_oracle(name):
+    sites = CASES[name]()
+    print(f"[case] completion {name}", flush=True)
+    gpu = AlgorithmGPU(sites, complete=4).build()
+    try:
+        orc = oracle.Diagram(sites, complete=4, edge_bound=1 if len(sites) > 30_000 else 0)
+        gc, gV, gE = gpu.all_cells()
+        oc, oV, oE = orc.all_cells()
+        for key in ("status", "vertex_nb", "edge_nb", "flags"):
+            assert np.array_equal(gc[key], oc[key]), key
+        assert np.array_equal(gV, oV) and np.array_equal(gE, oE)
+        g = gpu.to_host()
+        ox, oi = orc.vertices()
+        assert np.array_equal(g.vertices_id, oi)
+        assert np.array_equal(g.vertices.view(np.uint64), ox.view(np.uint64))
+        s, os_ = gpu.stats(), orc.stats()
+        assert s["completion_rounds"] == os_["completion_rounds"]
+        assert s["completion_updated_cells"] == os_["completion_updated_cells"]
+        assert s["completion_unrepairable"] == os_["completion_unrepairable"]
+        assert s["completion_validation_tasks"] == os_["completion_validation_tasks"]
+        print(f"[completion] {name}: rounds {s['completion_rounds']}, updated {s['completion_updated_cells']}, "
+              f"unrepairable {s['completion_unrepairable']}", flush=True)
+    finally:
+        gpu.close()
+
+
+def test_complete_vertex_set_independent_of_k():
+    sites = datasets.loguniform_sites(100_000, seed=3)
+    rows, plain = {}, {}
+    for k in (16, 24, 32):
+        print(f"[case] log-uniform 100k k={k}", flush=True)
+        for comp, out in ((4, rows), (0, plain)):
+            g = AlgorithmGPU(sites, k=k, complete=comp).build()
+            try:
+                d = g.to_host()
+                out[k] = _rows(d.vertices, d.vertices_id)
+            finally:
+                g.close()
+    print(f"[k] without completion: {[len(plain[k]) for k in (16, 24, 32)]} vertices", flush=True)
+    assert not np.array_equal(plain[16], plain[24])  # the reference's output depends on K here
+    assert np.array_equal(rows[16], rows[24]) and np.array_equal(rows[16], rows[32])
+
+
+def _set_difference(a, b):
+    va = {r.tobytes() for r in a}
+    vb = {r.tobytes() for r in b}
+    return len(va - vb), len(vb - va)
+
+
+def test_complete_learned_equals_plain_1m():
+    sites = datasets.loguniform_sites(1_000_000, seed=3)
+    print("[case] 1M log-uniform: label classes", flush=True)
+    _, labels = proposal.labels(sites)
+    classes = proposal.label_class(labels)
+    out = {}
+    for comp in (0, 4):
+        for path in ("plain", "learned"):
+            print(f"[case] 1M {path} complete={comp}", flush=True)
+            kw = {"classes": classes} if path == "learned" else {}
+            g = AlgorithmGPU(sites, complete=comp, **kw).build()
+            try:
+                d = g.to_host()
+                out[path, comp] = _rows(d.vertices, d.vertices_id)
+                if comp:
+                    viol = g.empty_sphere_violations()
+                    prec = g.precision()
+                    s = g.stats()
+                    print(f"[completion] 1M {path}: {len(d.vertices)} vertices, {viol} violations, "
+                          f"{prec['bad_quadrisectors']} bad / {prec['uncertain_quadrisectors']} uncertain, "
+                          f"rounds {s['completion_rounds']} updated {s['completion_updated_cells']} "
+                          f"unrepairable {s['completion_unrepairable']} {s['ms_completion']:.1f} ms", flush=True)
+                    assert viol == 0
+                    assert prec["bad_quadrisectors"] == 0
+            finally:
+                g.close()
+    print(f"[completion] 1M without completion: only plain / only learned = "
+          f"{_set_difference(out['plain', 0], out['learned', 0])}", flush=True)
+    assert np.array_equal(out["plain", 4], out["learned", 4])

@@ -97,6 +97,8 @@ def lib():
         L.orc_topology.argtypes = [ctypes.c_void_p, u64p, u32p, u32p, u32p]
         L.orc_get_knns.restype = None
         L.orc_get_knns.argtypes = [ctypes.c_void_p, u32p]
+        L.orc_box_reaches_half_space.restype = ctypes.c_int
+        L.orc_box_reaches_half_space.argtypes = [f32p, f32p, f32p]
         L.orc_get_stats.restype = None
         L.orc_get_stats.argtypes = [ctypes.c_void_p, u64p]
         L.orc_certified_k.restype = None
@@ -318,6 +320,13 @@ class Diagram:
                 "vertices", "buried", "bisector_changed", "completion_rounds", "completion_updated_cells",
                 "completion_unrepairable", "completion_validation_tasks", "reserved"]
         return dict(zip(keys, (int(x) for x in s)))
+
+
+def box_reaches_half_space(box6, n, p) -> bool:
+    b = np.ascontiguousarray(box6, np.float32)
+    nn = np.ascontiguousarray(n, np.float32)
+    pp = np.ascontiguousarray(p, np.float32)
+    return bool(lib().orc_box_reaches_half_space(_p(b, f32p), _p(nn, f32p), _p(pp, f32p)))
 
 
 def certified_k(xyzr, kmax: int = 64, threads: int = 0):

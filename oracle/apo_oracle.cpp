@@ -2069,6 +2069,11 @@ void orc_topology(const orc_diagram* d, uint64_t* counts3, uint32_t* bis, uint32
 
 void orc_get_knns(const orc_diagram* d, uint32_t* knn) { std::memcpy(knn, d->knn.data(), d->knn.size() * 4); }
 
+int orc_box_reaches_half_space(const float* box6, const float* n3, const float* p3) {
+    const Box b{{box6[0], box6[1], box6[2]}, {box6[3], box6[4], box6[5]}};
+    return boxReachesHalfSpace(b, F3{n3[0], n3[1], n3[2]}, F3{p3[0], p3[1], p3[2]}) ? 1 : 0;
+}
+
 void orc_get_stats(const orc_diagram* d, uint64_t* stats12) { std::memcpy(stats12, d->stats, sizeof(d->stats)); }
 
 void orc_certified_k(const float* xyzr, uint32_t n, uint32_t kmax, uint32_t* out_k, uint32_t threads) {

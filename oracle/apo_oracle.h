@@ -86,8 +86,8 @@ typedef struct orc_config {
                                    inside both tangent-plane half-spaces; the cells the bisector stage changed
                                    go through vertex and edge validation again; then up to `complete`
                                    consistency rounds: a cell holding fewer records of a quadruple (i,j,k,l)
-                                   than one of its other three cells is updated with the quadruple's sites it
-                                   does not know, and the cells that changed are validated again */
+                                   than one of its other three cells is updated with the smallest quadruple
+                                   site it does not know, and the cells that changed are validated again */
 } orc_config;
 
 void orc_default_config(orc_config* cfg);
@@ -131,6 +131,9 @@ void orc_get_knns(const orc_diagram* d, uint32_t* knn);
  * by an update), [10] tasks of the validation rounds after the bisector
  * stage and after each consistency round, [11] reserved.                 */
 void orc_get_stats(const orc_diagram* d, uint64_t* stats12);
+/* Completion mode's subtree test: 1 when some point of the box (min xyz,
+ * max xyz) may lie strictly inside the half-space n.(x - p) < 0. */
+int orc_box_reaches_half_space(const float* box6, const float* n3, const float* p3);
 /* Certified k per cell (first neighbour count at which the security-radius
  * test passes, 0xffffffff when the K neighbours never certify), computed
  * with the per-neighbour test of app/benchmarks/benchmark_contributing_nearest.cu:75-113. */

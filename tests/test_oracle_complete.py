@@ -1,0 +1,80 @@
+"""Completion mode of the oracle (config.complete), pinned on the CPU.
+
+* The subtree test of the completion mode's hyperbolic edge searches never
+  rejects a box that reaches into the half-space (exact corner minimum in
+  FP64), nor one holding a sphere the leaf test accepts, and rejects boxes
+  clearly outside.
+* With completion the diagram of small general-position sets still equals
+  the brute-force Apollonius diagram, at every K and pass count.
+* On sets where the reference's output depends on K (its edge searches skip
+  contributors a larger K inserts directly), completion gives one vertex
+  set for K = 16, 24 and 32, with no empty-sphere violation.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from apo_amd import datasets
+
+
+def test_half_space_box_test_is_conservative():
+    rng = np.random.default_rng(5)
+    reached = rejected = 0
+    for _ in range(20000):
+        lo = rng.uniform(-1, 1, 3).astype(np.float32)
+        box = np.concatenate([lo, lo + rng.uniform(0, 0.5, 3).astype(np.float32)])
+        n = rng.normal(size=3)
+        n = (n / np.linalg.norm(n)).astype(np.float32)
+        p = rng.uniform(-1.2, 1.2, 3).astype(np.float32)
+        corners = np.array([[box[3 * a + 0], box[3 * b + 1], box[3 * c + 2]]
+                            for a in (0, 1) for b in (0, 1) for c in (0, 1)], np.float64)
+        low = float(np.min((corners - p.astype(np.float64)) @ n.astype(np.float64)))
+        got = oracle.box_reaches_half_space(box, n, p)
+        if low < 0:
+            assert got, (box, n, p, low)
+            reached += 1
+        elif low > 1e-5:
+            assert not got, (box, n, p, low)
+            rejected += 1
+        # a sphere inside the box that the leaf test accepts keeps the box
+        lo64, hi64 = box[:3].astype(np.float64), box[3:].astype(np.float64)
+        r = min(rng.uniform(0, 0.1), float(np.min(hi64 - lo64)) * 0.45)
+        c = lo64 + r + rng.random(3) * (hi64 - lo64 - 2 * r)
+        r, c = np.float32(r), c.astype(np.float32)
+        if np.dot(n, c - p) < -r:
+            assert got
+    assert reached > 1000 and rejected > 1000
+
+
+def _rows(ids):
+    a = np.sort(np.asarray(ids, np.uint32).reshape(-1, 4), axis=1)
+    return a[np.lexsort(a.T[::-1])] if len(a) else a
+
+
+@pytest.mark.parametrize("k,passes", [(16, 1), (4, 3), (1, 4)])
+@pytest.mark.parametrize("n,seed", [(16, 2), (32, 4), (40, 5)])
+def test_completion_equals_brute_force(k, passes, n, seed):
+    sites = datasets.uniform_sites(n, seed=100 + seed, r_min=0.01, r_max=0.06)
+    _, ids = oracle.Diagram(sites, k=k, knn_passes=passes, complete=4).vertices()
+    assert np.array_equal(_rows(ids), _rows(oracle.bruteforce_vertices(sites)))
+
+
+def _vertex_rows(d):
+    x, ids = d.vertices()
+    a = np.concatenate([ids.astype(np.uint64), x.view(np.uint64)], axis=1)
+    return a[np.lexsort(a.T[::-1])], x, ids
+
+
+@pytest.mark.parametrize("gen,seed", [("protein", 8), ("loguniform", 6)])
+def test_completion_removes_the_dependence_on_k(gen, seed):
+    sites = (datasets.protein_like_sites(20_000, seed=seed) if gen == "protein"
+             else datasets.loguniform_sites(20_000, seed=seed))
+    plain = {k: _vertex_rows(oracle.Diagram(sites, k=k))[0] for k in (16, 24)}
+    assert not np.array_equal(plain[16], plain[24])
+    rows = {}
+    for k in (16, 24, 32):
+        d = oracle.Diagram(sites, k=k, complete=4)
+        rows[k], x, ids = _vertex_rows(d)
+        if gen != "protein":  # overlapping atoms give the method negative-radius vertices it counts
+            assert oracle.empty_sphere_violations(sites, x, ids) == 0
+    assert np.array_equal(rows[16], rows[24]) and np.array_equal(rows[16], rows[32])

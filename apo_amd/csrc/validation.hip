@@ -628,11 +628,13 @@ __device__ __forceinline__ uint32_t waveOr(uint32_t v) {
 }
 
 // Store the edges and append the vertices created during this kernel after
-// the cell's stored vertices (algorithm.inl:840-855, 1216-1231).
+// the cell's stored vertices (algorithm.inl:840-855, 1216-1231).  In the
+// completion mode the kernel updated the cell's whole vertex list, which
+// replaces the stored one.
 __device__ __forceinline__ void storeAppended(const CellCtx& c, const CellLds& L, WaveCell& w, uint32_t i,
                                               bool didChange, uint32_t family, uint32_t edgesLoaded) {
     const uint32_t lane = laneId();
-    const uint32_t baseV = c.vnb[i];
+    const uint32_t baseV = c.complete ? 0u : c.vnb[i];
     uint32_t total = baseV + w.V;
     if (total > kVertexCap) {
         total = kVertexCap;
@@ -780,7 +782,10 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
     const uint8_t st = c.status[i];
     if (st == kStatusCertified || st == kStatusBuried) return;
     WaveCell w;
-    loadCell(c, L, w, i, false);
+    // the reference's edge validation updates a vertex list that starts
+    // empty (algorithm.inl:526-527), so a contributor never invalidates the
+    // cell's stored vertices; the completion mode updates the whole list
+    loadCell(c, L, w, i, c.complete != 0u);
     const uint32_t edgesLoaded = w.E;
     for (uint32_t e = lane; e < w.E; e += 64)
         if (L.edg[e].w != 1u) L.edg[e].w = 2u;
@@ -1041,7 +1046,7 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
     const uint8_t st = c.status[i];
     if (st == kStatusBuried) return;
     WaveCell w;
-    loadCell(c, L, w, i, false);
+    loadCell(c, L, w, i, c.complete != 0u);
     const uint32_t edgesLoaded = w.E;
     uint32_t flags = 0;
     // sorted distinct neighbours above i (algorithm.inl:903-968)
@@ -1070,8 +1075,9 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
         flags |= kFlagBisectorOverflow;
     }
     waveSync();
-    // the vertex list of this kernel starts empty (algorithm.inl:970-971)
-    w.V = 0;
+    // the vertex list of this kernel starts empty (algorithm.inl:970-971),
+    // except in the completion mode, which updates the whole list
+    if (!c.complete) w.V = 0;
     const float kr = c.knownRadii[i];
     const D4 si = loadSite(c, i);
     const uint32_t root = rootNode(c, cellMolecule(c, i));

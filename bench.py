@@ -187,6 +187,59 @@ def _row_keys(ids):
     return a.view(np.dtype((np.void, 16))).ravel()
 
 
+def _vertex_rows(xyzw, ids):
+    """Every emitted vertex as (4 ids, FP64 bits of x, y, z, w), in a
+    canonical order: equal arrays mean identical vertex sets, positions
+    included."""
+    a = np.concatenate([ids.astype(np.uint64), np.ascontiguousarray(xyzw).view(np.uint64)], axis=1)
+    return a[np.lexsort(a.T[::-1])]
+
+
+def _completion_stats(s):
+    return {"ms": s.ms_completion, "rounds": int(s.completion_rounds),
+            "updated_cells": int(s.completion_updated_cells), "unrepairable": int(s.completion_unrepairable),
+            "validation_tasks": int(s.completion_validation_tasks)}
+
+
+def completion_section(dev, n, plain_tuning, steps, warmup, say):
+    """The plain build with the completion mode (config.complete = 4): one
+    HIP graph replay per step, then an eager build for its counts, the
+    FP64 empty-sphere check and the interval validator.  Returns the section
+    and the vertex rows the learned path is compared with."""
+    import torch
+    from apo_amd import _lib
+    tuning = dict(plain_tuning, complete=4)
+    gr = graph_replay(dev, n, tuning, steps, warmup, say)
+    ctx = _ctx(tuning)
+    L = _lib.lib()
+    ptr = _lib.ctypes.c_void_p(dev.data_ptr())
+    for _ in range(max(warmup, 1)):
+        _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+    wall = 0.0
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(ctx, L.apo_build_device(ctx, ptr, n))
+        torch.cuda.synchronize()
+        wall += time.perf_counter() - t0
+    eager_ms = 1000.0 * wall / steps
+    s = _lib.Stats()
+    _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
+    xyzw, ids = _vertices(ctx, L, _lib, int(s.vertices))
+    cnt = _lib.c_uint64()
+    _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
+    prec = _precision(ctx)
+    L.apo_destroy(ctx)
+    torch.cuda.synchronize()
+    say(f"completion: {gr['ms_per_step']:.1f} ms per graph replay, {int(s.vertices)} vertices, "
+        f"{int(cnt.value)} violations")
+    return ({"cells_per_s": gr["cells_per_s"], "ms_per_step": gr["ms_per_step"], "vertices": int(s.vertices),
+             "eager": {"cells_per_s": n / (eager_ms / 1000.0), "ms_per_step": eager_ms},
+             "empty_sphere_violations": int(cnt.value), "precision": prec, "completion": _completion_stats(s),
+             "config": tuning, "path": "plain + completion, one HIP graph replay per step"},
+            _vertex_rows(xyzw, ids), _row_keys(ids))
+
+
 def train_proposal(say):
     """Labels from the plain method on two 1M log-uniform sets (seeds
     501/502: 2M labelled sites) and 8 epochs of the HIP training kernels."""
@@ -214,10 +267,11 @@ def _precision(ctx):
     return {name: int(getattr(r, name)) for name, _ in r._fields_}
 
 
-def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None):
+def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None, plain_rows=None):
     """Time the workload with the proposal.  Cells the plain test does not
     certify after the proposed count go through the plain validation.
-    classes: per-site classes given instead of the MLP's."""
+    classes: per-site classes given instead of the MLP's.  plain_rows: the
+    plain build's vertex rows (_vertex_rows) for an exact comparison."""
     import torch
     from apo_amd import _lib
     ctx = _ctx(tuning, proposal=params)
@@ -238,12 +292,13 @@ def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None)
         wall += time.perf_counter() - t0
     _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
     v = int(s.vertices)
-    _, ids = _vertices(ctx, L, _lib, v)
+    xyzw, ids = _vertices(ctx, L, _lib, v)
     cnt = _lib.c_uint64()
     _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
     prec = _precision(ctx)
     L.apo_destroy(ctx)
     keys = _row_keys(ids)
+    same = None if plain_rows is None else bool(np.array_equal(_vertex_rows(xyzw, ids), plain_rows))
     ms = 1000.0 * wall / steps
     non_buried = n - int(s.buried)
     return {"cells_per_s": n / (ms / 1000.0), "ms_per_step": ms, "config": dict(tuning or {}),
@@ -253,7 +308,9 @@ def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None)
                          "validation": s.ms_validation, "bisector": s.ms_bisector},
             "vertices": v, "vertices_only_in_plain": int(len(np.setdiff1d(plain_keys, keys))),
             "vertices_only_in_learned": int(len(np.setdiff1d(keys, plain_keys))),
-            "empty_sphere_violations": int(cnt.value), "precision": prec}
+            "identical_vertex_rows": same,
+            "empty_sphere_violations": int(cnt.value), "precision": prec,
+            "completion": _completion_stats(s) if (tuning or {}).get("complete") else None}
 
 
 def c5_evaluation(params, plain_tuning, learned_tuning, say, n=5_000_000):
@@ -277,13 +334,27 @@ def c5_evaluation(params, plain_tuning, learned_tuning, say, n=5_000_000):
     s = _lib.Stats()
     _lib.check(ctx, L.apo_get_stats(ctx, _lib.byref(s)))
     _, ids = _vertices(ctx, L, _lib, int(s.vertices))
+    cnt = _lib.c_uint64()
+    _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
+    plain_violations = int(cnt.value)
+    plain_precision = _precision(ctx)
     L.apo_destroy(ctx)
-    say(f"C5 plain: {ms_plain:.1f} ms for {n} sites")
+    say(f"C5 plain: {ms_plain:.1f} ms for {n} sites, {plain_violations} empty-sphere violations")
     lr = run_learned(params, dev, n, learned_tuning, 1, 1, _row_keys(ids))
     say(f"C5 learned: {lr['ms_per_step']:.1f} ms, fallback rate {lr['fallback_rate']:.3f}")
     lr["plain_cells_per_s"] = n / (ms_plain / 1000.0)
     lr["speedup_vs_plain"] = lr["cells_per_s"] / lr["plain_cells_per_s"]
+    lr["plain_empty_sphere_violations"] = plain_violations
+    lr["plain_precision"] = plain_precision
     lr["workload"] = f"{n} sites, centres U[0,1]^3, radii log-uniform [1e-4,1e-2], seed 55"
+    # completion mode on the same set: plain, then learned against its rows
+    comp, crows, ckeys = completion_section(dev, n, plain_tuning, 1, 1, say)
+    lcc = run_learned(params, dev, n, dict(learned_tuning, complete=4), 1, 1, ckeys, plain_rows=crows)
+    lcc["speedup_vs_plain"] = lcc["cells_per_s"] / comp["eager"]["cells_per_s"]
+    lr["complete"] = comp
+    lr["learned_complete"] = lcc
+    say(f"C5 completion: plain {comp['eager']['ms_per_step']:.1f} ms, {comp['empty_sphere_violations']} violations; "
+        f"learned {lcc['ms_per_step']:.1f} ms, identical rows {lcc['identical_vertex_rows']}")
     del dev
     torch.cuda.empty_cache()
     return lr
@@ -381,10 +452,19 @@ def cpu_baseline(sites: np.ndarray, target: int):
     d = oracle.Diagram(sub, threads=threads)
     dt = time.perf_counter() - t0
     v = d.stats()["vertices"]
-    return {"value": len(sub) / dt, "unit": "cells/s", "cores": threads, "kind": "port",
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": len(sub) / dt, "unit": "cells/s", "cores": threads, "kind": "port", "cpu_model": model,
             "sample": f"{len(sub)} sites of the workload inside [0,{edge:.4f}]^3 (same density and radii), "
                       f"full construction by the oracle with the reference's edge searches, {dt:.2f} s, "
-                      f"{v} vertices"}
+                      f"{v} vertices, {threads} threads on {model}"}
 
 
 def main():
@@ -576,6 +656,20 @@ def main():
         ex["note"] = "classes of the exact certified-k labels of the same sites instead of the MLP's"
         out["learned_exact_labels"] = ex
         say(f"learned {lc['ms_per_step']:.1f} ms, exact-label classes {ex['ms_per_step']:.1f} ms")
+        # completion mode: plain and learned (MLP, then exact-label classes)
+        # must emit identical vertex rows
+        say("completion mode: plain, learned and exact-label builds")
+        out["complete"], crows, ckeys = completion_section(dev, n, plain_tuning, args.steps, args.warmup, say)
+        ltc = dict(ltune, complete=4)
+        lcc = run_learned(params, dev, n, ltc, args.steps, args.warmup, ckeys, plain_rows=crows)
+        lcc["speedup_vs_plain"] = lcc["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
+        out["learned_complete"] = lcc
+        exc = run_learned(params, dev, n, ltc, args.steps, args.warmup, ckeys, classes=prop.label_class(lab),
+                          plain_rows=crows)
+        exc["speedup_vs_plain"] = exc["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
+        out["learned_exact_labels_complete"] = exc
+        say(f"completion: learned {lcc['ms_per_step']:.1f} ms (identical rows {lcc['identical_vertex_rows']}), "
+            f"exact labels {exc['ms_per_step']:.1f} ms (identical rows {exc['identical_vertex_rows']})")
         if not args.no_c5:
             say("C5: the proposal on a 5M-site set")
             out["learned_5m"] = c5_evaluation(params, plain_tuning, ltune, say)

@@ -1353,7 +1353,10 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
         uint32_t flags = 0;
         for (Edg& e : cell.E)
             if (e.status != 1) e.status = 2;
-        std::vector<Vtx> Vloc;
+        // the reference updates a vertex list that starts empty
+        // (algorithm.inl:526-527); the completion mode the whole list
+        std::vector<Vtx> Vown;
+        std::vector<Vtx>& Vloc = C.cfg.complete ? cell.V : Vown;
         bool didChange = false;
         const uint32_t stepNb = uint32_t(cell.E.size()) / kWave + 1;
         uint32_t es = 0, restarts = 0;
@@ -1430,7 +1433,7 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
             }
         }
         C.status[i] = didChange ? 0 : 1;
-        for (const Vtx& v : Vloc) cell.V.push_back(v);
+        for (const Vtx& v : Vown) cell.V.push_back(v);
         if (cell.V.size() > C.cfg.vertex_capacity) {
             flags |= kFlagVertexOverflow;
             cell.V.resize(C.cfg.vertex_capacity);
@@ -1463,7 +1466,8 @@ void stageBisectorValidation(Context& C, uint64_t& changedCells) {
         }
         const F3 fci{float(si.x), float(si.y), float(si.z)};
         const float fri = float(si.w);
-        std::vector<Vtx> Vloc;
+        std::vector<Vtx> Vown;
+        std::vector<Vtx>& Vloc = C.cfg.complete ? cell.V : Vown;
         bool didChange = false;
         const uint32_t stepNb = uint32_t(bis.size()) / kWave + 1;
         uint32_t bsi = 0, restarts = 0;
@@ -1508,7 +1512,7 @@ void stageBisectorValidation(Context& C, uint64_t& changedCells) {
         }
         C.status[i] = didChange ? 0 : 1;
         if (didChange) changedCount++;
-        for (const Vtx& v : Vloc) cell.V.push_back(v);
+        for (const Vtx& v : Vown) cell.V.push_back(v);
         if (cell.V.size() > C.cfg.vertex_capacity) {
             flags |= kFlagVertexOverflow;
             cell.V.resize(C.cfg.vertex_capacity);

@@ -101,6 +101,7 @@ SIGNATURES = {
     "apo_proposal_labels": (ctypes.c_int, [_vp, _f32p, _u32, _f32p, _u8p]),
     "apo_proposal_train": (ctypes.c_int, [_vp, _f32p, _u8p, ctypes.c_uint64, _u32, _u32, ctypes.c_float, _u32,
                                           _f32p, _f32p, _f32p]),
+    "apo_proposal_gradient": (ctypes.c_int, [_vp, _f32p, _f32p, _u8p, _u32, _f32p, _f32p]),
     "apo_proposal_set": (ctypes.c_int, [_vp, _f32p]),
     "apo_proposal_infer": (ctypes.c_int, [_vp, _f32p, _u32, _u8p, _f32p]),
     "apo_proposal_classes": (ctypes.c_int, [_vp, _u8p]),

@@ -1612,26 +1612,27 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
         b = ((b + 0x7fffu + ((b >> 16) & 1u)) >> 16) << 16;
         std::memcpy(&wq[q], &b, 4);
     }
-    DeviceTemp<float> dF, dW, dWq, dG, dM, dV, dLoss, dX, dH1, dH2, dZ, dDz, dD2, dD1;
-    DeviceTemp<uint8_t> dY, dYb;
+    const uint32_t maxWaves = trainWaves(batch < m ? batch : m);
+    DeviceTemp<float> dF, dW, dWq, dM, dV, dPart, dLossPart, dLoss, dB1, dB2, dB3;
+    DeviceTemp<uint16_t> dW1, dW2, dW3;
+    DeviceTemp<uint8_t> dY;
     DeviceTemp<uint32_t> dOrder;
     APO_HIP(dF.alloc(f.size()));
     APO_HIP(dY.alloc(m));
     APO_HIP(dOrder.alloc(m));
     APO_HIP(dW.alloc(kProposalParams));
     APO_HIP(dWq.alloc(kProposalParams));
-    APO_HIP(dG.alloc(kProposalParams));
     APO_HIP(dM.alloc(kProposalParams));
     APO_HIP(dV.alloc(kProposalParams));
+    APO_HIP(dPart.alloc(size_t(maxWaves) * kProposalParams));
+    APO_HIP(dLossPart.alloc(maxWaves));
     APO_HIP(dLoss.alloc(1));
-    APO_HIP(dX.alloc(size_t(batch) * kFeatures));
-    APO_HIP(dYb.alloc(batch));
-    APO_HIP(dH1.alloc(size_t(batch) * kHidden));
-    APO_HIP(dH2.alloc(size_t(batch) * kHidden));
-    APO_HIP(dZ.alloc(size_t(batch) * kProposalClasses));
-    APO_HIP(dDz.alloc(size_t(batch) * kProposalClasses));
-    APO_HIP(dD2.alloc(size_t(batch) * kHidden));
-    APO_HIP(dD1.alloc(size_t(batch) * kHidden));
+    APO_HIP(dW1.alloc(size_t(kHidden) * kHidden));
+    APO_HIP(dW2.alloc(size_t(kHidden) * kHidden));
+    APO_HIP(dW3.alloc(size_t(16) * kHidden));
+    APO_HIP(dB1.alloc(kHidden));
+    APO_HIP(dB2.alloc(kHidden));
+    APO_HIP(dB3.alloc(16));
     APO_HIP(hipMemcpyAsync(dF.p, f.data(), f.size() * 4, hipMemcpyHostToDevice, st));
     APO_HIP(hipMemcpyAsync(dY.p, y.data(), m, hipMemcpyHostToDevice, st));
     APO_HIP(hipMemcpyAsync(dW.p, w.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
@@ -1639,7 +1640,9 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
     std::vector<float> zeros(kProposalParams, 0.f);
     APO_HIP(hipMemcpyAsync(dM.p, zeros.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
     APO_HIP(hipMemcpyAsync(dV.p, zeros.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
-    TrainBuffers tb{dX.p, dYb.p, dH1.p, dH2.p, dZ.p, dDz.p, dD2.p, dD1.p, dW.p, dWq.p, dG.p, dM.p, dV.p, dLoss.p};
+    const TrainBuffers tb{dF.p, dY.p, dW.p, dWq.p, dM.p, dV.p, dPart.p, dLossPart.p, dLoss.p, maxWaves,
+                          ProposalWeights{dW1.p, dW2.p, dW3.p, dB1.p, dB2.p, dB3.p}};
+    APO_HIP(packWeightsLaunch(dW.p, tb.pw, st));
     std::vector<uint32_t> order(m);
     for (uint32_t q = 0; q < m; q++) order[q] = q;
     std::mt19937 shuffler(seed ^ 0x9e3779b9u);
@@ -1647,12 +1650,10 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
     for (uint32_t ep = 0; ep < epochs; ep++) {
         std::shuffle(order.begin(), order.end(), shuffler);
         APO_HIP(hipMemcpyAsync(dOrder.p, order.data(), size_t(m) * 4, hipMemcpyHostToDevice, st));
-        const float zero = 0.f;
-        APO_HIP(hipMemcpyAsync(dLoss.p, &zero, 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(dLoss.p), 1u, 0u);
         for (uint32_t b0 = 0; b0 < m; b0 += batch) {
             const uint32_t rows = (m - b0) < batch ? (m - b0) : batch;
-            APO_HIP(gatherLaunch(dF.p, dY.p, dOrder.p + b0, rows, dX.p, dYb.p, st));
-            APO_HIP(mlpTrainStep(tb, rows, lr, step++, st));
+            APO_HIP(mlpTrainStep(tb, dOrder.p + b0, rows, lr, step++, st));
         }
         float loss = 0.f;
         APO_HIP(hipMemcpyAsync(&loss, dLoss.p, 4, hipMemcpyDeviceToHost, st));
@@ -1661,6 +1662,67 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
     }
     APO_HIP(hipMemcpyAsync(params, dW.p, kProposalParams * 4, hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+
+apo_status apo_proposal_gradient(apo_ctx* ctx, const float* params, const float* features, const uint8_t* classes,
+                                 uint32_t count, float* grad, float* loss) {
+    if (!ctx || !params || !features || !classes || !grad || count == 0) return APO_ERR_INVALID;
+    for (uint32_t q = 0; q < count; q++)
+        if (classes[q] >= kProposalClasses) return fail(ctx, APO_ERR_INVALID, "class out of range");
+    APO_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    std::vector<float> wq(kProposalParams);
+    for (uint32_t q = 0; q < kProposalParams; q++) {
+        uint32_t b;
+        std::memcpy(&b, &params[q], 4);
+        b = ((b + 0x7fffu + ((b >> 16) & 1u)) >> 16) << 16;
+        std::memcpy(&wq[q], &b, 4);
+    }
+    std::vector<uint32_t> order(count);
+    for (uint32_t q = 0; q < count; q++) order[q] = q;
+    const uint32_t waves = trainWaves(count);
+    DeviceTemp<float> dF, dW, dWq, dPart, dLossPart, dB1, dB2, dB3;
+    DeviceTemp<uint16_t> dW1, dW2, dW3;
+    DeviceTemp<uint8_t> dY;
+    DeviceTemp<uint32_t> dOrder;
+    APO_HIP(dF.alloc(size_t(count) * kFeatures));
+    APO_HIP(dY.alloc(count));
+    APO_HIP(dOrder.alloc(count));
+    APO_HIP(dW.alloc(kProposalParams));
+    APO_HIP(dWq.alloc(kProposalParams));
+    APO_HIP(dPart.alloc(size_t(waves) * kProposalParams));
+    APO_HIP(dLossPart.alloc(waves));
+    APO_HIP(dW1.alloc(size_t(kHidden) * kHidden));
+    APO_HIP(dW2.alloc(size_t(kHidden) * kHidden));
+    APO_HIP(dW3.alloc(size_t(16) * kHidden));
+    APO_HIP(dB1.alloc(kHidden));
+    APO_HIP(dB2.alloc(kHidden));
+    APO_HIP(dB3.alloc(16));
+    APO_HIP(hipMemcpyAsync(dF.p, features, size_t(count) * kFeatures * 4, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dY.p, classes, count, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dOrder.p, order.data(), size_t(count) * 4, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dW.p, params, kProposalParams * 4, hipMemcpyHostToDevice, st));
+    APO_HIP(hipMemcpyAsync(dWq.p, wq.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
+    const TrainBuffers tb{dF.p, dY.p, dW.p, dWq.p, nullptr, nullptr, dPart.p, dLossPart.p, nullptr, waves,
+                          ProposalWeights{dW1.p, dW2.p, dW3.p, dB1.p, dB2.p, dB3.p}};
+    APO_HIP(packWeightsLaunch(dW.p, tb.pw, st));
+    APO_HIP(trainGradLaunch(tb, dOrder.p, count, 1.f / float(count), st));
+    std::vector<float> part(size_t(waves) * kProposalParams), lp(waves);
+    APO_HIP(hipMemcpyAsync(part.data(), dPart.p, part.size() * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipMemcpyAsync(lp.data(), dLossPart.p, waves * 4, hipMemcpyDeviceToHost, st));
+    APO_HIP(hipStreamSynchronize(st));
+    // the partials summed in the order the Adam kernel sums them
+    for (uint32_t q = 0; q < kProposalParams; q++) {
+        float g = 0.f;
+        for (uint32_t wv = 0; wv < waves; wv++) g += part[size_t(wv) * kProposalParams + q];
+        grad[q] = g;
+    }
+    if (loss) {
+        float l = 0.f;
+        for (uint32_t wv = 0; wv < waves; wv++) l += lp[wv];
+        *loss = l / float(count);
+    }
     return APO_OK;
 }
 

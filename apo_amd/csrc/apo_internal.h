@@ -309,11 +309,18 @@ struct ProposalWeights {
     uint16_t *w1, *w2, *w3;  // BF16, transposed [out][64]
     float *b1, *b2, *b3;
 };
+// Training state: all features and classes on the device, per-wave partial
+// gradients and losses, master weights (w), their BF16-rounded copy (wq,
+// read by the backward pass), Adam moments and the packed forward weights.
 struct TrainBuffers {
-    float* x;
-    uint8_t* y;
-    float *h1, *h2, *z, *dz, *d2, *d1;
-    float *w, *wq, *grad, *m, *v, *loss;
+    const float* feat;
+    const uint8_t* cls;
+    float *w, *wq, *m, *v;
+    float* partial;      // maxWaves x kProposalParams
+    float* lossPartial;  // maxWaves
+    float* epochLoss;
+    uint32_t maxWaves;
+    ProposalWeights pw;
 };
 hipError_t cellsKLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, const uint8_t* cls, uint8_t* labels,
                         uint32_t mode, uint32_t fallbackK, hipStream_t st);
@@ -326,9 +333,14 @@ hipError_t gatherClassesLaunch(const uint32_t* indices, const uint8_t* given, ui
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
                           float cost, uint32_t fallbackK, uint32_t maxK, hipStream_t st);
 hipError_t packWeightsLaunch(const float* params, const ProposalWeights& w, hipStream_t st);
-hipError_t mlpTrainStep(const TrainBuffers& t, uint32_t rows, float lr, uint32_t step, hipStream_t st);
-hipError_t gatherLaunch(const float* feat, const uint8_t* labels, const uint32_t* order, uint32_t rows, float* x,
-                        uint8_t* y, hipStream_t st);
+// Waves (and partial gradients) of a batch of `rows`.
+uint32_t trainWaves(uint32_t rows);
+// Partial gradients of the mean loss over rows order[0..rows) (scaled by
+// invBatch) with the current forward and backward weights.
+hipError_t trainGradLaunch(const TrainBuffers& t, const uint32_t* order, uint32_t rows, float invBatch,
+                           hipStream_t st);
+hipError_t mlpTrainStep(const TrainBuffers& t, const uint32_t* order, uint32_t rows, float lr, uint32_t step,
+                        hipStream_t st);
 
 // Topology (topology.hip)
 hipError_t topologyCountLaunch(const CellCtx& c, uint32_t* counts, hipStream_t st);

@@ -263,84 +263,170 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
 }
 
 // ---------------------------------------------------------------------------
-// Training (FP32).  Parameters, flat: W1 [52][64], b1 [64], W2 [64][64],
-// b2 [64], W3 [64][8], b3 [8] (weights [in][out]).
+// Training on the matrix cores.  Parameters, flat: W1 [52][64], b1 [64],
+// W2 [64][64], b2 [64], W3 [64][8], b3 [8] (weights [in][out]).
+//
+// The forward pass is the inference pass above (BF16 weights and
+// activations, FP32 accumulation on v_mfma_f32_16x16x16_bf16), so the
+// network trained is the network that runs.  The backward pass runs in
+// FP32 on v_mfma_f32_16x16x4_f32 (lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; D as above) against the BF16-rounded weights the
+// forward pass used.  Each wave takes kTrainTiles tiles of 16 rows of the
+// batch and writes its partial gradient (and loss) once; the Adam kernel
+// sums the partials, so no atomics and the sum order is fixed.
 // ---------------------------------------------------------------------------
-// C[M][N] (+)= sum_k A(m, k) B(k, n); A(m, k) = ta ? A[k * lda + m] : A[m * lda + k]
-__global__ void k_gemm(const float* A, uint32_t lda, uint32_t ta, const float* B, uint32_t ldb, uint32_t tb,
-                       float* C, uint32_t ldc, uint32_t M, uint32_t N, uint32_t K, float beta) {
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= M * N) return;
-    const uint32_t m = idx / N, nn = idx % N;
-    float acc = 0.f;
-    for (uint32_t k = 0; k < K; k++) {
-        const float a = ta ? A[size_t(k) * lda + m] : A[size_t(m) * lda + k];
-        const float b = tb ? B[size_t(nn) * ldb + k] : B[size_t(k) * ldb + nn];
-        acc += a * b;
+constexpr uint32_t kTrainTiles = 8;
+constexpr uint32_t kOffB1 = kFeatures * kHidden, kOffW2 = kOffB1 + kHidden, kOffB2 = kOffW2 + kHidden * kHidden,
+                   kOffW3 = kOffB2 + kHidden, kOffB3 = kOffW3 + kHidden * kProposalClasses;
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(64) void k_train_grad(const float* __restrict__ feat, const uint8_t* __restrict__ cls,
+                                                   const uint32_t* __restrict__ order, uint32_t rows,
+                                                   float invBatch, const uint16_t* w1t, const float* b1,
+                                                   const uint16_t* w2t, const float* b2, const uint16_t* w3t,
+                                                   const float* b3, const float* __restrict__ wq,
+                                                   float* __restrict__ partial, float* __restrict__ lossPartial) {
+    __shared__ uint16_t X[16][kHidden];
+    __shared__ uint16_t H1[16][kHidden];
+    __shared__ uint16_t H2[16][kHidden];
+    __shared__ float Z[16][16];    // logits, then dLoss/dlogits
+    __shared__ float D2[16][kHidden];
+    __shared__ float D1[16][kHidden];
+    const uint32_t lane = laneId();
+    const uint32_t lr16 = lane & 15u, lq = lane >> 4;
+    const float* W2 = wq + kOffW2;
+    const float* W3 = wq + kOffW3;
+    f32x4 g1[4][4], g2[4][4], g3[4];
+    for (int rb = 0; rb < 4; rb++) {
+        g3[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int cb = 0; cb < 4; cb++) g1[rb][cb] = g2[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
-    float* c = C + size_t(m) * ldc + nn;
-    *c = beta != 0.f ? beta * *c + acc : acc;
-}
-
-// rows of x[batch][cols] += bias, optional ReLU
-__global__ void k_bias_act(float* x, const float* bias, uint32_t rows, uint32_t cols, uint32_t relu) {
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * cols) return;
-    float v = x[idx] + bias[idx % cols];
-    if (relu) v = fmaxf(v, 0.f);
-    x[idx] = v;
-}
-
-// softmax cross-entropy: grad = (softmax - onehot) / batch, loss summed
-__global__ void k_softmax_xent(const float* logits, const uint8_t* labels, uint32_t rows, float* grad, float* loss) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const float* z = logits + size_t(r) * kProposalClasses;
-    float m = z[0];
-    for (uint32_t q = 1; q < kProposalClasses; q++) m = fmaxf(m, z[q]);
-    float s = 0.f;
-    for (uint32_t q = 0; q < kProposalClasses; q++) s += __expf(z[q] - m);
-    const uint32_t y = labels[r];
-    for (uint32_t q = 0; q < kProposalClasses; q++) {
-        const float p = __expf(z[q] - m) / s;
-        grad[size_t(r) * kProposalClasses + q] = (p - (q == y ? 1.f : 0.f)) / float(rows);
+    float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, loss = 0.f;
+    const uint32_t first = blockIdx.x * kTrainTiles * 16u;
+    for (uint32_t t = 0; t < kTrainTiles; t++) {
+        const uint32_t base = first + t * 16u;
+        if (base >= rows) break;
+        for (uint32_t idx = lane; idx < 16u * kHidden; idx += 64u) {
+            const uint32_t r = idx / kHidden, col = idx % kHidden;
+            const uint32_t row = base + r;
+            const float v = (col < kFeatures && row < rows) ? feat[size_t(order[row]) * kFeatures + col] : 0.f;
+            X[r][col] = toBf16(v);
+        }
+        waveSync();
+        mlpLayer<kHidden / 16u, true>(X, w1t, b1, H1, Z);
+        waveSync();
+        mlpLayer<kHidden / 16u, true>(H1, w2t, b2, H2, Z);
+        waveSync();
+        mlpLayer<1, false>(H2, w3t, b3, H1, Z);
+        waveSync();
+        // softmax cross-entropy of the batch mean: dL/dz = (p - onehot) / batch
+        if (lane < 16u) {
+            const uint32_t row = base + lane;
+            float z[kProposalClasses];
+            for (uint32_t q = 0; q < kProposalClasses; q++) z[q] = Z[lane][q];
+            if (row < rows) {
+                const uint32_t y = cls[order[row]];
+                float m = z[0];
+                for (uint32_t q = 1; q < kProposalClasses; q++) m = fmaxf(m, z[q]);
+                float e[kProposalClasses], sum = 0.f;
+                for (uint32_t q = 0; q < kProposalClasses; q++) {
+                    e[q] = expf(z[q] - m);
+                    sum += e[q];
+                }
+                for (uint32_t q = 0; q < kProposalClasses; q++)
+                    Z[lane][q] = (e[q] / sum - (q == y ? 1.f : 0.f)) * invBatch;
+                loss += (m + logf(sum)) - z[y < kProposalClasses ? y : 0u];
+            } else {
+                for (uint32_t q = 0; q < kProposalClasses; q++) Z[lane][q] = 0.f;
+            }
+            for (uint32_t q = kProposalClasses; q < 16u; q++) Z[lane][q] = 0.f;
+        }
+        waveSync();
+        // dH2 = dZ W3^T through the ReLU
+        for (uint32_t cb = 0; cb < 4u; cb++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            const uint32_t i = cb * 16u + lr16;
+            for (uint32_t ks = 0; ks < kProposalClasses / 4u; ks++) {
+                const uint32_t o = ks * 4u + lq;
+                acc = mfma4(Z[lr16][o], W3[i * kProposalClasses + o], acc);
+            }
+            for (int e = 0; e < 4; e++) {
+                const uint32_t r = lq * 4u + e;
+                D2[r][i] = fromBf16(H2[r][i]) > 0.f ? acc[e] : 0.f;
+            }
+        }
+        waveSync();
+        // dH1 = dH2 W2^T through the ReLU
+        for (uint32_t cb = 0; cb < 4u; cb++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            const uint32_t i = cb * 16u + lr16;
+            for (uint32_t ks = 0; ks < kHidden / 4u; ks++) {
+                const uint32_t j = ks * 4u + lq;
+                acc = mfma4(D2[lr16][j], W2[i * kHidden + j], acc);
+            }
+            for (int e = 0; e < 4; e++) {
+                const uint32_t r = lq * 4u + e;
+                D1[r][i] = fromBf16(H1[r][i]) > 0.f ? acc[e] : 0.f;
+            }
+        }
+        waveSync();
+        // weight gradients: G[in][out] += sum over rows of act[row][in] * d[row][out]
+        for (uint32_t ks = 0; ks < 4u; ks++) {
+            const uint32_t r = ks * 4u + lq;
+            for (uint32_t rb = 0; rb < 4u; rb++) {
+                const uint32_t in = rb * 16u + lr16;
+                const float a1 = fromBf16(X[r][in]), a2 = fromBf16(H1[r][in]), a3 = fromBf16(H2[r][in]);
+                g3[rb] = mfma4(a3, Z[r][lr16], g3[rb]);
+                for (uint32_t cb = 0; cb < 4u; cb++) {
+                    const uint32_t out = cb * 16u + lr16;
+                    g2[rb][cb] = mfma4(a2, D2[r][out], g2[rb][cb]);
+                    g1[rb][cb] = mfma4(a1, D1[r][out], g1[rb][cb]);
+                }
+            }
+        }
+        for (uint32_t r = 0; r < 16u; r++) {
+            gb1 += D1[r][lane];
+            gb2 += D2[r][lane];
+            if (lane < kProposalClasses) gb3 += Z[r][lane];
+        }
+        waveSync();
     }
-    atomicAdd(loss, (m + __logf(s)) - z[y]);
+    float* p = partial + size_t(blockIdx.x) * kProposalParams;
+    for (uint32_t rb = 0; rb < 4u; rb++)
+        for (int e = 0; e < 4; e++) {
+            const uint32_t in = rb * 16u + lq * 4u + e;
+            for (uint32_t cb = 0; cb < 4u; cb++) {
+                const uint32_t out = cb * 16u + lr16;
+                if (in < kFeatures) p[in * kHidden + out] = g1[rb][cb][e];
+                p[kOffW2 + in * kHidden + out] = g2[rb][cb][e];
+            }
+            if (lr16 < kProposalClasses) p[kOffW3 + in * kProposalClasses + lr16] = g3[rb][e];
+        }
+    p[kOffB1 + lane] = gb1;
+    p[kOffB2 + lane] = gb2;
+    if (lane < kProposalClasses) p[kOffB3 + lane] = gb3;
+    float tot = loss;
+    for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o, 64);
+    if (lane == 0) lossPartial[blockIdx.x] = tot;
 }
 
-// d *= (h > 0)
-__global__ void k_relu_back(float* d, const float* h, uint32_t count) {
+// Adam on the FP32 master weights with the gradient summed over the waves'
+// partials, the BF16-rounded copy the backward pass reads, and the epoch's
+// summed loss
+__global__ void k_adam(float* w, const float* __restrict__ partial, uint32_t waves, float* m, float* v, float* wq,
+                       float lr, float b1t, float b2t, const float* __restrict__ lossPartial, float* epochLoss) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < count && !(h[idx] > 0.f)) d[idx] = 0.f;
-}
-
-// g[col] = sum over rows of d[row][col]
-__global__ void k_col_sum(const float* d, uint32_t rows, uint32_t cols, float* g) {
-    const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= cols) return;
-    float s = 0.f;
-    for (uint32_t r = 0; r < rows; r++) s += d[size_t(r) * cols + col];
-    g[col] = s;
-}
-
-// gather batch rows of features and labels
-__global__ void k_gather(const float* feat, const uint8_t* labels, const uint32_t* order, uint32_t rows, float* x,
-                         uint8_t* y) {
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * kFeatures) return;
-    const uint32_t r = idx / kFeatures, col = idx % kFeatures;
-    const uint32_t src = order[r];
-    x[idx] = feat[size_t(src) * kFeatures + col];
-    if (col == 0) y[r] = labels[src];
-}
-
-// Adam step on the FP32 master weights, then the BF16-rounded copy the
-// forward pass uses
-__global__ void k_adam(float* w, const float* g, float* m, float* v, float* wq, uint32_t count, float lr, float b1t,
-                       float b2t) {
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= count) return;
-    const float gi = g[idx];
+    if (idx == 0) {
+        float l = 0.f;
+        for (uint32_t q = 0; q < waves; q++) l += lossPartial[q];
+        *epochLoss += l;
+    }
+    if (idx >= kProposalParams) return;
+    float gi = 0.f;
+    for (uint32_t q = 0; q < waves; q++) gi += partial[size_t(q) * kProposalParams + idx];
     const float mi = 0.9f * m[idx] + 0.1f * gi;
     const float vi = 0.999f * v[idx] + 0.001f * gi * gi;
     m[idx] = mi;
@@ -422,62 +508,26 @@ hipError_t packWeightsLaunch(const float* params, const ProposalWeights& w, hipS
     return hipGetLastError();
 }
 
-static inline dim3 blocksFor(size_t count) { return dim3(uint32_t((count + 255u) / 256u)); }
+uint32_t trainWaves(uint32_t rows) { return (rows + kTrainTiles * 16u - 1u) / (kTrainTiles * 16u); }
 
-// One training step on a gathered batch; returns the summed loss in *loss.
-hipError_t mlpTrainStep(const TrainBuffers& t, uint32_t rows, float lr, uint32_t step, hipStream_t st) {
-    const float* W1 = t.wq;
-    const float* B1 = W1 + kFeatures * kHidden;
-    const float* W2 = B1 + kHidden;
-    const float* B2 = W2 + kHidden * kHidden;
-    const float* W3 = B2 + kHidden;
-    const float* B3 = W3 + kHidden * kProposalClasses;
-    float* gW1 = t.grad;
-    float* gB1 = gW1 + kFeatures * kHidden;
-    float* gW2 = gB1 + kHidden;
-    float* gB2 = gW2 + kHidden * kHidden;
-    float* gW3 = gB2 + kHidden;
-    float* gB3 = gW3 + kHidden * kProposalClasses;
-    // forward
-    hipLaunchKernelGGL(k_gemm, blocksFor(rows * kHidden), dim3(256), 0, st, t.x, kFeatures, 0u, W1, kHidden, 0u, t.h1,
-                       kHidden, rows, kHidden, kFeatures, 0.f);
-    hipLaunchKernelGGL(k_bias_act, blocksFor(rows * kHidden), dim3(256), 0, st, t.h1, B1, rows, kHidden, 1u);
-    hipLaunchKernelGGL(k_gemm, blocksFor(rows * kHidden), dim3(256), 0, st, t.h1, kHidden, 0u, W2, kHidden, 0u, t.h2,
-                       kHidden, rows, kHidden, kHidden, 0.f);
-    hipLaunchKernelGGL(k_bias_act, blocksFor(rows * kHidden), dim3(256), 0, st, t.h2, B2, rows, kHidden, 1u);
-    hipLaunchKernelGGL(k_gemm, blocksFor(rows * kProposalClasses), dim3(256), 0, st, t.h2, kHidden, 0u, W3,
-                       kProposalClasses, 0u, t.z, kProposalClasses, rows, kProposalClasses, kHidden, 0.f);
-    hipLaunchKernelGGL(k_bias_act, blocksFor(rows * kProposalClasses), dim3(256), 0, st, t.z, B3, rows,
-                       kProposalClasses, 0u);
-    hipLaunchKernelGGL(k_softmax_xent, blocksFor(rows), dim3(256), 0, st, t.z, t.y, rows, t.dz, t.loss);
-    // backward
-    hipLaunchKernelGGL(k_gemm, blocksFor(kHidden * kProposalClasses), dim3(256), 0, st, t.h2, kHidden, 1u, t.dz,
-                       kProposalClasses, 0u, gW3, kProposalClasses, kHidden, kProposalClasses, rows, 0.f);
-    hipLaunchKernelGGL(k_col_sum, blocksFor(kProposalClasses), dim3(256), 0, st, t.dz, rows, kProposalClasses, gB3);
-    hipLaunchKernelGGL(k_gemm, blocksFor(rows * kHidden), dim3(256), 0, st, t.dz, kProposalClasses, 0u, W3,
-                       kProposalClasses, 1u, t.d2, kHidden, rows, kHidden, kProposalClasses, 0.f);
-    hipLaunchKernelGGL(k_relu_back, blocksFor(rows * kHidden), dim3(256), 0, st, t.d2, t.h2, rows * kHidden);
-    hipLaunchKernelGGL(k_gemm, blocksFor(kHidden * kHidden), dim3(256), 0, st, t.h1, kHidden, 1u, t.d2, kHidden, 0u,
-                       gW2, kHidden, kHidden, kHidden, rows, 0.f);
-    hipLaunchKernelGGL(k_col_sum, blocksFor(kHidden), dim3(256), 0, st, t.d2, rows, kHidden, gB2);
-    hipLaunchKernelGGL(k_gemm, blocksFor(rows * kHidden), dim3(256), 0, st, t.d2, kHidden, 0u, W2, kHidden, 1u, t.d1,
-                       kHidden, rows, kHidden, kHidden, 0.f);
-    hipLaunchKernelGGL(k_relu_back, blocksFor(rows * kHidden), dim3(256), 0, st, t.d1, t.h1, rows * kHidden);
-    hipLaunchKernelGGL(k_gemm, blocksFor(kFeatures * kHidden), dim3(256), 0, st, t.x, kFeatures, 1u, t.d1, kHidden, 0u,
-                       gW1, kHidden, kFeatures, kHidden, rows, 0.f);
-    hipLaunchKernelGGL(k_col_sum, blocksFor(kHidden), dim3(256), 0, st, t.d1, rows, kHidden, gB1);
-    // update
-    const float b1t = __builtin_powf(0.9f, float(step + 1)), b2t = __builtin_powf(0.999f, float(step + 1));
-    hipLaunchKernelGGL(k_adam, blocksFor(kProposalParams), dim3(256), 0, st, t.w, t.grad, t.m, t.v, t.wq,
-                       kProposalParams, lr, b1t, b2t);
+hipError_t trainGradLaunch(const TrainBuffers& t, const uint32_t* order, uint32_t rows, float invBatch,
+                           hipStream_t st) {
+    const uint32_t waves = trainWaves(rows);
+    if (waves > t.maxWaves) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_train_grad, dim3(waves), dim3(64), 0, st, t.feat, t.cls, order, rows, invBatch, t.pw.w1,
+                       t.pw.b1, t.pw.w2, t.pw.b2, t.pw.w3, t.pw.b3, t.wq, t.partial, t.lossPartial);
     return hipGetLastError();
 }
 
-hipError_t gatherLaunch(const float* feat, const uint8_t* labels, const uint32_t* order, uint32_t rows, float* x,
-                        uint8_t* y, hipStream_t st) {
-    hipLaunchKernelGGL(k_gather, blocksFor(size_t(rows) * kFeatures), dim3(256), 0, st, feat, labels, order, rows, x,
-                       y);
-    return hipGetLastError();
+// One step: the batch's gradient, Adam, and the forward weights of the next step.
+hipError_t mlpTrainStep(const TrainBuffers& t, const uint32_t* order, uint32_t rows, float lr, uint32_t step,
+                        hipStream_t st) {
+    hipError_t e = trainGradLaunch(t, order, rows, 1.f / float(rows), st);
+    if (e != hipSuccess) return e;
+    const float b1t = __builtin_powf(0.9f, float(step + 1)), b2t = __builtin_powf(0.999f, float(step + 1));
+    hipLaunchKernelGGL(k_adam, dim3((kProposalParams + 255u) / 256u), dim3(256), 0, st, t.w, t.partial,
+                       trainWaves(rows), t.m, t.v, t.wq, lr, b1t, b2t, t.lossPartial, t.epochLoss);
+    return packWeightsLaunch(t.w, t.pw, st);
 }
 
 }  // namespace apo_hip

@@ -75,6 +75,23 @@ def train(features: np.ndarray, labels_: np.ndarray, epochs: int = 8, batch: int
     return params, loss
 
 
+def gradient(params: np.ndarray, features: np.ndarray, classes: np.ndarray):
+    """(gradient (8072,), mean loss) of the softmax cross-entropy over the
+    rows at `params`, from the training kernels."""
+    p = np.ascontiguousarray(params, np.float32)
+    f = np.ascontiguousarray(features, np.float32)
+    c = np.ascontiguousarray(classes, np.uint8)
+    g = np.zeros(PARAMS, np.float32)
+    loss = np.zeros(1, np.float32)
+    ctx = _ctx()
+    try:
+        _lib.check(ctx, _lib.lib().apo_proposal_gradient(ctx, _lib.fptr(p), _lib.fptr(f), _lib.u8ptr(c), len(c),
+                                                          _lib.fptr(g), _lib.fptr(loss)))
+    finally:
+        _lib.lib().apo_destroy(ctx)
+    return g, float(loss[0])
+
+
 def infer(params: np.ndarray, features: np.ndarray):
     """Classes and logits of feature rows through the matrix-core inference."""
     p = np.ascontiguousarray(params, np.float32)

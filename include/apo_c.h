@@ -293,13 +293,21 @@ apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_
  * 255: buried), plus its 52 features; both in input order. */
 apo_status apo_proposal_labels(apo_ctx* ctx, const float* xyzr, uint32_t n, float* features, uint8_t* labels);
 /* Train the MLP on (features, labels) from apo_proposal_labels with HIP
- * kernels (FP32, BF16-rounded forward weights, Adam, softmax cross-entropy
- * over the 8 classes).  init: starting parameters or NULL (seeded uniform);
+ * kernels on the matrix cores (the inference pass forward, BF16 weights and
+ * activations with FP32 accumulation; an FP32 backward pass; Adam on FP32
+ * master weights; softmax cross-entropy over the 8 classes).  init: starting parameters or NULL (seeded uniform);
  * params receives APO_PROPOSAL_PARAMS floats, epoch_loss (may be NULL) the
  * mean loss of each epoch. */
 apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t* labels, uint64_t count,
                               uint32_t epochs, uint32_t batch, float lr, uint32_t seed, const float* init,
                               float* params, float* epoch_loss);
+/* Gradient of the mean softmax cross-entropy over `count` feature rows with
+ * classes 0..7, at parameters `params`, computed by the training kernels
+ * (BF16 forward pass as in inference, FP32 backward pass on the matrix
+ * cores); grad receives APO_PROPOSAL_PARAMS floats, loss (may be NULL) the
+ * mean loss.  For checking the training kernels. */
+apo_status apo_proposal_gradient(apo_ctx* ctx, const float* params, const float* features, const uint8_t* classes,
+                                 uint32_t count, float* grad, float* loss);
 /* Use these parameters in the following builds (NULL: the plain path). */
 apo_status apo_proposal_set(apo_ctx* ctx, const float* params);
 /* Classes (and optionally the 8 logits) the set MLP gives to count feature rows. */

@@ -6,6 +6,12 @@
   accumulation) agrees with a float64 numpy evaluation of the same
   BF16-rounded network: logits within 2e-2 and the same class wherever the
   top two logits differ by more than that.
+* One batch's gradient from the training kernels (BF16 forward pass as in
+  inference, FP32 backward pass on the matrix cores) agrees with a float64
+  numpy backward pass of the same BF16-rounded network to 1e-4 relative
+  (Frobenius norm per tensor); the loss to 1e-4.
+* The 52 features of every site equal a numpy computation from the sites
+  and their 16 nearest neighbours (the kNN lists of a build).
 * Training lowers the loss and beats the majority class.
 * A build with the proposal equals, record for record, the oracle run with
   the same per-site classes, and has no empty-sphere violation; so does a
@@ -62,6 +68,104 @@ def _numpy_logits(p, x):
     a = _bf16(np.maximum(a @ _bf16(W1) + b1, 0.0))
     a = _bf16(np.maximum(a @ _bf16(W2) + b2, 0.0))
     return a @ _bf16(W3) + b3
+
+
+def _split(p):
+    f, h = proposal.FEATURES, 64
+    o, out = 0, []
+    for shape in ((f, h), (h,), (h, h), (h,), (h, 8), (8,)):
+        size = int(np.prod(shape))
+        out.append(p[o:o + size].reshape(shape))
+        o += size
+    return out
+
+
+def _numpy_gradient(p, x, y):
+    """Float64 backward pass of the network the training kernels run:
+    BF16-rounded weights, features and hidden activations."""
+    W1, b1, W2, b2, W3, b3 = _split(p)
+    W1, W2, W3 = _bf16(W1), _bf16(W2), _bf16(W3)
+    a0 = _bf16(x)
+    h1 = _bf16(np.maximum(a0 @ W1 + b1, 0.0))
+    h2 = _bf16(np.maximum(h1 @ W2 + b2, 0.0))
+    z = h2 @ W3 + b3
+    z = z - z.max(axis=1, keepdims=True)
+    e = np.exp(z)
+    prob = e / e.sum(axis=1, keepdims=True)
+    n = len(y)
+    loss = float(np.mean(-np.log(prob[np.arange(n), y])))
+    dz = prob.copy()
+    dz[np.arange(n), y] -= 1.0
+    dz /= n
+    d2 = (dz @ W3.T) * (h2 > 0)
+    d1 = (d2 @ W2.T) * (h1 > 0)
+    grads = [a0.T @ d1, d1.sum(0), h1.T @ d2, d2.sum(0), h2.T @ dz, dz.sum(0)]
+    return grads, loss
+
+
+def test_training_gradient_matches_float64(training):
+    sites, feat, lab = training
+    rng = np.random.default_rng(3)
+    params = rng.normal(0.0, 0.2, proposal.PARAMS).astype(np.float32)
+    keep = np.nonzero(lab != proposal.LABEL_BURIED)[0][:4000]  # 250 tiles of 16 rows, the last one partial
+    x = feat[keep][:3990]
+    y = proposal.label_class(lab[keep][:3990])
+    g, loss = proposal.gradient(params, x, y)
+    ref, ref_loss = _numpy_gradient(params.astype(np.float64), x.astype(np.float64), y.astype(np.int64))
+    _say(f"[grad] loss {loss:.6f} (float64 {ref_loss:.6f})")
+    assert abs(loss - ref_loss) <= 1e-4 * abs(ref_loss)
+    for name, got, want in zip(("W1", "b1", "W2", "b2", "W3", "b3"), _split(g.astype(np.float64)), ref):
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        _say(f"[grad] {name}: relative error {err:.2e}")
+        assert err < 1e-4, name
+
+
+def test_features_match_numpy(training):
+    sites, feat, lab = training
+    g = AlgorithmGPU(sites, k=16, stage_mask=8).build()  # the LBVH and the 16-NN lists only
+    try:
+        s, perm = g.sorted_sites()
+        knn = g.knns()
+    finally:
+        g.close()
+    n = len(s)
+    want = np.zeros((n, proposal.FEATURES), np.float32)
+    f32 = np.float32
+    for i in range(n):
+        si = s[i]
+        nb = knn[i]
+        d16 = f32(0)
+        for t in range(15, -1, -1):
+            if nb[t] < n:
+                dv = s[nb[t], :3] - si[:3]
+                d16 = np.sqrt((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2])
+                break
+        inv = f32(1) / d16 if d16 > 0 else f32(0)
+        row = want[i]
+        row[0] = si[3] * inv
+        larger, rsum, have = f32(0), f32(0), 0
+        gmin, gmax = f32(np.finfo(np.float32).max), f32(-np.finfo(np.float32).max)
+        for t in range(16):
+            j = nb[t]
+            if j >= n:
+                continue
+            o = s[j]
+            dv = o[:3] - si[:3]
+            d = np.sqrt((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2])
+            gg = ((d - si[3]) - o[3]) * inv
+            row[1 + 3 * t], row[2 + 3 * t], row[3 + 3 * t] = d * inv, o[3] * inv, gg
+            larger += f32(1) if o[3] > si[3] else f32(0)
+            rsum += o[3]
+            gmin, gmax = min(gmin, gg), max(gmax, gg)
+            have += 1
+        row[49] = larger / f32(have) if have else 0
+        lr = f32(0)
+        if have and rsum > 0 and si[3] > 0:
+            lr = f32(np.log2(si[3] / (rsum / f32(have)))) * f32(0.125)
+        row[50] = min(f32(1), max(f32(-1), lr))
+        row[51] = gmax - gmin if have else 0
+    got = feat[perm]  # input order -> sorted order
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-6)
 
 
 def test_mfma_inference_matches_numpy(training):

@@ -109,9 +109,20 @@ def infer(params: np.ndarray, features: np.ndarray):
 
 
 def load(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
+    """Parameters from a raw float32 file (no pickle).  A JSON header beside
+    it (same name, .json) records the training recipe and a SHA-256 of the
+    parameters, which is checked."""
     p = np.fromfile(path, dtype="<f4")
     if p.size != PARAMS:
         raise ValueError(f"{path}: expected {PARAMS} float32 parameters, found {p.size}")
+    header = os.path.splitext(path)[0] + ".json"
+    if os.path.exists(header):
+        import hashlib
+        import json
+        with open(header) as f:
+            want = json.load(f).get("sha256")
+        if want and hashlib.sha256(p.tobytes()).hexdigest() != want:
+            raise ValueError(f"{path}: parameters do not match the checksum in {header}")
     return p
 
 

@@ -241,21 +241,21 @@ def completion_section(dev, n, plain_tuning, steps, warmup, say):
 
 
 def train_proposal(say):
-    """Labels from the plain method on two 1M log-uniform sets (seeds
-    501/502: 2M labelled sites) and 8 epochs of the HIP training kernels."""
-    from apo_amd import datasets, proposal
-    t0 = time.perf_counter()
-    fa, la = proposal.labels(datasets.loguniform_sites(1_000_000, seed=501))
-    fb, lb = proposal.labels(datasets.loguniform_sites(1_000_000, seed=502))
-    t_labels = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    params, loss = proposal.train(np.concatenate([fa, fb]), np.concatenate([la, lb]), epochs=8, batch=4096,
-                                  lr=2e-3, seed=7)
-    t_train = time.perf_counter() - t0
-    say(f"proposal trained on 2M labelled sites: labels {t_labels:.1f} s, training {t_train:.1f} s, "
-        f"loss {loss[0]:.3f} -> {loss[-1]:.3f}")
-    return params, {"labelled_sites": int(len(la) + len(lb)), "epochs": 8, "loss_first_epoch": float(loss[0]),
-                    "loss_last_epoch": float(loss[-1]), "label_s": t_labels, "train_s": t_train}
+    """The committed proposal (apo_amd/models/proposal_c5.f32), and its recipe
+    run again: labels from the plain method on two 1M log-uniform sets
+    (seeds 501/502: 2M labelled sites) and 8 epochs of the HIP training
+    kernels, timed; the training is deterministic, so the parameters must
+    come out bit for bit as committed."""
+    from apo_amd import proposal
+    from apo_amd.models import train_c5
+    committed = proposal.load()
+    params, loss, t_labels, t_train = train_c5.train()
+    same = bool(np.array_equal(params.view(np.uint32), committed.view(np.uint32)))
+    say(f"proposal trained on 2M labelled sites: labels {t_labels:.1f} s, training {t_train:.2f} s, "
+        f"loss {loss[0]:.3f} -> {loss[-1]:.3f}, equal to the committed weights: {same}")
+    return committed, {"labelled_sites": 2_000_000, "epochs": 8, "loss_first_epoch": float(loss[0]),
+                       "loss_last_epoch": float(loss[-1]), "label_s": t_labels, "train_s": t_train,
+                       "weights": os.path.relpath(proposal.DEFAULT_WEIGHTS, ROOT), "retrained_equals_committed": same}
 
 
 def _precision(ctx):

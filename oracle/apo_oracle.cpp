@@ -621,6 +621,7 @@ struct NewEdge { uint32_t k; bool two; bool closed; };
 struct Context {
     uint32_t n = 0;
     orc_config cfg{};
+
     std::vector<D4> S;   // sorted sites + 6 artificial, FP64
     std::vector<F4> Sf;  // float copies (toFloat)
     const Lbvh* bvh = nullptr;
@@ -631,6 +632,13 @@ struct Context {
     std::vector<uint32_t> flags;
     std::vector<uint32_t> siteK;  // per-site neighbour count of the proposal path (empty: all K)
 };
+
+// Steps of the validation loops: 64 records, contributors applied lowest
+// lane first (the wave64 order the HIP kernels use), or with lanes = 32 the
+// reference's own order, 32-lane warps visited highest lane first (forEach,
+// apo/include/apo/gpu/utils.cuh:17-32).
+inline int waveWidth(const Context& C) { return C.cfg.lanes == 32 ? 32 : kWave; }
+inline int laneOrder(const Context& C, int t, int count) { return C.cfg.lanes == 32 ? count - 1 - t : t; }
 
 // Neighbours asked for by each class of the learned proposal (mirrors the
 // product's mapping; class 7, not certified within 64, runs `fallback`).
@@ -1308,12 +1316,13 @@ void stageVertexValidation(Context& C, const std::vector<uint32_t>& tasks) {
         const D4 si = C.S[i];
         uint32_t flags = 0;
         uint32_t vs = 0, restarts = 0;
-        while (vs < uint32_t(cell.V.size()) / kWave + 1 && restarts < C.cfg.max_restarts) {
-            const uint32_t base = vs * kWave;
+        const int W = waveWidth(C);
+        while (vs < uint32_t(cell.V.size()) / uint32_t(W) + 1 && restarts < C.cfg.max_restarts) {
+            const uint32_t base = vs * uint32_t(W);
             vs++;
             uint32_t invalid[kWave];
             int nInvalid = 0;
-            for (int lane = 0; lane < kWave; lane++) {
+            for (int lane = 0; lane < W; lane++) {
                 const uint32_t vid = base + uint32_t(lane);
                 if (vid >= cell.V.size() || cell.V[vid].status != 0) continue;
                 const D4 cur = vertexPosition(C, si, cell.V[vid]);
@@ -1325,8 +1334,8 @@ void stageVertexValidation(Context& C, const std::vector<uint32_t>& tasks) {
                 cell.V[vid].status = 1;
                 if (nb != kNone && intersectD(cur, C.S[nb])) invalid[nInvalid++] = nb;
             }
-            for (int t2 = 0; t2 < nInvalid; t2++) {
-                const uint32_t j = invalid[t2];
+            for (int t3 = 0; t3 < nInvalid; t3++) {
+                const uint32_t j = invalid[laneOrder(C, t3, nInvalid)];
                 if (known(cell, j)) continue;
                 updateCell(C, i, si, cell.V, cell, j, C.S[j], flags);
             }
@@ -1358,13 +1367,14 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
         std::vector<Vtx> Vown;
         std::vector<Vtx>& Vloc = C.cfg.complete ? cell.V : Vown;
         bool didChange = false;
-        const uint32_t stepNb = uint32_t(cell.E.size()) / kWave + 1;
+        const int W = waveWidth(C);
+        const uint32_t stepNb = uint32_t(cell.E.size()) / uint32_t(W) + 1;
         uint32_t es = 0, restarts = 0;
         while (es < stepNb) {
             uint32_t found[kWave];
             int nFound = 0;
-            for (int lane = 0; lane < kWave; lane++) {
-                const uint32_t eid = es * kWave + uint32_t(lane);
+            for (int lane = 0; lane < W; lane++) {
+                const uint32_t eid = es * uint32_t(W) + uint32_t(lane);
                 const bool inRange = eid < cell.E.size();
                 bool needSearch = inRange && cell.E[eid].status == 2;
                 uint32_t k = kNone, l = kNone;
@@ -1417,8 +1427,8 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
                 else found[nFound++] = c;
             }
             bool changed = false;
-            for (int t2 = 0; t2 < nFound; t2++) {
-                const uint32_t j = found[t2];
+            for (int t3 = 0; t3 < nFound; t3++) {
+                const uint32_t j = found[laneOrder(C, t3, nFound)];
                 if (known(cell, j)) continue;
                 const bool ch = updateCell(C, i, si, Vloc, cell, j, C.S[j], flags);
                 didChange = didChange || ch;
@@ -1469,13 +1479,14 @@ void stageBisectorValidation(Context& C, uint64_t& changedCells) {
         std::vector<Vtx> Vown;
         std::vector<Vtx>& Vloc = C.cfg.complete ? cell.V : Vown;
         bool didChange = false;
-        const uint32_t stepNb = uint32_t(bis.size()) / kWave + 1;
+        const int W = waveWidth(C);
+        const uint32_t stepNb = uint32_t(bis.size()) / uint32_t(W) + 1;
         uint32_t bsi = 0, restarts = 0;
         while (bsi < stepNb) {
             uint32_t found[kWave];
             int nFound = 0;
-            for (int lane = 0; lane < kWave; lane++) {
-                const uint32_t bid = bsi * kWave + uint32_t(lane);
+            for (int lane = 0; lane < W; lane++) {
+                const uint32_t bid = bsi * uint32_t(W) + uint32_t(lane);
                 if (bid >= bis.size() || bis[bid] == kNone) continue;
                 const uint32_t bk = bis[bid];
                 const D4 sk = C.S[bk];
@@ -1495,8 +1506,8 @@ void stageBisectorValidation(Context& C, uint64_t& changedCells) {
                 else found[nFound++] = c;
             }
             bool changed = false;
-            for (int t2 = 0; t2 < nFound; t2++) {
-                const uint32_t j = found[t2];
+            for (int t3 = 0; t3 < nFound; t3++) {
+                const uint32_t j = found[laneOrder(C, t3, nFound)];
                 if (known(cell, j)) continue;
                 const bool ch = updateCell(C, i, si, Vloc, cell, j, C.S[j], flags);
                 didChange = didChange || ch;
@@ -1790,6 +1801,7 @@ void orc_default_config(orc_config* cfg) {
     cfg->fallback_k = 16;
     cfg->knn_passes = 1;
     cfg->complete = 0;
+    cfg->lanes = 64;
 }
 
 static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {

@@ -88,6 +88,9 @@ typedef struct orc_config {
                                    consistency rounds: a cell holding fewer records of a quadruple (i,j,k,l)
                                    than one of its other three cells is updated with the smallest quadruple
                                    site it does not know, and the cells that changed are validated again */
+    uint32_t lanes;             /* order of the validation loops: 64 (default) steps of 64 records with
+                                   contributors applied lowest lane first, the HIP kernels' order; 32 the
+                                   reference's 32-lane warps visited highest lane first (utils.cuh:17-32) */
 } orc_config;
 
 void orc_default_config(orc_config* cfg);

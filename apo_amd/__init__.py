@@ -66,7 +66,7 @@ class AlgorithmGPU:
     def __init__(self, sites, k: int = 16, max_iterations: int = 10, max_restarts: int = 100,
                  stage_mask: int = 7, waves_per_cu: int = 0, device: int = 0, proposal=None,
                  proposal_cost: float = 0.0, offsets=None, graph: bool = False, classes=None,
-                 knn_validation_step: int = 1, **tuning):
+                 knn_validation_step: int = 1, radii=None, **tuning):
         self.sites = np.ascontiguousarray(np.asarray(sites, dtype=np.float32).reshape(-1, 4))
         self.offsets = None if offsets is None else np.ascontiguousarray(offsets, np.uint32)
         self.k = int(k)
@@ -92,6 +92,11 @@ class AlgorithmGPU:
         if proposal is not None:
             self._proposal = np.ascontiguousarray(proposal, np.float32)
             _lib.check(self._ctx, _lib.lib().apo_proposal_set(self._ctx, _lib.fptr(self._proposal)))
+        if radii is not None:
+            # per-site search radii (input order): range-query cell pass
+            self._radii = np.ascontiguousarray(radii, np.float32)
+            _lib.check(self._ctx, _lib.lib().apo_proposal_set_radii(self._ctx, _lib.fptr(self._radii),
+                                                                    len(self._radii)))
         if classes is not None:
             # per-site proposal classes (input order) instead of the MLP's
             self._classes = np.ascontiguousarray(classes, np.uint8)
@@ -259,6 +264,13 @@ class AlgorithmGPU:
         _lib.check(self._ctx, _lib.lib().apo_get_all_cells(self._ctx, _lib.uptr(V), _lib.uptr(E)))
         return c, V, E
 
+
+    def cell_radii(self) -> np.ndarray:
+        """Per-site cell radius of the last build (input order, FP32 rounded
+        up): twice it is the smallest search radius that certifies the cell."""
+        out = np.zeros(len(self.sites), np.float32)
+        _lib.check(self._ctx, _lib.lib().apo_cell_radii(self._ctx, _lib.fptr(out)))
+        return out
 
     def proposal_classes(self) -> np.ndarray:
         """Classes the learned proposal gave each site in the last build."""

@@ -106,6 +106,8 @@ SIGNATURES = {
     "apo_proposal_infer": (ctypes.c_int, [_vp, _f32p, _u32, _u8p, _f32p]),
     "apo_proposal_classes": (ctypes.c_int, [_vp, _u8p]),
     "apo_proposal_set_classes": (ctypes.c_int, [_vp, _u8p, _u32]),
+    "apo_proposal_set_radii": (ctypes.c_int, [_vp, _f32p, _u32]),
+    "apo_cell_radii": (ctypes.c_int, [_vp, _f32p]),
 }
 
 

@@ -34,11 +34,12 @@ constexpr uint32_t kTimedPasses = 32;
 struct GraphKey {
     const void* sites = nullptr;
     uint32_t n = 0, molecules = 0;
-    bool learned = false, given = false;
+    bool learned = false, given = false, ranged = false;
     uint64_t outCap = 0;
     apo_config cfg{};
     bool operator==(const GraphKey& o) const {
         return sites == o.sites && n == o.n && molecules == o.molecules && learned == o.learned && given == o.given &&
+               ranged == o.ranged &&
                outCap == o.outCap && std::memcmp(&cfg, &o.cfg, sizeof(cfg)) == 0;
     }
 };
@@ -156,6 +157,10 @@ struct apo_ctx {
     uint32_t capP = 0;          // site capacity of feat / cls / labels
     uint32_t* want = nullptr;     // completion mode: smallest wanted site per cell (kNone: none)
     uint32_t capW = 0;
+    float* givenRadii = nullptr;  // search radii given by the caller (input order): range-query cell pass
+    uint32_t givenRN = 0;
+    uint32_t givenRCap = 0;
+    float* radiiSorted = nullptr;  // the same in sorted order, capacity capP
     uint8_t* givenCls = nullptr;  // classes given by the caller (input order), used instead of the MLP
     uint32_t givenN = 0;          // their site count (0: none given)
     uint32_t givenCap = 0;
@@ -274,13 +279,14 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
 }
 
 // The learned path runs with MLP weights or with classes given by the caller.
-bool learnedPath(const apo_ctx* ctx) { return ctx->proposalOn || ctx->givenN != 0u; }
+bool learnedPath(const apo_ctx* ctx) { return ctx->proposalOn || ctx->givenN != 0u || ctx->givenRN != 0u; }
 
 apo_status ensureProposalBuffers(apo_ctx* ctx, uint32_t n) {
     if (n <= ctx->capP) return APO_OK;
     APO_HIP(alloc(ctx->feat, size_t(n) * kFeatures));
     APO_HIP(alloc(ctx->cls, n));
     APO_HIP(alloc(ctx->labels, n));
+    APO_HIP(alloc(ctx->radiiSorted, n));
     ctx->capP = n;
     return APO_OK;
 }
@@ -389,17 +395,27 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     // the learned path's lists hold the most neighbours any class may ask for
     const uint32_t Kk = learned ? proposalListK(ctx->cfg.proposal_max_k, ctx->cfg.proposal_fallback_k) : K;
     const uint32_t passes = learned ? 1u : ctx->cfg.knn_passes;
+    // search radii given by the caller: every list is a range query below its radius
+    const bool ranged = learned && ctx->givenRN != 0u;
+    if (ranged) APO_HIP(gatherRadiiLaunch(ctx->idxA, ctx->givenRadii, n, ctx->radiiSorted, st));
     if (passes == 0u) {
         // no kNN or cell pass (knnValidationStep = 0): empty lists, every cell
         // in validation; the cell launch below writes the octahedra
         APO_HIP(cellsResetLaunch(cc, ctx->knn, Kk, st));
     } else {
-        APO_HIP(knnLaunch(cc, Kk, ctx->knn, st, 0u));
+        APO_HIP(knnLaunch(cc, Kk, ctx->knn, st, 0u, ranged ? ctx->radiiSorted : nullptr));
     }
     APO_TIMED_STAGE("knn");
     APO_HIP(mark(2));
     if (ctx->cfg.stage_mask & 8u) return APO_OK;  // stop after the kNN pass
-    if (learned && (ctx->cfg.stage_mask & 1u)) {
+    if (ranged && (ctx->cfg.stage_mask & 1u)) {
+        // each cell updated with its whole range list, the known radius its
+        // range when the list holds fewer sites than its capacity
+        APO_HIP(mark(7));
+        APO_HIP(cellsLaunch(cc, ctx->knn, Kk, ctx->counters + kCellQueue,
+                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st, nullptr,
+                            ctx->cfg.proposal_fallback_k, false, ctx->radiiSorted));
+    } else if (learned && (ctx->cfg.stage_mask & 1u)) {
         // features of the 16 nearest, proposal classes, then each cell with its
         // proposed neighbour count and the plain certification test
         if (ctx->givenN) {
@@ -612,6 +628,7 @@ apo_status runPipeline(apo_ctx* ctx, const F4* sitesDev, uint32_t n) {
     key.cfg = ctx->cfg;
     key.learned = learned;
     key.given = ctx->givenN != 0u;
+    key.ranged = ctx->givenRN != 0u;
     key.outCap = ctx->outCap;
     if (!ctx->graphExec || !(key == ctx->graphKey)) {
         dropGraph(ctx);
@@ -770,7 +787,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
-                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->givenRadii, ctx->radiiSorted, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -815,6 +832,10 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     ctx->built = false;
     if (ctx->givenN && ctx->givenN != n)
         return fail(ctx, APO_ERR_INVALID, "proposal classes were given for another site count");
+    if (ctx->givenRN && ctx->givenRN != n)
+        return fail(ctx, APO_ERR_INVALID, "search radii were given for another site count");
+    if (ctx->givenRN && offsets && molecules > 1u)
+        return fail(ctx, APO_ERR_INVALID, "search radii apply to single builds");
     if (learnedPath(ctx) && ctx->cfg.knn_passes != 1u)
         return fail(ctx, APO_ERR_INVALID, "the learned proposal runs one kNN pass (knn_passes = 1)");
     ctx->cavValid = false;
@@ -1765,6 +1786,47 @@ apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t coun
     APO_HIP(hipMemcpyAsync(classes, dC.p, count, hipMemcpyDeviceToHost, st));
     if (logits) APO_HIP(hipMemcpyAsync(logits, dL.p, size_t(count) * kProposalClasses * 4, hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
+    return APO_OK;
+}
+
+apo_status apo_cell_radii(apo_ctx* ctx, float* radii) {
+    if (!ctx || !radii) return APO_ERR_INVALID;
+    if (!ctx->built || (ctx->cfg.stage_mask & 8u)) return fail(ctx, APO_ERR_STATE, "no complete build");
+    APO_HIP(hipSetDevice(ctx->device));
+    const uint32_t n = ctx->n;
+    DeviceTemp<float> d;
+    APO_HIP(d.alloc(n));
+    APO_HIP(cellRadiiLaunch(cellCtx(ctx), d.p, persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), ctx->stream));
+    std::vector<float> sorted(n);
+    std::vector<uint32_t> idx(n);
+    APO_HIP(hipMemcpyAsync(sorted.data(), d.p, size_t(n) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    APO_HIP(hipMemcpyAsync(idx.data(), ctx->idxA, size_t(n) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    APO_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = 0; i < n; i++) radii[idx[i]] = sorted[i];
+    return APO_OK;
+}
+
+apo_status apo_proposal_set_radii(apo_ctx* ctx, const float* radii, uint32_t n) {
+    if (!ctx) return APO_ERR_INVALID;
+    APO_HIP(hipSetDevice(ctx->device));
+    dropGraph(ctx);
+    if (!radii) {
+        ctx->givenRN = 0;
+        return APO_OK;
+    }
+    if (n < 2) return fail(ctx, APO_ERR_INVALID, "need radii for at least 2 sites");
+    for (uint32_t q = 0; q < n; q++)
+        if (!(radii[q] == radii[q])) return fail(ctx, APO_ERR_INVALID, "a search radius is NaN");
+    if (n > ctx->givenRCap) {
+        if (ctx->givenRadii) APO_HIP(hipFree(ctx->givenRadii));
+        ctx->givenRadii = nullptr;
+        ctx->givenRCap = 0;
+        APO_HIP(alloc(ctx->givenRadii, n));
+        ctx->givenRCap = n;
+    }
+    APO_HIP(hipMemcpyAsync(ctx->givenRadii, radii, size_t(n) * 4, hipMemcpyHostToDevice, ctx->stream));
+    APO_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->givenRN = n;
     return APO_OK;
 }
 

@@ -143,6 +143,21 @@ __device__ __forceinline__ bool boxReachesHalfSpace(const Box& b, F3 n, F3 p) {
     return ex + ey + ez < margin;
 }
 
+// Upper gap bound of a range query of radius b around a site of radius r.
+// A subtree box holding the site's centre has box gap -r, and the gap of a
+// site whose sphere holds that centre is below -r, so the box gap bounds
+// such sites' gaps from above, not below: a bound at or below -r would
+// prune every box around the centre.  The bound is therefore raised to the
+// float just above -r; every site whose gap is below it is then in reach
+// (box gaps bound the gaps of the other sites from below), the site that
+// buries this one (gap < -2 r) included.
+__device__ __host__ __forceinline__ float rangeTop(float b, float r) {
+    const float negR = -r;
+    const uint32_t bits = __builtin_bit_cast(uint32_t, negR);
+    const float above = negR < 0.f ? __builtin_bit_cast(float, bits - 1u) : __builtin_bit_cast(float, 1u);
+    return b > above ? b : above;
+}
+
 // ---------------------------------------------------------------------------
 // LBVH node: both child boxes live in the parent so one 64-byte line per
 // visited node carries everything the traversal needs.

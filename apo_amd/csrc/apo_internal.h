@@ -168,18 +168,26 @@ __device__ __forceinline__ void countBytes(const CellCtx& c, uint32_t family, ui
 
 // One kNN pass (k in 1..64 neighbours per site); pass > 0 continues the
 // lists of the previous pass in place (knnValidationStep, algorithm.inl:43-60).
-hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st, uint32_t pass = 0u);
+// bound (sorted order, may be null, pass 0 only): per-site upper bound on the
+// gap of the listed sites (a range query of at most k sites).
+hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st, uint32_t pass = 0u,
+                     const float* bound = nullptr);
 // No kNN pass: empty lists (n * k words), status in validation, flags clear.
 hipError_t cellsResetLaunch(const CellCtx& c, uint32_t* knn, uint32_t k, hipStream_t st);
 hipError_t cellsOccupancy(uint32_t* blocksPerCu);
+// cellRadius of every stored cell (sorted order), rounded up to FP32; -FLT_MAX when buried.
+hipError_t cellRadiiLaunch(const CellCtx& c, float* out, uint32_t blocks, hipStream_t st);
 // Persistent cell pass over all sites; queue = kXcdQueueWords heads.
 // cls (may be null): learned-proposal classes in sorted order; each cell then
 // runs min(proposalK(class, fallbackK), k) neighbours.  resume: the cells
 // still in validation continue from their stored records (a later kNN pass)
 // instead of the octahedron, and the others are left untouched.
+// bound (sorted order, may be null): the lists are range queries below these
+// gaps (knnLaunch with bound); a cell whose list is not full knows every site
+// below its bound, which then serves as its known radius.
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
                        hipStream_t st, const uint8_t* cls = nullptr, uint32_t fallbackK = 16u,
-                       bool resume = false);
+                       bool resume = false, const float* bound = nullptr);
 
 // Device counters of a build (words of one buffer): work-queue heads of the
 // persistent kernels (8 per queue, one range per XCD), the task count of
@@ -328,6 +336,8 @@ hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, flo
 // cls[i] = given[indices[i]]: caller-given classes (input order) in sorted order.
 hipError_t gatherClassesLaunch(const uint32_t* indices, const uint8_t* given, uint32_t n, uint8_t* cls,
                                hipStream_t st);
+// radii[i] = given[indices[i]]: caller-given search radii (input order) in sorted order.
+hipError_t gatherRadiiLaunch(const uint32_t* indices, const float* given, uint32_t n, float* radii, hipStream_t st);
 // Classes of n sites; classes asking for more than maxK neighbours are
 // replaced by the last class (not certified within maxK).
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,

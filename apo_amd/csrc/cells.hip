@@ -21,13 +21,24 @@
 
 namespace apo_hip {
 
+// Known radius of a range query of bound b that found fewer sites than its
+// list holds: every site whose FP32 gap is below rangeTop(b, r_i) was
+// listed, so every site whose exact gap is below it, less the FP32 rounding
+// of a gap (a margin of 2^-18 of the magnitudes involved), is known.
+__device__ __forceinline__ double rangeKnownRadius(const CellCtx& c, uint32_t i, float b, const D4& si) {
+    const double top = double(rangeTop(b, float(si.w)));
+    const double scale = __builtin_fabs(top) + __builtin_fabs(si.x) + __builtin_fabs(si.y) + __builtin_fabs(si.z) +
+                         __builtin_fabs(si.w);
+    return top - scale * 0x1p-18;
+}
+
 // getCells (algorithm.inl:101-206) for cell i: K updates in kNN order (the
 // first `use` of the K-long list), then the security-radius certification.
 // The first pass starts from the octahedron; a resumed pass (a later kNN
 // pass) continues the stored cell and leaves certified and buried cells as
 // they are.
 __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uint32_t* __restrict__ knn, uint32_t K,
-                                         uint32_t use, uint32_t i, bool resume) {
+                                         uint32_t use, uint32_t i, bool resume, const float* __restrict__ bound) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
     WaveCell w;
@@ -67,6 +78,7 @@ __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uin
             updateCell(c, L, w, i, si, j, sj);
             knownRadius = __builtin_fmax(knownRadius, sphereDistD(si, sj));
         }
+        if (bound && count < K) knownRadius = __builtin_fmax(knownRadius, rangeKnownRadius(c, i, bound[i], si));
         const double maxR = cellRadius(c, L, w, si);
         certified = gtD(knownRadius, 2.0 * maxR);
         kr = float(knownRadius);
@@ -89,7 +101,7 @@ __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uin
 // at most the list length K.
 __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
                                               uint32_t* queue, const uint8_t* __restrict__ cls, uint32_t fallbackK,
-                                              uint32_t resume) {
+                                              uint32_t resume, const float* __restrict__ bound) {
     __shared__ CellLds L;
     for (;;) {
         const uint32_t i = popWork(queue, c.n);
@@ -100,7 +112,7 @@ __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restr
             use = q < kProposalClasses ? proposalK(q, fallbackK) : K;
             if (use > K) use = K;
         }
-        cellTask(c, L, knn, K, use, i, resume != 0u);
+        cellTask(c, L, knn, K, use, i, resume != 0u, bound);
         waveSync();
     }
 }
@@ -123,6 +135,38 @@ hipError_t cellsResetLaunch(const CellCtx& c, uint32_t* knn, uint32_t k, hipStre
     return hipGetLastError();
 }
 
+// The double rounded up to a float (never below it).
+__device__ __forceinline__ float roundUpF(double d) {
+    float f = float(d);
+    if (double(f) < d) {
+        const uint32_t b = __builtin_bit_cast(uint32_t, f);
+        f = f == 0.f ? __builtin_bit_cast(float, 1u) : __builtin_bit_cast(float, f > 0.f ? b + 1u : b - 1u);
+    }
+    return f;
+}
+
+// cellRadius (topology_update.inl:595-653) of every stored cell, rounded up
+// to FP32; -FLT_MAX for buried cells.  One wave per cell.
+__global__ __launch_bounds__(64) void k_cell_radii(CellCtx c, float* __restrict__ out) {
+    __shared__ CellLds L;
+    for (uint32_t i = blockIdx.x; i < c.n; i += gridDim.x) {
+        const uint8_t st = c.status[i];
+        float r = -FLT_MAX;
+        if (st != kStatusBuried) {
+            WaveCell w;
+            loadCell(c, L, w, i, true);
+            r = roundUpF(cellRadius(c, L, w, loadSite(c, i)));
+        }
+        if (laneId() == 0) out[i] = r;
+        waveSync();
+    }
+}
+
+hipError_t cellRadiiLaunch(const CellCtx& c, float* out, uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_cell_radii, dim3(blocks), dim3(64), 0, st, c, out);
+    return hipGetLastError();
+}
+
 hipError_t cellsOccupancy(uint32_t* blocksPerCu) {
     int b = 0;
     const hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cells, 64, 0);
@@ -131,11 +175,11 @@ hipError_t cellsOccupancy(uint32_t* blocksPerCu) {
 }
 
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
-                       hipStream_t st, const uint8_t* cls, uint32_t fallbackK, bool resume) {
+                       hipStream_t st, const uint8_t* cls, uint32_t fallbackK, bool resume, const float* bound) {
     if (k > 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, queue, kXcdQueueWords, 0u);
     hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue, cls, fallbackK,
-                       resume ? 1u : 0u);
+                       resume ? 1u : 0u, bound);
     return hipGetLastError();
 }
 

@@ -19,8 +19,13 @@ namespace apo_hip {
 // One pass of getKnns for 64 sites.  KT bounds the list length k in LDS;
 // pass > 0 continues each list after the previous pass's last neighbour
 // (algorithm.inl:43-60): the lists are read and rewritten in place.
+// bound (sorted order, may be null; first pass only): the list of site i
+// holds only sites whose gap is below rangeTop(bound[i], r_i) (the heap
+// starts full of that gap instead of FLT_MAX, so the traversal also prunes
+// beyond it).
 template <uint32_t KT>
-__global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ knn, uint32_t K, uint32_t pass) {
+__global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ knn, uint32_t K, uint32_t pass,
+                                            const float* __restrict__ bound) {
     __shared__ uint32_t hKey[KT][64];
     __shared__ float hVal[KT][64];
     __shared__ uint32_t stNode[kStackDepth][64];
@@ -32,9 +37,11 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
     const uint32_t n = c.n;
     bool active = i < n;
 
+    float top = FLT_MAX;
+    if (bound && active) top = rangeTop(bound[i], c.sorted[i].w);
     for (uint32_t k = 0; k < K; k++) {
         hKey[k][lane] = kNone;
-        hVal[k][lane] = FLT_MAX;
+        hVal[k][lane] = top;
     }
     uint32_t overflow = 0;
     float startingDistance = -FLT_MAX;
@@ -174,14 +181,16 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
     }
 }
 
-hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st, uint32_t pass) {
+hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st, uint32_t pass,
+                     const float* bound) {
+    if (bound && pass) return hipErrorInvalidValue;
     const uint32_t blocks = (c.n + 63u) / 64u;
     if (k == 0 || k > 64) return hipErrorInvalidValue;
     // the smallest LDS list that holds k entries
-    if (k <= 8) hipLaunchKernelGGL(k_knn<8>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
-    else if (k <= 16) hipLaunchKernelGGL(k_knn<16>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
-    else if (k <= 32) hipLaunchKernelGGL(k_knn<32>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
-    else hipLaunchKernelGGL(k_knn<64>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass);
+    if (k <= 8) hipLaunchKernelGGL(k_knn<8>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass, bound);
+    else if (k <= 16) hipLaunchKernelGGL(k_knn<16>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass, bound);
+    else if (k <= 32) hipLaunchKernelGGL(k_knn<32>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass, bound);
+    else hipLaunchKernelGGL(k_knn<64>, dim3(blocks), dim3(64), 0, st, c, knn, k, pass, bound);
     return hipGetLastError();
 }
 

@@ -487,6 +487,18 @@ hipError_t gatherClassesLaunch(const uint32_t* indices, const uint8_t* given, ui
     return hipGetLastError();
 }
 
+// Per-site search radii given by the caller in input order, moved to sorted order.
+__global__ __launch_bounds__(256) void k_gather_radii(const uint32_t* __restrict__ indices,
+                                                      const float* __restrict__ given, uint32_t n, float* radii) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) radii[i] = given[indices[i]];
+}
+
+hipError_t gatherRadiiLaunch(const uint32_t* indices, const float* given, uint32_t n, float* radii, hipStream_t st) {
+    hipLaunchKernelGGL(k_gather_radii, dim3((n + 255u) / 256u), dim3(256), 0, st, indices, given, n, radii);
+    return hipGetLastError();
+}
+
 hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st) {
     if (K < 16) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_features, dim3((c.n + 255u) / 256u), dim3(256), 0, st, c, knn, K, feat);

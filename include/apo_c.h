@@ -320,6 +320,28 @@ apo_status apo_proposal_classes(apo_ctx* ctx, uint8_t* classes);
  * validation are unchanged.  NULL: back to the MLP (or the plain path). */
 apo_status apo_proposal_set_classes(apo_ctx* ctx, const uint8_t* classes, uint32_t n);
 
+/* Per-site search radii (input order) for the following builds of n sites,
+ * used instead of neighbour counts: each cell is updated with every site
+ * whose gap (|c_i - c_j| - r_i - r_j) is below its radius, nearest first, at
+ * most max(proposal_max_k, proposal_fallback_k) of them, through a range
+ * query over the LBVH; when the list holds fewer, every site below the
+ * radius is known and the radius (less the FP32 rounding of a gap) serves as
+ * the cell's known radius in the plain certification test
+ * (algorithm.inl:197-201).  Cells the test does not certify go through the
+ * plain validation.  A radius at or below -r_i is raised to the float just
+ * above -r_i (sites whose spheres hold c_i stay in reach, the one that
+ * buries site i among them).  Single builds only.  NULL: back to the MLP (or
+ * the plain path).  No counterpart in the reference. */
+apo_status apo_proposal_set_radii(apo_ctx* ctx, const float* radii, uint32_t n);
+
+/* Per-site cell radius of the last build (input order): the largest
+ * empty-sphere radius over the cell's vertices and the extreme points of its
+ * closed trisector edges (cellRadius, topology_update.inl:595-653), the
+ * quantity the certification test compares with half the known radius;
+ * -FLT_MAX for buried sites.  Twice it is the smallest search radius that
+ * certifies the final cell. */
+apo_status apo_cell_radii(apo_ctx* ctx, float* radii);
+
 /* ---- introspection of the device state (the public members dSites,
  * bvh, dStatus, dVertices, dEdges of AlgorithmGPU, algorithm.cuh:147-155) ---- */
 

@@ -75,6 +75,8 @@ def lib():
         L.orc_build.argtypes = [f32p, ctypes.c_uint32, ctypes.POINTER(OrcConfig)]
         L.orc_build_proposal.restype = ctypes.c_void_p
         L.orc_build_proposal.argtypes = [f32p, ctypes.c_uint32, ctypes.POINTER(OrcConfig), u8p]
+        L.orc_build_radii.restype = ctypes.c_void_p
+        L.orc_build_radii.argtypes = [f32p, ctypes.c_uint32, ctypes.POINTER(OrcConfig), f32p]
         L.orc_free.restype = None
         L.orc_free.argtypes = [ctypes.c_void_p]
         L.orc_vertex_count.restype = ctypes.c_uint64
@@ -237,12 +239,18 @@ def default_config(**kw) -> OrcConfig:
 class Diagram:
     """Result of the oracle's full construction."""
 
-    def __init__(self, xyzr, classes=None, **cfg):
-        """classes: per-site proposal classes (input order) for the learned path."""
+    def __init__(self, xyzr, classes=None, radii=None, **cfg):
+        """classes: per-site proposal classes (input order) for the learned
+        path; radii: per-site search radii (input order) for the range path."""
         self.sites = _sites(xyzr)
         self.n = len(self.sites)
         self.cfg = default_config(**cfg)
-        if classes is None:
+        if radii is not None:
+            self._radii = np.ascontiguousarray(radii, np.float32)
+            assert len(self._radii) == self.n
+            self._h = lib().orc_build_radii(_p(self.sites, f32p), self.n, ctypes.byref(self.cfg),
+                                            _p(self._radii, f32p))
+        elif classes is None:
             self._h = lib().orc_build(_p(self.sites, f32p), self.n, ctypes.byref(self.cfg))
         else:
             self._classes = np.ascontiguousarray(classes, np.uint8)

@@ -547,12 +547,13 @@ void heapsort(uint32_t* keys, float* vals, int size) {
 
 // kNN traversal (lbvh.inl:8-105). Returns max stack depth.
 int knnQuery(const Lbvh& b, const std::vector<F4>& sf, uint32_t i, uint32_t K, float startingDistance,
-             uint32_t* keys, float* vals) {
+             uint32_t* keys, float* vals, float top = FLT_MAX) {
     const uint32_t n = b.n;
     const F4 s = sf[i];
+    // top < FLT_MAX: a range query, only gaps below top are listed
     for (uint32_t k = 0; k < K; k++) {
         keys[k] = kNone;
-        vals[k] = FLT_MAX;
+        vals[k] = top;
     }
     struct Entry { uint32_t node; float d; };
     std::vector<Entry> stack;
@@ -631,7 +632,37 @@ struct Context {
     std::vector<Cell> cells;
     std::vector<uint32_t> flags;
     std::vector<uint32_t> siteK;  // per-site neighbour count of the proposal path (empty: all K)
+    std::vector<float> range;     // per-site search radius of the range path (empty: none)
 };
+
+// Upper gap bound of a range query of radius b around a site of radius r:
+// raised to the float just above -r, since a box holding the site's centre
+// has box gap -r while the sites whose spheres hold that centre have gaps
+// below -r (a lower bound there would prune every box around the centre).
+inline float rangeTop(float b, float r) {
+    const float negR = -r;
+    uint32_t bits;
+    std::memcpy(&bits, &negR, 4);
+    float above;
+    if (negR < 0.f) {
+        bits -= 1u;
+        std::memcpy(&above, &bits, 4);
+    } else {
+        const uint32_t one = 1u;
+        std::memcpy(&above, &one, 4);
+    }
+    return b > above ? b : above;
+}
+
+// Known radius of a range query of bound b that listed fewer sites than its
+// capacity: every site whose FP32 gap is below rangeTop(b, r_i) was listed,
+// so every site whose exact gap is below it, less a margin of 2^-18 of the
+// magnitudes involved (the FP32 rounding of a gap), is known.
+inline double rangeKnownRadius(float b, const D4& si) {
+    const double top = double(rangeTop(b, float(si.w)));
+    const double scale = std::fabs(top) + std::fabs(si.x) + std::fabs(si.y) + std::fabs(si.z) + std::fabs(si.w);
+    return top - scale * 0x1p-18;
+}
 
 // Steps of the validation loops: 64 records, contributors applied lowest
 // lane first (the wave64 order the HIP kernels use), or with lanes = 32 the
@@ -1290,6 +1321,8 @@ void stageCells(Context& C, const std::vector<uint32_t>& knn, uint32_t K, bool r
         double knownRadius = -DBL_MAX;
         uint32_t flags = 0;
         const uint32_t use = C.siteK.empty() ? K : std::min(K, C.siteK[i]);
+        uint32_t count = 0;
+        while (count < K && knn[size_t(i) * K + count] != kNone) count++;
         for (uint32_t nId = 0; nId < use; nId++) {
             const uint32_t j = knn[size_t(i) * K + nId];
             if (j == kNone) break;
@@ -1298,6 +1331,7 @@ void stageCells(Context& C, const std::vector<uint32_t>& knn, uint32_t K, bool r
             updateCell(C, i, si, cell.V, cell, j, sj, flags);
             knownRadius = std::fmax(knownRadius, sphereDistD(si, sj));
         }
+        if (!C.range.empty() && count < K) knownRadius = std::fmax(knownRadius, rangeKnownRadius(C.range[i], si));
         const double maxR = cellRadius(C, si, cell);
         if (gtD(knownRadius, 2.0 * maxR)) C.status[i] = kCertified;
         C.knownRadii[i] = float(knownRadius);
@@ -1804,7 +1838,8 @@ void orc_default_config(orc_config* cfg) {
     cfg->lanes = 64;
 }
 
-static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {
+static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes,
+                                 const float* radii = nullptr) {
     if (n < 2) return nullptr;
     orc_config cfg;
     if (cfgIn) cfg = *cfgIn;
@@ -1842,9 +1877,15 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
         cfg.k = 64;
         C.cfg.k = 64;
     }
+    if (radii) {
+        C.range.resize(n);
+        for (uint32_t i = 0; i < n; i++) C.range[i] = radii[d->bvh.indices[i]];
+        cfg.k = 64;
+        C.cfg.k = 64;
+    }
     const uint32_t K = cfg.k;
-    // the learned path runs one pass of its own lists
-    const uint32_t passes = classes ? 1u : cfg.knn_passes;
+    // the learned paths run one pass of their own lists
+    const uint32_t passes = (classes || radii) ? 1u : cfg.knn_passes;
 
     // getKnns + getCells, knnValidationStep times (algorithm.inl:32-206,
     // 1535-1549).  The lists start as 0xffffffff (the memset of :1526).
@@ -1867,7 +1908,8 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
                 }
                 startingDistance = sphereDistF(C.Sf[last], C.Sf[i]);
             }
-            const int depth = knnQuery(d->bvh, C.Sf, i, K, startingDistance, keys, &dist[size_t(i) * K]);
+            const float top = C.range.empty() ? FLT_MAX : rangeTop(C.range[i], C.Sf[i].w);
+            const int depth = knnQuery(d->bvh, C.Sf, i, K, startingDistance, keys, &dist[size_t(i) * K], top);
             if (depth > 32) C.flags[i] |= kFlagStackOverflow;
             if (pass > 0 || keys[0] == kNone) continue;
             // buried test against the nearest site, first pass only (:85-98)
@@ -1969,6 +2011,13 @@ orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfgIn) {
 // certification and validation.
 orc_diagram* orc_build_proposal(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes) {
     return buildDiagram(xyzr, n, cfgIn, classes);
+}
+
+// The range path with given search radii (input order): 64-long lists of the
+// sites whose gap is below the radius, each cell updated with its whole list,
+// and the radius as the known radius when the list is not full.
+orc_diagram* orc_build_radii(const float* xyzr, uint32_t n, const orc_config* cfgIn, const float* radii) {
+    return buildDiagram(xyzr, n, cfgIn, nullptr, radii);
 }
 
 void orc_free(orc_diagram* d) { delete d; }

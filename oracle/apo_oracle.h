@@ -102,6 +102,11 @@ orc_diagram* orc_build(const float* xyzr, uint32_t n, const orc_config* cfg);
  * lists, each cell runs the neighbour count of its class (20, 24, 28, 32,
  * 40, 48, 64; class 7: cfg->fallback_k) before certification and validation. */
 orc_diagram* orc_build_proposal(const float* xyzr, uint32_t n, const orc_config* cfg, const uint8_t* classes);
+/* The range path given per-site search radii (input order): 64-long lists of
+ * the sites whose gap is below max(radius, -2 r_i), nearest first; each cell
+ * runs its whole list and, when the list is not full, takes the radius (less
+ * the FP32 rounding of a gap) as its known radius. */
+orc_diagram* orc_build_radii(const float* xyzr, uint32_t n, const orc_config* cfg, const float* radii);
 void         orc_free(orc_diagram* d);
 
 /* Emitted Apollonius vertices (apo toHost semantics). */

@@ -229,6 +229,8 @@ def completion_section(dev, n, plain_tuning, steps, warmup, say):
     cnt = _lib.c_uint64()
     _lib.check(ctx, L.apo_check_empty_spheres(ctx, None, None, 0, 1e-9, _lib.byref(cnt)))
     prec = _precision(ctx)
+    cell_radii = np.zeros(n, np.float32)
+    _lib.check(ctx, L.apo_cell_radii(ctx, _lib.fptr(cell_radii)))
     L.apo_destroy(ctx)
     torch.cuda.synchronize()
     say(f"completion: {gr['ms_per_step']:.1f} ms per graph replay, {int(s.vertices)} vertices, "
@@ -237,7 +239,7 @@ def completion_section(dev, n, plain_tuning, steps, warmup, say):
              "eager": {"cells_per_s": n / (eager_ms / 1000.0), "ms_per_step": eager_ms},
              "empty_sphere_violations": int(cnt.value), "precision": prec, "completion": _completion_stats(s),
              "config": tuning, "path": "plain + completion, one HIP graph replay per step"},
-            _vertex_rows(xyzw, ids), _row_keys(ids))
+            _vertex_rows(xyzw, ids), _row_keys(ids), cell_radii)
 
 
 def train_proposal(say):
@@ -267,10 +269,11 @@ def _precision(ctx):
     return {name: int(getattr(r, name)) for name, _ in r._fields_}
 
 
-def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None, plain_rows=None):
+def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None, plain_rows=None, radii=None):
     """Time the workload with the proposal.  Cells the plain test does not
     certify after the proposed count go through the plain validation.
-    classes: per-site classes given instead of the MLP's.  plain_rows: the
+    classes: per-site classes given instead of the MLP's.  radii: per-site
+    search radii (range-query cell pass) instead of either.  plain_rows: the
     plain build's vertex rows (_vertex_rows) for an exact comparison."""
     import torch
     from apo_amd import _lib
@@ -278,6 +281,9 @@ def run_learned(params, dev, n, tuning, steps, warmup, plain_keys, classes=None,
     if classes is not None:
         cls = np.ascontiguousarray(classes, np.uint8)
         _lib.check(ctx, _lib.lib().apo_proposal_set_classes(ctx, _lib.u8ptr(cls), len(cls)))
+    if radii is not None:
+        rad = np.ascontiguousarray(radii, np.float32)
+        _lib.check(ctx, _lib.lib().apo_proposal_set_radii(ctx, _lib.fptr(rad), len(rad)))
     L = _lib.lib()
     ptr = _lib.ctypes.c_void_p(dev.data_ptr())
     for _ in range(warmup):
@@ -348,7 +354,7 @@ def c5_evaluation(params, plain_tuning, learned_tuning, say, n=5_000_000):
     lr["plain_precision"] = plain_precision
     lr["workload"] = f"{n} sites, centres U[0,1]^3, radii log-uniform [1e-4,1e-2], seed 55"
     # completion mode on the same set: plain, then learned against its rows
-    comp, crows, ckeys = completion_section(dev, n, plain_tuning, 1, 1, say)
+    comp, crows, ckeys, _ = completion_section(dev, n, plain_tuning, 1, 1, say)
     lcc = run_learned(params, dev, n, dict(learned_tuning, complete=4), 1, 1, ckeys, plain_rows=crows)
     lcc["speedup_vs_plain"] = lcc["cells_per_s"] / comp["eager"]["cells_per_s"]
     lr["complete"] = comp
@@ -659,7 +665,8 @@ def main():
         # completion mode: plain and learned (MLP, then exact-label classes)
         # must emit identical vertex rows
         say("completion mode: plain, learned and exact-label builds")
-        out["complete"], crows, ckeys = completion_section(dev, n, plain_tuning, args.steps, args.warmup, say)
+        out["complete"], crows, ckeys, cradii = completion_section(dev, n, plain_tuning, args.steps, args.warmup,
+                                                                   say)
         ltc = dict(ltune, complete=4)
         lcc = run_learned(params, dev, n, ltc, args.steps, args.warmup, ckeys, plain_rows=crows)
         lcc["speedup_vs_plain"] = lcc["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
@@ -668,8 +675,29 @@ def main():
                           plain_rows=crows)
         exc["speedup_vs_plain"] = exc["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
         out["learned_exact_labels_complete"] = exc
+        # the ceiling of a search-radius proposal: twice every cell's final
+        # radius (apo_cell_radii of the completed plain build), 1e-3 above,
+        # through the range-query cell pass
+        buried = cradii == np.float32(-np.finfo(np.float32).max)
+        exact_r = (2 * np.where(buried, np.float32(0), cradii) * np.float32(1 + 1e-3)).astype(np.float32)
+        exr = run_learned(None, dev, n, ltc, args.steps, args.warmup, ckeys, plain_rows=crows, radii=exact_r)
+        exr["speedup_vs_plain"] = exr["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
+        exr["speedup_vs_plain_without_completion"] = exr["cells_per_s"] / out["eager"]["cells_per_s"]
+        exr["note"] = ("search radius 2 x (1 + 1e-3) x the final cell radius of every site (range query, at most "
+                       f"{max(ltc.get('proposal_max_k', 64), ltc.get('proposal_fallback_k', 0))} sites per list)")
+        out["learned_exact_radius"] = exr
+        # the same radii with lists of up to 64 sites: more lists end below
+        # their radius, so more cells certify in the cell pass
+        t64 = dict(ltc, proposal_max_k=64, proposal_fallback_k=64)
+        ex64 = run_learned(None, dev, n, t64, args.steps, args.warmup, ckeys, plain_rows=crows, radii=exact_r)
+        ex64["speedup_vs_plain"] = ex64["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
+        ex64["note"] = "as learned_exact_radius with at most 64 sites per list"
+        out["learned_exact_radius_64"] = ex64
         say(f"completion: learned {lcc['ms_per_step']:.1f} ms (identical rows {lcc['identical_vertex_rows']}), "
-            f"exact labels {exc['ms_per_step']:.1f} ms (identical rows {exc['identical_vertex_rows']})")
+            f"exact labels {exc['ms_per_step']:.1f} ms (identical rows {exc['identical_vertex_rows']}), "
+            f"exact radii {exr['ms_per_step']:.1f} ms (identical rows {exr['identical_vertex_rows']}, "
+            f"{exr['cells_certified_by_cell_pass']} certified by the cell pass), with 64-site lists "
+            f"{ex64['ms_per_step']:.1f} ms ({ex64['cells_certified_by_cell_pass']} certified)")
         if not args.no_c5:
             say("C5: the proposal on a 5M-site set")
             out["learned_5m"] = c5_evaluation(params, plain_tuning, ltune, say)

@@ -35,11 +35,20 @@ constexpr int kCoverBalls = 8;       // balls per cover
 constexpr uint32_t kCoverRoots = 512;
 constexpr int kCoverRuns = kCoverBalls + 1;
 
-// Buffers of the cover construction; they overlay the traversal stacks,
-// which hold nothing while a cover is built.
+// Buffers of the cover construction in the wave's device region (covers
+// with more than kCoverLdsRoots roots).
 struct CoverLds {
     double roots[kCoverRoots];   // tangency roots, then the keep flags (bytes)
     double sorted[kCoverRoots];  // roots in ascending order
+};
+// Where a cover's roots live: in LDS (the cell's update scratch list, idle
+// between the neighbour rebuild and the searches) when 2F + 1 of them fit,
+// else in the wave's device region.  The capacity only decides where they
+// live: a cover fails when it exceeds kCoverRoots either way.
+constexpr uint32_t kCoverLdsRoots = uint32_t(sizeof(uint32_t) * kListCap / (2 * sizeof(double)));
+struct CoverBuf {
+    double* roots;
+    double* sorted;
 };
 struct CoverOut {
     double runA[kCoverRuns], runB[kCoverRuns], runS[kCoverRuns];
@@ -109,15 +118,27 @@ __device__ __forceinline__ double curveH(const TriCurve& t, double w) {
     return __builtin_sqrt(fmax(t.A * w * w + t.B * w + t.Cc, 0.0));
 }
 
-// Some neighbour other than k, l intersects the sphere at x by more than
-// margin + tol (one lane, over the neighbour set).
-__device__ __forceinline__ bool clearlyHit(const CellCtx& c, const CellLds& L, uint32_t F, const D4& x, uint32_t k,
-                                           uint32_t l, double margin, double tolBase) {
-    const double lim = -(margin + 1e-9 + 1e-7 * (__builtin_fabs(x.w) + tolBase));
-    for (uint32_t q = 0; q < F; q++) {
-        const uint32_t s = L.nb[q];
-        if (s == k || s == l) continue;
-        if (sphereDistD(nbSite(c, L, q, s), x) < lim) return true;
+// Gap below which a neighbour clearly hits the sphere at x: more than
+// margin + tol inside it.
+__device__ __forceinline__ double clearLimit(const D4& x, double margin, double tolBase) {
+    return -(margin + 1e-9 + 1e-7 * (__builtin_fabs(x.w) + tolBase));
+}
+
+// Some neighbour other than k, l has a gap below lim to the sphere at x (x,
+// lim uniform; the wave tests the neighbours in parallel).  With F <= 64
+// each lane's neighbour site is preloaded (site, id).
+__device__ __forceinline__ bool waveClearlyHit(const CellCtx& c, const CellLds& L, uint32_t F, const D4& x,
+                                               double lim, uint32_t k, uint32_t l, const D4& site, uint32_t id) {
+    const uint32_t lane = laneId();
+    if (F <= 64u) return ballot(lane < F && id != k && id != l && sphereDistD(site, x) < lim) != 0;
+    for (uint32_t b = 0; b < F; b += 64) {
+        const uint32_t q = b + lane;
+        bool h = false;
+        if (q < F) {
+            const uint32_t s = L.nb[q];
+            h = s != k && s != l && sphereDistD(nbSite(c, L, q, s), x) < lim;
+        }
+        if (ballot(h) != 0) return true;
     }
     return false;
 }
@@ -125,7 +146,7 @@ __device__ __forceinline__ bool clearlyHit(const CellCtx& c, const CellLds& L, u
 // Wave-cooperative cover of edge (k,l) of the cell in L; si, k, l are
 // uniform.  Returns the number of balls left in O.ball (uniform), or 0 when
 // no cover can be certified (the search stays unpruned).
-__device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L, CoverLds& B, CoverOut& O,
+__device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L, CoverBuf B, CoverOut& O,
                                              uint32_t F, const D4& si, uint32_t k, uint32_t l) {
     const uint32_t lane = laneId();
     TriCurve t;
@@ -222,9 +243,17 @@ __device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L,
     }
     waveSync();
 
-    // intervals of each piece: keep flags, then runs of kept intervals
+    // intervals of each piece: keep flags, then runs of kept intervals.  A
+    // lane evaluates one interval's test point; the wave then tests each
+    // point against all neighbours at once.
     uint8_t* keep = reinterpret_cast<uint8_t*>(B.roots);
     const double qn = len(t.Q);
+    D4 nbS{0, 0, 0, 0};
+    uint32_t nbId = kNone;
+    if (F <= 64u && lane < F) {
+        nbId = L.nb[lane];
+        nbS = nbSite(c, L, lane, nbId);
+    }
     uint32_t nStart = 0, nEnd = 0;
     double wRun = -DBL_MAX;
     for (int p = 0; p < np; p++) {
@@ -242,15 +271,20 @@ __device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L,
         auto bp = [&](uint32_t m) { return m == 0 ? pa : (m == M ? pb : B.sorted[lo + m - 1u]); };
         for (uint32_t b = 0; b < 2u * M; b += 64) {
             const uint32_t e = b + lane;
-            if (e < 2u * M) {
+            const bool act = e < 2u * M;
+            D4 x{0, 0, 0, 0};
+            double lim = 0.0;
+            bool unbounded = false;
+            if (act) {
                 const uint32_t h = e >= M ? 1u : 0u, m = e - h * M;
                 const double sgn = h ? -1.0 : 1.0;
                 const double u = bp(m), v = bp(m + 1u);
-                bool kp;
                 if (v == inf) {
+                    // must be clearly hit: the run may reach infinity
                     const double ws = u + .5 * fmax(u + r1, 0.0) + 1e-6 * (1.0 + r1);
-                    if (!clearlyHit(c, L, F, point(ws, sgn), k, l, 0.0, tolBase)) fail = true;  // may run to infinity
-                    kp = false;
+                    x = point(ws, sgn);
+                    lim = clearLimit(x, 0.0, tolBase);
+                    unbounded = true;
                 } else {
                     // a narrow interval must be cleared for all its points: the
                     // spheres on [u, v] move by at most the displacement bound
@@ -260,10 +294,25 @@ __device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L,
                         margin = qn * (v - u) * .5 +
                                  fmax(__builtin_fabs(curveH(t, u) - hm), __builtin_fabs(curveH(t, v) - hm)) +
                                  (v - u) * .5;
-                    kp = !clearlyHit(c, L, F, point(wm, sgn), k, l, margin, tolBase);
+                    x = point(wm, sgn);
+                    lim = clearLimit(x, margin, tolBase);
                 }
-                keep[e] = kp ? 1 : 0;
             }
+            bool kp = false;
+            uint64_t am = ballot(act);
+            while (am) {
+                const uint32_t src = uint32_t(__builtin_ctzll(am));
+                am &= am - 1;
+                const bool hit = waveClearlyHit(c, L, F, readLaneD4(x, src), readLaneD(lim, src), k, l, nbS, nbId);
+                if (lane == src) {
+                    if (unbounded) {
+                        if (!hit) fail = true;
+                    } else {
+                        kp = !hit;
+                    }
+                }
+            }
+            if (act) keep[e] = kp ? 1 : 0;
         }
         if (ballot(fail) != 0) return 0;
         waveSync();

@@ -24,37 +24,43 @@
 
 namespace apo_hip {
 
-// Traversal stack of the vertex and edge searches: the first
-// kVertexLdsDepth slots of every lane in LDS, the rest of the reference's
-// kStackDepth in a per-wave region of device memory (reached only by deeper
-// traversals), so the kernels' LDS admits more waves per CU.  The edge
-// kernel keeps its cover buffers in the same region, after the stack.
-constexpr uint32_t kVertexLdsDepth = 4;
+// Traversal stack of the vertex and edge searches: the newest
+// kVertexLdsDepth entries of every lane in an LDS ring, older ones evicted
+// to a per-wave region of device memory (the reference's kStackDepth in
+// all), so the kernels' LDS admits more waves per CU while pushes and pops
+// at the top of a depth-first search stay in LDS.  A lane's slots [0, lo)
+// live in the device region, slots [lo, sp) in the ring at slot %
+// kVertexLdsDepth.  The ring occupies the cell's update scratch list, which
+// holds nothing while the searches run (the cell is updated after them).
+// The edge kernel keeps its cover buffers in the same device region, after
+// the stack.
+constexpr uint32_t kVertexLdsDepth = 8;
+static_assert(kVertexLdsDepth * 64u <= kListCap, "the stack ring fits the cell's scratch list");
 constexpr uint32_t kVertexSpillWords = (kStackDepth - kVertexLdsDepth) * 64;
 constexpr uint32_t kWaveScratchWords = kVertexSpillWords + uint32_t(sizeof(CoverLds) / sizeof(uint32_t));
 static_assert(kVertexSpillWords % 2u == 0u, "the cover buffers of a wave region are 8-byte aligned");
+static_assert((kVertexLdsDepth & (kVertexLdsDepth - 1u)) == 0u, "the ring index is a mask");
 struct VStack {
     uint32_t (*lds)[64];
     uint32_t* spill;  // kVertexSpillWords words of this wave
-    __device__ __forceinline__ uint32_t get(uint32_t slot, uint32_t lane) const {
-        return slot < kVertexLdsDepth ? lds[slot][lane] : spill[(slot - kVertexLdsDepth) * 64u + lane];
-    }
-    __device__ __forceinline__ void set(uint32_t slot, uint32_t lane, uint32_t v) const {
-        if (slot < kVertexLdsDepth) lds[slot][lane] = v;
-        else spill[(slot - kVertexLdsDepth) * 64u + lane] = v;
+    // slot of a lane whose device-region part is [0, lo)
+    __device__ __forceinline__ uint32_t get(uint32_t slot, uint32_t lane, uint32_t lo) const {
+        return slot < lo ? spill[slot * 64u + lane] : lds[slot & (kVertexLdsDepth - 1u)][lane];
     }
 };
 
 // LDS of each kernel (one wave per block)
+__device__ __forceinline__ uint32_t (*stackRing(CellLds& L))[64] {
+    return reinterpret_cast<uint32_t (*)[64]>(L.list);
+}
+
 struct VertexLds {
     CellLds cell;
-    uint32_t stNode[kVertexLdsDepth][64];
     float resD[64];       // per owner lane: smallest gap found by shared items
     uint32_t resN[64];    // ... and its site (earliest in traversal order on ties)
 };
 struct EdgeLds {
     CellLds cell;
-    uint32_t stNode[kVertexLdsDepth][64];
     CoverOut coverOut;
     uint32_t res[64];     // per owner lane: earliest contributor found by a helper
 };
@@ -75,6 +81,7 @@ struct Trav {
     uint32_t node, sp;
     bool live;
     uint32_t sb = 0, base = 0;
+    uint32_t lo = 0;  // slots below lo are in the device region (VStack)
 };
 
 // Lane of the k-th (0-based) set bit of a wave mask: the donor a receiving
@@ -98,12 +105,29 @@ __device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
         t.node = stNode[t.sp][laneId()];
     }
 }
+// Take the top entry of a non-empty VStack (sp > sb).
+__device__ __forceinline__ uint32_t topEntry(const VStack& st, Trav& t) {
+    t.sp--;
+    const uint32_t v = st.get(t.sp, laneId(), t.lo);
+    if (t.sp < t.lo) t.lo = t.sp;
+    return v;
+}
+// Push onto a VStack: when the ring is full its oldest entry moves to the
+// device region first.
+__device__ __forceinline__ void pushEntry(const VStack& st, Trav& t, uint32_t v) {
+    const uint32_t lane = laneId();
+    if (t.sp >= t.lo + kVertexLdsDepth) {
+        st.spill[t.lo * 64u + lane] = st.lds[t.lo & (kVertexLdsDepth - 1u)][lane];
+        t.lo++;
+    }
+    st.lds[t.sp & (kVertexLdsDepth - 1u)][lane] = v;
+    t.sp++;
+}
 __device__ __forceinline__ void popNode(const VStack& st, Trav& t) {
     if (t.sp == t.sb) {
         t.live = false;
     } else {
-        t.sp--;
-        t.node = st.get(t.sp, laneId());
+        t.node = topEntry(st, t);
     }
 }
 
@@ -164,8 +188,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, c
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode.set(t.sp, lane, node | (farChild == R0 ? 0x80000000u : 0u));
-                    t.sp++;
+                    pushEntry(stNode, t, node | (farChild == R0 ? 0x80000000u : 0u));
                 } else {
                     flags |= kFlagStackOverflow;
                 }
@@ -184,8 +207,7 @@ __device__ __forceinline__ void vertexStep(const CellCtx& c, const CellLds& L, c
     }
     bool more = false;
     while (t.sp > t.sb) {
-        t.sp--;
-        const uint32_t e = stNode.get(t.sp, lane);
+        const uint32_t e = topEntry(stNode, t);
         const Node pn = c.nodes[e & 0x7fffffffu];
         Box lb, rb;
         nodeBoxes(pn, lb, rb);
@@ -289,8 +311,7 @@ __device__ __forceinline__ void vertexNearStep(const CellCtx& c, const CellLds& 
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode.set(t.sp, lane, farChild);
-                    t.sp++;
+                    pushEntry(stNode, t, farChild);
                 } else {
                     flags |= kFlagStackOverflow;
                 }
@@ -358,7 +379,7 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                                                    const WaveCell& w, uint32_t i, uint32_t root, bool need,
                                                    const uint4 vx0, const F4 cur0, uint32_t* spill, uint32_t& flags) {
     const uint32_t lane = laneId();
-    const VStack st{S.stNode, spill};
+    const VStack st{stackRing(S.cell), spill};
     const float reach = 1e-5f * (__builtin_fabsf(cur0.x) + __builtin_fabsf(cur0.y) + __builtin_fabsf(cur0.z) +
                                  __builtin_fabsf(cur0.w)) + 1e-6f;
     bool exact = need;
@@ -388,9 +409,10 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                 waveSync();  // the donors' stack slots are visible to the takers
                 const uint32_t src = take ? laneOfRank(donors, ir) : lane;
                 const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
+                const uint32_t dLo = __shfl(tr.lo, int(src), 64);
                 const uint32_t dOwner = __shfl(owner, int(src), 64);
                 const float dNd = __shfl(nd, int(src), 64);
-                const uint32_t node = take ? st.get(dSb, src) : 0u;
+                const uint32_t node = take ? st.get(dSb, src, dLo) : 0u;
                 cur = shflF4(cur, src);
                 slack = __shfl(slack, int(src), 64);
                 vx.x = __shfl(vx.x, int(src), 64);
@@ -408,6 +430,7 @@ __device__ __forceinline__ uint32_t vertexSearches(const CellCtx& c, VertexLds& 
                     tr.node = node;
                     tr.sp = 0u;
                     tr.sb = 0u;
+                    tr.lo = 0u;
                     tr.base = dBase + dSb;
                     tr.live = true;
                 }
@@ -593,8 +616,7 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, const VStack& stN
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode.set(t.sp, lane, farChild);
-                    t.sp++;
+                    pushEntry(stNode, t, farChild);
                 } else {
                     flags |= kFlagStackOverflow;
                 }
@@ -673,7 +695,7 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
                                               uint32_t i, const D4& si, uint32_t root, bool need, uint32_t k,
                                               uint32_t l, const EdgeQuery& q, uint32_t* spill, uint32_t& flags) {
     const uint32_t lane = laneId();
-    const VStack st{S.stNode, spill};
+    const VStack st{stackRing(S.cell), spill};
     S.res[lane] = kNone;
     uint64_t settled = 0;  // owner lanes whose own item found the contributor
     // the item this lane works on: the query it serves and its owner
@@ -741,8 +763,9 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
             waveSync();  // the donors' stack slots are visible to the takers
             const uint32_t src = take ? laneOfRank(donors, ir) : lane;
             const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
+            const uint32_t dLo = __shfl(tr.lo, int(src), 64);
             const uint32_t dOwner = __shfl(owner, int(src), 64);
-            const uint32_t node = take ? st.get(dSb, src) : 0u;
+            const uint32_t node = take ? st.get(dSb, src, dLo) : 0u;
             wq.bs = shflF4(wq.bs, src);
             wq.hyperbola = __shfl(wq.hyperbola ? 1 : 0, int(src), 64) != 0;
             wq.n1 = shflF3(wq.n1, src);
@@ -762,6 +785,7 @@ __device__ __forceinline__ uint32_t edgeSearches(const CellCtx& c, EdgeLds& S, c
                 tr.node = node;
                 tr.sp = 0u;
                 tr.sb = 0u;
+                tr.lo = 0u;
                 tr.base = dBase + dSb;
                 tr.live = true;
             }
@@ -852,8 +876,10 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
         while (bm) {
             const uint32_t t = uint32_t(__builtin_ctzll(bm));
             bm &= bm - 1;
-            const int cnt =
-                waveEdgeCover(c, L, cover, S.coverOut, w.F, si, readLane(k, t), readLane(l, t));
+            double* ldsRoots = reinterpret_cast<double*>(L.list);
+            const CoverBuf buf = 2u * w.F + 1u <= kCoverLdsRoots ? CoverBuf{ldsRoots, ldsRoots + kCoverLdsRoots}
+                                                                 : CoverBuf{cover.roots, cover.sorted};
+            const int cnt = waveEdgeCover(c, L, buf, S.coverOut, w.F, si, readLane(k, t), readLane(l, t));
             const bool mine = cnt > 0 && lane == t;
             q.cover = mine ? cnt : q.cover;
 #pragma unroll

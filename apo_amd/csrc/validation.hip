@@ -807,7 +807,7 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
     if (st == kStatusCertified || st == kStatusBuried) return;
     WaveCell w;
     // the reference's edge validation updates a vertex list that starts
-    // empty (algorithm.inl:526-527), so a contributor never invalidates the
+    // empty (algorithm.inl:532), so a contributor never invalidates the
     // cell's stored vertices; the completion mode updates the whole list
     loadCell(c, L, w, i, c.complete != 0u);
     const uint32_t edgesLoaded = w.E;

@@ -1397,7 +1397,7 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
         for (Edg& e : cell.E)
             if (e.status != 1) e.status = 2;
         // the reference updates a vertex list that starts empty
-        // (algorithm.inl:526-527); the completion mode the whole list
+        // (algorithm.inl:532); the completion mode the whole list
         std::vector<Vtx> Vown;
         std::vector<Vtx>& Vloc = C.cfg.complete ? cell.V : Vown;
         bool didChange = false;

@@ -51,7 +51,8 @@ class Stats(ctypes.Structure):
                 ("ms_proposal", ctypes.c_double), ("proposal", ctypes.c_uint64), ("cells_certified", ctypes.c_uint64),
                 ("ms_completion", ctypes.c_double), ("completion_rounds", ctypes.c_uint64),
                 ("completion_updated_cells", ctypes.c_uint64), ("completion_unrepairable", ctypes.c_uint64),
-                ("completion_validation_tasks", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 2)]
+                ("completion_validation_tasks", ctypes.c_uint64), ("completion_rebuilt_cells", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 1)]
 
 
 _vp = ctypes.c_void_p

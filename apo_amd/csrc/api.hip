@@ -133,7 +133,7 @@ struct apo_ctx {
     uint32_t* enb = nullptr;
     uint32_t* knn = nullptr;
     uint32_t* counters = nullptr;   // [0] cell queue, [1..15] validation
-    uint32_t* spill = nullptr;      // deep stack slots of the vertex-validation waves
+    uint32_t* spill = nullptr;      // per-wave regions of the validation waves (deep stack slots, covers)
     uint32_t spillBlocks = 0;
     uint32_t* err = nullptr;        // device guard bits
     uint32_t* tasks = nullptr;
@@ -157,6 +157,7 @@ struct apo_ctx {
     uint32_t capP = 0;          // site capacity of feat / cls / labels
     uint32_t* want = nullptr;     // completion mode: smallest wanted site per cell (kNone: none)
     uint32_t capW = 0;
+    uint4* stuck = nullptr;       // completion mode: a round's stuck records (kStuckCap)
     float* givenRadii = nullptr;  // search radii given by the caller (input order): range-query cell pass
     uint32_t givenRN = 0;
     uint32_t givenRCap = 0;
@@ -454,10 +455,11 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
                              reinterpret_cast<int32_t*>(ctx->cellCount), reinterpret_cast<int2*>(ctx->blockSums),
                              ctx->counters + kTreeHeight, st));
     const ValidationBuffers vb{ctx->tasks, ctx->counters, ctx->spill, ctx->spillBlocks};
-    // vertex and edge validation waves each own a device region: at most spillBlocks of them
+    // validation waves each own a device region: at most spillBlocks of them
     const ValidationGrid vg{std::min(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex), ctx->spillBlocks),
                             std::min(persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge), ctx->spillBlocks),
-                            persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector)};
+                            std::min(persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector),
+                                     ctx->spillBlocks)};
     if (ctx->cfg.stage_mask & 2u)
         APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st,
                                  timed ? &ctx->vt : nullptr));
@@ -476,7 +478,7 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
                                  ctx->counters + kCompletionTasks));
         const uint32_t blocks = persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells);
         for (uint32_t r = 0; r < ctx->cfg.complete; r++) {
-            APO_HIP(consistencyLaunch(cc, ctx->want, ctx->counters, r, blocks, st));
+            APO_HIP(consistencyLaunch(cc, ctx->want, ctx->stuck, ctx->counters, r, blocks, st));
             APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st, nullptr,
                                      ctx->counters + kCompletionTasks + (r + 1u) * kMaxRounds));
         }
@@ -591,7 +593,9 @@ apo_status finishBuild(apo_ctx* ctx, uint32_t n, bool learned, bool timed) {
             S.completion_rounds++;
             S.completion_updated_cells += counters[kCompletionUpdated + r];
         }
-        S.completion_unrepairable = counters[kCompletionStuck];
+        // records still missing when the rounds end: the last round's count
+        if (S.completion_rounds) S.completion_unrepairable = counters[kCompletionStuck + S.completion_rounds - 1];
+        S.completion_rebuilt_cells = counters[kCompletionRebuilt];
         for (uint32_t q = 0; q <= ctx->cfg.complete; q++)
             for (uint32_t r = 0; r < ctx->cfg.max_iterations; r++)
                 S.completion_validation_tasks += counters[kCompletionTasks + q * kMaxRounds + r];
@@ -787,7 +791,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
-                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->givenRadii, ctx->radiiSorted, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->stuck, ctx->givenRadii, ctx->radiiSorted, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -839,10 +843,11 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     if (learnedPath(ctx) && ctx->cfg.knn_passes != 1u)
         return fail(ctx, APO_ERR_INVALID, "the learned proposal runs one kNN pass (knn_passes = 1)");
     ctx->cavValid = false;
-    // per-wave device regions of the vertex and edge validation grids (the
-    // two kernels run one after the other on the stream and share them)
-    const uint32_t vblocks = std::max(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
-                                      persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge));
+    // per-wave device regions of the vertex, edge and bisector validation
+    // grids (the kernels run one after the other on the stream and share them)
+    const uint32_t vblocks = std::max(std::max(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
+                                               persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge)),
+                                      persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector));
     if (vblocks > ctx->spillBlocks) {
         dropGraph(ctx);
         if (ctx->spill) APO_HIP(hipFree(ctx->spill));
@@ -857,6 +862,10 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
         dropGraph(ctx);
         APO_HIP(alloc(ctx->want, n));
         ctx->capW = n;
+    }
+    if (ctx->cfg.complete && !ctx->stuck) {
+        dropGraph(ctx);
+        APO_HIP(alloc(ctx->stuck, kStuckCap));
     }
     if (learnedPath(ctx)) {
         s = ensureProposalBuffers(ctx, n);
@@ -913,7 +922,7 @@ namespace {
     X(scene) X(boxPartials) X(boxTicket) X(sortScratch) X(range) X(startLists) X(codesA) X(codesB) X(idxA) X(idxB) \
     X(sorted) X(nodes) X(parent) X(sizes) X(refitCounter) X(rootBox) X(status) X(knownRadii) X(flags) X(vtx)       \
     X(vnb) X(edg) X(enb) X(knn) X(counters) X(spill) X(err) X(tasks) X(cellCount) X(blockSums) X(total)            \
-    X(outXyzw) X(outIds) X(bytes) X(want)
+    X(outXyzw) X(outIds) X(bytes) X(want) X(stuck)
 
 struct BuildBuffers {
 #define APO_DECLARE(f) decltype(apo_ctx::f) f;
@@ -995,6 +1004,7 @@ bool carveStatic(void* base, uint32_t n, uint32_t K, double* xyzw, uint32_t* ids
     c.take(b.total, 1);
     c.take(b.bytes, size_t(kByteFamilies) * kByteSlots);
     c.take(b.want, n);
+    c.take(b.stuck, kStuckCap);
     c.take(b.spill, size_t(kStaticWaveRegions) * waveScratchWords());
     b.spillBlocks = kStaticWaveRegions;
     b.outXyzw = reinterpret_cast<D4*>(xyzw);

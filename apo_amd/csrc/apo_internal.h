@@ -202,14 +202,22 @@ constexpr uint32_t kBisectorQueue = 32;
 constexpr uint32_t kTreeHeight = 40;   // deepest leaf of the build's trees (root = 0)
 constexpr uint32_t kRoundTasks = 64;
 constexpr uint32_t kMaxRounds = 256;
-// completion mode: cells updated by each consistency round, the records no
-// update can repair, then the task counts of its validation rounds (after
-// the bisector stage, then after each consistency round: kMaxRounds each)
+// completion mode: cells updated by each consistency round, the missing
+// records whose cell knows all their sites found by each round (the append
+// index of the round's stuck-record list), then the task counts of its
+// validation rounds (after the bisector stage, then after each consistency
+// round: kMaxRounds each), then the cells rebuilt over all rounds
 constexpr uint32_t kMaxCompletionRounds = 8;
 constexpr uint32_t kCompletionUpdated = kRoundTasks + kMaxRounds;
 constexpr uint32_t kCompletionStuck = kCompletionUpdated + kMaxCompletionRounds;
-constexpr uint32_t kCompletionTasks = kCompletionStuck + 8;
-constexpr uint32_t kCounterWords = kCompletionTasks + (1 + kMaxCompletionRounds) * kMaxRounds;
+constexpr uint32_t kCompletionTasks = kCompletionStuck + kMaxCompletionRounds;
+constexpr uint32_t kCompletionRebuilt = kCompletionTasks + (1 + kMaxCompletionRounds) * kMaxRounds;
+constexpr uint32_t kCounterWords = kCompletionRebuilt + 1;
+// A round's list of stuck records (cell, the record's other three sites):
+// a round that finds more rebuilds no cell.  want[] codes besides site ids:
+constexpr uint32_t kStuckCap = 16384;
+constexpr uint32_t kWantRebuilt = 0xfffffffdu;  // rebuilt in this build: not again
+constexpr uint32_t kWantRebuild = 0xfffffffeu;  // stuck: rebuild in this round
 
 static __global__ void k_fill_words(uint32_t* p, uint32_t count, uint32_t value) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,9 +277,10 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const 
                             uint32_t iterations, uint32_t maxRestarts, bool bisector, hipStream_t st,
                             ValidationTiming* tm, uint32_t* roundTasks = nullptr);
 // One consistency round of the completion mode (complete.hip): want holds n
-// words, kNone between rounds; round < kMaxCompletionRounds.
-hipError_t consistencyLaunch(const CellCtx& c, uint32_t* want, uint32_t* counters, uint32_t round, uint32_t blocks,
-                             hipStream_t st);
+// words (kNone or kWantRebuilt between rounds, kNone at the build's start),
+// stuck kStuckCap records; round < kMaxCompletionRounds.
+hipError_t consistencyLaunch(const CellCtx& c, uint32_t* want, uint4* stuck, uint32_t* counters, uint32_t round,
+                             uint32_t blocks, hipStream_t st);
 
 // Output (algorithm.inl:1235-1307): count, scan, write.
 struct OutputBuffers {

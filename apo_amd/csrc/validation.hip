@@ -66,7 +66,6 @@ struct EdgeLds {
 };
 struct BisectorLds {
     CellLds cell;
-    uint32_t stNode[kStackDepth][64];
     uint32_t bis[kBisectorCap];
     uint32_t res[64];     // per owner lane: earliest contributor found by a helper
 };
@@ -96,15 +95,6 @@ __device__ __forceinline__ uint32_t laneOfRank(uint64_t mask, uint32_t k) {
     return lo;
 }
 
-// Pop the next node of a lane's depth-first traversal, or end it.
-__device__ __forceinline__ void popNode(uint32_t (*stNode)[64], Trav& t) {
-    if (t.sp == t.sb) {
-        t.live = false;
-    } else {
-        t.sp--;
-        t.node = stNode[t.sp][laneId()];
-    }
-}
 // Take the top entry of a non-empty VStack (sp > sb).
 __device__ __forceinline__ uint32_t topEntry(const VStack& st, Trav& t) {
     t.sp--;
@@ -921,7 +911,7 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
 // One step of a lane's bisector traversal (algorithm.inl:1019-1176).
 // Returns the leaf's site when it is a candidate: other than i, bk, inside
 // the search sphere and the cylinder around the bisector axis.
-__device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*stNode)[64], uint32_t i, uint32_t bk,
+__device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, const VStack& stNode, uint32_t i, uint32_t bk,
                                                  F3 fci, F3 fck, F4 bs, float cylR, Trav& t, uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
@@ -949,8 +939,7 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*st
             t.node = ec ? closeChild : farChild;
             if (ec && ef) {
                 if (t.base + t.sp < kStackDepth) {
-                    stNode[t.sp][lane] = farChild;
-                    t.sp++;
+                    pushEntry(stNode, t, farChild);
                 } else {
                     flags |= kFlagStackOverflow;
                 }
@@ -988,8 +977,9 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, uint32_t (*st
 __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorLds& S, const CellLds& L,
                                                      const WaveCell& w, uint32_t i, const D4& si, uint32_t root,
                                                      bool need, uint32_t bk, D4 sk, F3 fci, F3 fck, F4 bsph,
-                                                     float cylR, uint32_t& flags) {
+                                                     float cylR, uint32_t* spill, uint32_t& flags) {
     const uint32_t lane = laneId();
+    const VStack st{stackRing(S.cell), spill};
     S.res[lane] = kNone;
     uint64_t settled = 0;  // owner lanes whose own item found the contributor
     uint32_t owner = need ? lane : kNone;
@@ -1041,8 +1031,9 @@ __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorL
             waveSync();  // the donors' stack slots are visible to the takers
             const uint32_t src = take ? laneOfRank(donors, ir) : lane;
             const uint32_t dSb = __shfl(tr.sb, int(src), 64), dBase = __shfl(tr.base, int(src), 64);
+            const uint32_t dLo = __shfl(tr.lo, int(src), 64);
             const uint32_t dOwner = __shfl(owner, int(src), 64);
-            const uint32_t node = take ? S.stNode[dSb][src] : 0u;
+            const uint32_t node = take ? st.get(dSb, src, dLo) : 0u;
             bk = __shfl(bk, int(src), 64);
             sk = shflD4(sk, src);
             fck = shflF3(fck, src);
@@ -1054,19 +1045,21 @@ __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorL
                 tr.node = node;
                 tr.sp = 0u;
                 tr.sb = 0u;
+                tr.lo = 0u;
                 tr.base = dBase + dSb;
                 tr.live = true;
             }
             if (give) tr.sb++;
             waveSync();
         }
-        if (tr.live && cand == kNone) cand = bisectorStep(c, S.stNode, i, bk, fci, fck, bsph, cylR, tr, flags);
+        if (tr.live && cand == kNone) cand = bisectorStep(c, st, i, bk, fci, fck, bsph, cylR, tr, flags);
     }
     waveSync();
     return need && found == kNone ? S.res[lane] : found;
 }
 
-__device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, uint32_t i, uint32_t maxRestarts) {
+__device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, uint32_t i, uint32_t maxRestarts,
+                                             uint32_t* spill) {
     CellLds& L = S.cell;
     const uint32_t lane = laneId();
     const uint8_t st = c.status[i];
@@ -1138,7 +1131,7 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
         }
         // the first site of each lane's traversal whose trisector with i, bk
         // has two points, the second clear of the cell's neighbours
-        found = bisectorSearches(c, S, L, w, i, si, root, need, bk, sk, fci, fck, bsph, cylR, flags);
+        found = bisectorSearches(c, S, L, w, i, si, root, need, bk, sk, fci, fck, bsph, cylR, spill, flags);
         if (need && found == kNone) S.bis[bid] = kNone;
         const uint64_t m = ballot(found != kNone);
         waveSync();
@@ -1200,12 +1193,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
 }
 
-__global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t* counters, uint32_t maxRestarts) {
+__global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t* counters, uint32_t maxRestarts,
+                                                            uint32_t* spill) {
     __shared__ BisectorLds S;
+    uint32_t* mySpill = spill + size_t(blockIdx.x) * kWaveScratchWords;
     for (;;) {
         const uint32_t i = popWork(counters + kBisectorQueue, c.n);
         if (i == kNone) break;
-        bisectorTask(c, S, i, maxRestarts);
+        bisectorTask(c, S, i, maxRestarts, mySpill);
         waveSync();
     }
 }
@@ -1246,8 +1241,10 @@ hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const 
                             ValidationTiming* tm, uint32_t* roundTasks) {
     if (!roundTasks) roundTasks = v.counters + kRoundTasks;
     if (bisector) {
+        if (g.bisector > v.spillBlocks) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, v.counters + kBisectorQueue, kXcdQueueWords, 0u);
-        hipLaunchKernelGGL(k_bisector_validation, dim3(g.bisector), dim3(64), 0, st, c, v.counters, maxRestarts);
+        hipLaunchKernelGGL(k_bisector_validation, dim3(g.bisector), dim3(64), 0, st, c, v.counters, maxRestarts,
+                           v.spill);
         return hipGetLastError();
     }
     if (iterations > kMaxRounds || g.vertex > v.spillBlocks || g.edge > v.spillBlocks) return hipErrorInvalidValue;

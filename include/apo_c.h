@@ -113,15 +113,17 @@ typedef struct apo_stats {
     uint64_t proposal;
     uint64_t cells_certified;
     /* completion mode (config.complete): its time, the consistency rounds that
-     * ran, the cells they updated, the missing records whose sites the cell
-     * already knows (no update repairs them), and the tasks of its validation
-     * rounds */
+     * ran, the cells they updated (rebuilt ones included), the records still
+     * missing from a cell that knows their sites when the rounds end (counted
+     * by the last round), the tasks of its validation rounds, and the cells
+     * rebuilt because a missing record's sites were all known */
     double ms_completion;
     uint64_t completion_rounds;
     uint64_t completion_updated_cells;
     uint64_t completion_unrepairable;
     uint64_t completion_validation_tasks;
-    uint64_t reserved[2];
+    uint64_t completion_rebuilt_cells;
+    uint64_t reserved[1];
 } apo_stats;
 
 void apo_default_config(apo_config* cfg);

@@ -326,7 +326,7 @@ class Diagram:
         lib().orc_get_stats(self._h, _p(s, u64p))
         keys = ["uncertified_after_cells", "validation_tasks", "validation_iterations", "flagged_cells",
                 "vertices", "buried", "bisector_changed", "completion_rounds", "completion_updated_cells",
-                "completion_unrepairable", "completion_validation_tasks", "reserved"]
+                "completion_unrepairable", "completion_validation_tasks", "completion_rebuilt_cells"]
         return dict(zip(keys, (int(x) for x in s)))
 
 

@@ -633,7 +633,15 @@ struct Context {
     std::vector<uint32_t> flags;
     std::vector<uint32_t> siteK;  // per-site neighbour count of the proposal path (empty: all K)
     std::vector<float> range;     // per-site search radius of the range path (empty: none)
+    std::vector<uint32_t> want;   // completion mode: per cell, kNone, a wanted site, or a rebuild code
 };
+
+// Completion mode: a round's stuck-record list and the codes of want[]
+// besides site ids (the HIP kernels use the same).
+constexpr uint32_t kStuckCap = 16384;
+constexpr uint32_t kRebuildSites = 528;        // LDS list of a rebuild (2 x the edge capacity)
+constexpr uint32_t kWantRebuilt = 0xfffffffdu;  // rebuilt in this build: not again
+constexpr uint32_t kWantRebuild = 0xfffffffeu;  // stuck: rebuild in this round
 
 // Upper gap bound of a range query of radius b around a site of radius r:
 // raised to the float just above -r, since a box holding the site's centre
@@ -1600,22 +1608,38 @@ inline uint32_t recordCount(const Cell& c, uint32_t a, uint32_t b, uint32_t d) {
 // quadruple sites it does not know; each cell is then updated with the
 // smallest site it wants (one per round, so the result does not depend on
 // the order of the requests) and goes back to validation when it changes.
-// Buried cells, artificial sites and cells with a capacity flag take no
-// part.  A cell that knows all three sites yet holds fewer records cannot be
-// repaired by an update and is counted in *unrepairable.  Returns the cells
-// the round changed.
-uint64_t consistencyRound(Context& C, uint64_t& unrepairable) {
+// A cell that knows all three sites yet holds fewer records is stuck (the
+// order in which its sites were inserted lost an edge that no update
+// creates again): the record goes to the round's stuck list, and a stuck
+// cell with no wanted site is rebuilt from its octahedron, first with the
+// sites of its stuck records (ascending), then with its other real,
+// unburied neighbours nearest first (FP32 gap, ties by index).  A cell is
+// rebuilt at most once per build; a round whose stuck list exceeds
+// kStuckCap rebuilds nothing, nor does a cell whose stuck records hold more
+// than kRebuildSites sites (with repeats).  Buried cells, artificial sites
+// and cells with a capacity flag take no part.  *stuckCount receives the
+// round's stuck records, *rebuilt is increased by the cells rebuilt.
+// Returns the cells the round changed.
+uint64_t consistencyRound(Context& C, uint64_t& stuckCount, uint64_t& rebuilt) {
     const uint32_t n = C.n;
-    std::vector<std::atomic<uint32_t>> want(n);
-    for (auto& w : want) w.store(kNone, std::memory_order_relaxed);
-    std::atomic<uint64_t> stuck{0};
     auto part = [&](uint32_t m) { return C.status[m] != kBuried && C.flags[m] == 0u; };
+#ifdef _OPENMP
+    const int threads = omp_get_max_threads();
+#else
+    const int threads = 1;
+#endif
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> req(threads);
+    std::vector<std::vector<std::array<uint32_t, 4>>> stuckT(threads);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < int64_t(n); ii++) {
         const uint32_t i = uint32_t(ii);
         if (!part(i)) continue;
+#ifdef _OPENMP
+        const int tid = omp_get_thread_num();
+#else
+        const int tid = 0;
+#endif
         const Cell& ci = C.cells[i];
-        uint64_t myStuck = 0;
         for (const Vtx& v : ci.V) {
             if (v.l >= n) continue;  // j < k < l: an artificial site is last
             const uint32_t own = recordCount(ci, v.j, v.k, v.l);
@@ -1631,32 +1655,70 @@ uint64_t consistencyRound(Context& C, uint64_t& unrepairable) {
                 bool any = false;
                 for (int u = 0; u < 3; u++)
                     if (!known(C.cells[m], o[u])) {
-                        uint32_t cur = want[m].load(std::memory_order_relaxed);
-                        while (o[u] < cur && !want[m].compare_exchange_weak(cur, o[u])) {
-                        }
+                        req[tid].push_back({m, o[u]});
                         any = true;
                     }
-                if (!any) myStuck++;
+                if (!any) {
+                    stuckT[tid].push_back({m, o[0], o[1], o[2]});
+                    req[tid].push_back({m, kWantRebuild});
+                }
             }
         }
-        if (myStuck) stuck += myStuck;
     }
-    unrepairable += stuck.load();
-    std::atomic<uint64_t> updated{0};
+    std::vector<std::array<uint32_t, 4>> stuck;
+    for (int t = 0; t < threads; t++) {
+        for (const auto& r : req[t]) C.want[r.first] = std::min(C.want[r.first], r.second);
+        stuck.insert(stuck.end(), stuckT[t].begin(), stuckT[t].end());
+    }
+    std::sort(stuck.begin(), stuck.end());
+    stuckCount = stuck.size();
+    const bool rebuildOk = stuck.size() <= kStuckCap;
+    std::atomic<uint64_t> updated{0}, nRebuilt{0};
 #pragma omp parallel for schedule(dynamic, 16)
     for (int64_t mm = 0; mm < int64_t(n); mm++) {
         const uint32_t m = uint32_t(mm);
-        const uint32_t j = want[m].load(std::memory_order_relaxed);
-        if (j == kNone) continue;
+        const uint32_t j = C.want[m];
+        if (j == kNone || j == kWantRebuilt) continue;
         Cell& cell = C.cells[m];
         uint32_t flags = 0;
+        if (j == kWantRebuild) {
+            // the stuck records of m are contiguous in the sorted list
+            const auto lo = std::lower_bound(stuck.begin(), stuck.end(), std::array<uint32_t, 4>{m, 0, 0, 0});
+            std::vector<uint32_t> first;
+            for (auto it = lo; it != stuck.end() && (*it)[0] == m; ++it)
+                for (int u = 1; u < 4; u++) first.push_back((*it)[u]);
+            if (!rebuildOk || first.size() > kRebuildSites) {
+                C.want[m] = kNone;
+                continue;
+            }
+            std::sort(first.begin(), first.end());
+            first.erase(std::unique(first.begin(), first.end()), first.end());
+            std::vector<std::pair<float, uint32_t>> rest;
+            for (const Edg& e : cell.E)
+                for (uint32_t x : {e.j, e.k})
+                    if (x < n && C.status[x] != kBuried && !std::binary_search(first.begin(), first.end(), x))
+                        rest.push_back({sphereDistF(C.Sf[m], C.Sf[x]), x});
+            std::sort(rest.begin(), rest.end());
+            rest.erase(std::unique(rest.begin(), rest.end()), rest.end());
+            initCell(C, cell);
+            for (uint32_t x : first) updateCell(C, m, C.S[m], cell.V, cell, x, C.S[x], flags);
+            for (const auto& pr : rest) updateCell(C, m, C.S[m], cell.V, cell, pr.second, C.S[pr.second], flags);
+            C.flags[m] |= flags;
+            C.status[m] = 0;
+            C.want[m] = kWantRebuilt;
+            updated++;
+            nRebuilt++;
+            continue;
+        }
         const bool changed = updateCell(C, m, C.S[m], cell.V, cell, j, C.S[j], flags);
         C.flags[m] |= flags;
+        C.want[m] = kNone;
         if (changed) {
             C.status[m] = 0;
             updated++;
         }
     }
+    rebuilt += nRebuilt.load();
     return updated.load();
 }
 
@@ -1955,10 +2017,11 @@ static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config
         // every round's validation rounds run (over the cells still in
         // validation); a round after one that updated no cell checks nothing
         uint64_t updated = 1;
+        C.want.assign(n, kNone);
         for (uint32_t round = 0; round < cfg.complete; round++) {
             if (updated) {
                 d->stats[7]++;
-                updated = consistencyRound(C, d->stats[9]);
+                updated = consistencyRound(C, d->stats[9], d->stats[11]);
                 d->stats[8] += updated;
             }
             validationRounds(C, d->stats[10]);

@@ -134,10 +134,11 @@ void orc_get_knns(const orc_diagram* d, uint32_t* knn);
 /* Stage statistics: [0] uncertified after cells, [1] tasks per validation
  * iteration summed, [2] iterations run, [3] overflowed cells, [4] vertices
  * emitted, [5] buried cells, [6] cells changed by bisector validation,
- * [7] completion: consistency rounds run, [8] cells updated by them,
- * [9] missing records whose sites the cell already knows (not repairable
- * by an update), [10] tasks of the validation rounds after the bisector
- * stage and after each consistency round, [11] reserved.                 */
+ * [7] completion: consistency rounds run, [8] cells updated by them
+ * (rebuilt ones included), [9] missing records whose sites the cell
+ * already knows, counted by the last round that ran (left unrepaired),
+ * [10] tasks of the validation rounds after the bisector stage and after
+ * each consistency round, [11] cells rebuilt.                            */
 void orc_get_stats(const orc_diagram* d, uint64_t* stats12);
 /* Completion mode's subtree test: 1 when some point of the box (min xyz,
  * max xyz) may lie strictly inside the half-space n.(x - p) < 0. */

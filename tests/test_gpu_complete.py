@@ -6,6 +6,9 @@
 * The reference method's output depends on K (its hyperbolic edge searches
   skip contributors that a larger K inserts directly); with completion the
   vertex set no longer does: K = 16, 24 and 32 give the same rows.
+* A cell whose insertion order lost one of its edges is rebuilt (the
+  vertex its three other cells hold comes back): K = 16 and 24 give the
+  same rows on a 1,061-site piece of the 5M C5 set, GPU equal to the oracle.
 * At full size (1M log-uniform sites, the learned-proposal workload): the
   plain build with completion has 0 empty-sphere violations and 0 bad
   interval vertices, and the learned path (per-site neighbour counts, here
@@ -55,6 +58,7 @@ def test_complete_matches_oracle(name):
         assert s["completion_updated_cells"] == os_["completion_updated_cells"]
         assert s["completion_unrepairable"] == os_["completion_unrepairable"]
         assert s["completion_validation_tasks"] == os_["completion_validation_tasks"]
+        assert s["completion_rebuilt_cells"] == os_["completion_rebuilt_cells"]
         print(f"[completion] {name}: rounds {s['completion_rounds']}, updated {s['completion_updated_cells']}, "
               f"unrepairable {s['completion_unrepairable']}", flush=True)
     finally:
@@ -76,6 +80,42 @@ def test_complete_vertex_set_independent_of_k():
     print(f"[k] without completion: {[len(plain[k]) for k in (16, 24, 32)]} vertices", flush=True)
     assert not np.array_equal(plain[16], plain[24])  # the reference's output depends on K here
     assert np.array_equal(rows[16], rows[24]) and np.array_equal(rows[16], rows[32])
+
+
+def _lost_edge_set():
+    """1,061 sites of the 5M C5 set (log-uniform radii, seed 55) within 0.03
+    of site 1326879: with K = 24 one cell inserts its sites in an order that
+    never creates one of its trisector edges, so a vertex the three other
+    cells of its quadruple hold is missing from it; only a rebuild of that
+    cell repairs it."""
+    s = datasets.loguniform_sites(5_000_000, seed=55)
+    return s[np.all(np.abs(s[:, :3] - s[1326879, :3]) < 0.03, axis=1)]
+
+
+def test_rebuild_repairs_lost_edge():
+    sites = _lost_edge_set()
+    rows = {}
+    for k in (16, 24):
+        print(f"[case] lost-edge set k={k}", flush=True)
+        g = AlgorithmGPU(sites, k=k, complete=4).build()
+        try:
+            d = g.to_host()
+            rows[k] = _rows(d.vertices, d.vertices_id)
+            orc = oracle.Diagram(sites, k=k, complete=4)
+            ox, oi = orc.vertices()
+            assert np.array_equal(rows[k], _rows(ox, oi))
+            st, ost = g.stats(), orc.stats()
+            for key in ("completion_rounds", "completion_updated_cells", "completion_unrepairable",
+                        "completion_rebuilt_cells"):
+                assert st[key] == ost[key], key
+            print(f"[completion] k={k}: rebuilt {st['completion_rebuilt_cells']}, "
+                  f"unrepairable {st['completion_unrepairable']}", flush=True)
+            if k == 24:
+                assert st["completion_rebuilt_cells"] > 0
+            assert g.empty_sphere_violations() == 0
+        finally:
+            g.close()
+    assert np.array_equal(rows[16], rows[24])
 
 
 def _set_difference(a, b):

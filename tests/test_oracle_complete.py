@@ -78,3 +78,30 @@ def test_completion_removes_the_dependence_on_k(gen, seed):
         if gen != "protein":  # overlapping atoms give the method negative-radius vertices it counts
             assert oracle.empty_sphere_violations(sites, x, ids) == 0
     assert np.array_equal(rows[16], rows[24]) and np.array_equal(rows[16], rows[32])
+
+
+def test_rebuild_repairs_lost_edge():
+    """A cell of this 1,061-site piece of the 5M C5 set (within 0.03 of site
+    1326879) inserts its sites, with K = 24, in an order that never creates
+    one of its edges: the vertex of quadruple (1326879, 2326511, 602170,
+    279520), tangent to its four sites (gaps below 1e-13) and clear of every
+    other site by 1.9e-7, is held by three cells and missing from the fourth.
+    The rebuild restores it: K = 16 and 24 emit the same rows, and that
+    vertex is among them."""
+    s = datasets.loguniform_sites(5_000_000, seed=55)
+    idx = np.nonzero(np.all(np.abs(s[:, :3] - s[1326879, :3]) < 0.03, axis=1))[0]
+    sub = s[idx]
+    quad = sorted(int(np.searchsorted(idx, o)) for o in (1326879, 2326511, 602170, 279520))
+    rows = {}
+    for k in (16, 24):
+        d = oracle.Diagram(sub, k=k, complete=4)
+        x, ids = d.vertices()
+        a = np.concatenate([ids.astype(np.uint64), x.view(np.uint64)], axis=1)
+        rows[k] = a[np.lexsort(a.T[::-1])]
+        assert quad in np.sort(ids, axis=1).tolist()
+        assert oracle.empty_sphere_violations(sub, x, ids) == 0
+        if k == 24:
+            assert d.stats()["completion_rebuilt_cells"] > 0
+            _, plain_ids = oracle.Diagram(sub, k=k).vertices()
+            assert quad not in np.sort(plain_ids, axis=1).tolist()
+    assert np.array_equal(rows[16], rows[24])

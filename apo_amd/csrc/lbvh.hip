@@ -414,6 +414,186 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
 }
 
 // ---------------------------------------------------------------------------
+// The same pass over key digits with tiles of 6144 keys (8 waves x 12
+// rounds x 64 lanes) for large sorts: fewer tiles, so a shorter look-back
+// chain and longer runs per digit in the scatter.  The keys are staged in
+// LDS by output position and written; each slot then takes its output index
+// (low word) and value (high word), and the values are written: one 48 KB
+// staging buffer for both, so two blocks fit a CU.  Same ranking, so the
+// same stable order.  Below kWideMinKeys keys the 4096-key tiles fill the
+// chip better (1M keys: 0.35 against 0.37 ms for the LBVH; 16M keys: 3.04
+// against 3.28 ms).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kWideWaves = 8;
+constexpr uint32_t kWideRounds = 12;
+constexpr uint32_t kWideThreads = 64u * kWideWaves;
+constexpr uint32_t kWideKeys = kWideThreads * kWideRounds;
+constexpr uint32_t kWideMinKeys = 1u << 22;
+
+__global__ __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_onesweep_wide(const uint64_t* __restrict__ keysIn,
+                                                               const uint32_t* __restrict__ valsIn,
+                                                               uint64_t* __restrict__ keysOut,
+                                                               uint32_t* __restrict__ valsOut, uint32_t n, int pass,
+                                                               const uint32_t* __restrict__ digitHist,
+                                                               uint32_t* lookback, uint32_t* tileCounter,
+                                                               uint32_t numTiles, uint32_t* err) {
+    __shared__ uint64_t sSlot[kWideKeys];
+    __shared__ uint32_t sWave[kWideWaves][256];
+    __shared__ uint32_t sTileStart[256];
+    __shared__ uint32_t sGlobal[256];
+    __shared__ uint32_t sScan[4];
+    __shared__ uint32_t sScan2[4];
+    __shared__ uint32_t sTile;
+
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = laneId();
+    if (tid == 0) sTile = atomicAdd(tileCounter, 1u);
+    for (uint32_t t = tid; t < kWideWaves * 256u; t += kWideThreads) (&sWave[0][0])[t] = 0u;
+    __syncthreads();
+    const uint32_t tile = sTile;
+    if (tile >= numTiles) {
+        if (tid == 0) atomicOr(err, kErrSortTile);
+        return;
+    }
+    const uint32_t base = tile * kWideKeys + wave * (kWideRounds * 64u);
+    const int shift = 8 * pass;
+
+    uint64_t key[kWideRounds];
+    uint32_t val[kWideRounds];
+    uint32_t pos[kWideRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kWideRounds; r++) {
+        const uint32_t idx = base + r * 64u + lane;
+        const bool valid = idx < n;
+        key[r] = valid ? keysIn[idx] : 0ull;
+        val[r] = valid ? valsIn[idx] : 0u;
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kWideRounds; r++) {
+        const bool valid = base + r * 64u + lane < n;
+        const uint32_t d = uint32_t(key[r] >> shift) & 255u;
+        uint64_t peers = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t bb = ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t below = countBelow(peers);
+        const uint32_t prev = sWave[wave][d];
+        waveSync();
+        if (valid && below == 0) sWave[wave][d] = prev + popc64(peers);
+        waveSync();
+        pos[r] = prev + below;
+    }
+    __syncthreads();
+
+    uint32_t tileCount = 0, excl = 0, g = 0, gInc = 0, tInc = 0;
+    const uint32_t d = tid;
+    if (tid < 256u) {
+        for (uint32_t w = 0; w < kWideWaves; w++) {
+            const uint32_t c = sWave[w][d];
+            sWave[w][d] = tileCount;
+            tileCount += c;
+        }
+        uint32_t* lb = lookback + size_t(tile) * 256u + d;
+        if (tile == 0) {
+            __hip_atomic_store(lb, kFlagInc | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(lb, kFlagAgg | tileCount, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int t = int(tile) - 1;
+            uint32_t spins = 0;
+            while (t >= 0) {
+                uint32_t v[kLookBack];
+#pragma unroll
+                for (int j = 0; j < int(kLookBack); j++)
+                    v[j] = t - j >= 0 ? __hip_atomic_load(lookback + size_t(t - j) * 256u + d, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : kFlagInc;
+                int used = 0;
+                bool done = false;
+#pragma unroll
+                for (int j = 0; j < int(kLookBack); j++) {
+                    const uint32_t flag = v[j] & ~kValueMask;
+                    if (flag == 0u) break;
+                    excl += v[j] & kValueMask;
+                    used = j + 1;
+                    if (flag == kFlagInc) {
+                        done = true;
+                        break;
+                    }
+                }
+                if (done) break;
+                t -= used;
+                if (used < int(kLookBack)) {
+                    if (++spins > (1u << 22)) {
+                        atomicOr(err, kErrSortSpin);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            __hip_atomic_store(lb, kFlagInc | (excl + tileCount), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        g = digitHist[d];
+        gInc = waveInclusiveSum(g);
+        tInc = waveInclusiveSum(tileCount);
+        if (lane == 63) {
+            sScan[wave] = gInc;
+            sScan2[wave] = tInc;
+        }
+    }
+    __syncthreads();
+    if (tid < 256u) {
+        uint32_t gOff = 0, tOff = 0;
+        for (uint32_t w = 0; w < wave; w++) {
+            gOff += sScan[w];
+            tOff += sScan2[w];
+        }
+        sGlobal[d] = gOff + gInc - g + excl;
+        sTileStart[d] = tOff + tInc - tileCount;
+    }
+    __syncthreads();
+
+    // keys by output position
+#pragma unroll
+    for (uint32_t r = 0; r < kWideRounds; r++) {
+        const uint32_t idx = base + r * 64u + lane;
+        uint32_t p = kNone;
+        if (idx < n) {
+            const uint32_t dg = uint32_t(key[r] >> shift) & 255u;
+            p = sTileStart[dg] + sWave[wave][dg] + pos[r];
+            if (p < kWideKeys) {
+                sSlot[p] = key[r];
+            } else {
+                atomicOr(err, kErrSortScatter);
+                p = kNone;
+            }
+        }
+        pos[r] = p;
+    }
+    __syncthreads();
+    const uint32_t tileBase = tile * kWideKeys;
+    const uint32_t tileValid = (n - tileBase) < kWideKeys ? (n - tileBase) : kWideKeys;
+    uint32_t* slot32 = reinterpret_cast<uint32_t*>(sSlot);
+    for (uint32_t p = tid; p < tileValid; p += kWideThreads) {
+        const uint64_t k = sSlot[p];
+        const uint32_t dg = uint32_t(k >> shift) & 255u;
+        const uint32_t out = sGlobal[dg] + (p - sTileStart[dg]);
+        if (out < n) keysOut[out] = k;
+        else atomicOr(err, kErrSortScatter);
+        slot32[2u * p] = out;  // the slot's key is read: its low word takes the output index
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kWideRounds; r++)
+        if (pos[r] != kNone) slot32[2u * pos[r] + 1u] = val[r];
+    __syncthreads();
+    for (uint32_t p = tid; p < tileValid; p += kWideThreads) {
+        const uint32_t out = slot32[2u * p];
+        if (out < n) valsOut[out] = slot32[2u * p + 1u];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Karras 2012 hierarchy (lbvh.cu:144-223) + gather of the sorted sites
 // (setLeaves, lbvh.cu:88-114).  Thread i handles internal node i and leaf i.
 // ---------------------------------------------------------------------------
@@ -874,16 +1054,23 @@ hipError_t treeHeightLaunch(const uint32_t* range, uint32_t n, uint32_t internal
     return hipGetLastError();
 }
 
+// Sort pass p of n keys runs the wide kernel: key digits of a large sort.
+bool widePass(uint32_t n, int p) { return p < kRadixPasses && n >= kWideMinKeys; }
+uint32_t passTiles(uint32_t n, int p) {
+    return widePass(n, p) ? (n + kWideKeys - 1u) / kWideKeys : (n + kTileKeys - 1u) / kTileKeys;
+}
+
 size_t sortScratchWords(uint32_t n) {
-    const size_t tiles = (n + kTileKeys - 1) / kTileKeys;
+    const size_t tiles = (n + kTileKeys - 1) / kTileKeys;  // the most any pass has
     return size_t(kMaxRadixPasses) * 256u + kMaxRadixPasses + size_t(kMaxRadixPasses) * tiles * 256u;
 }
 
 hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const Segments& seg, hipStream_t st) {
-    const uint32_t tiles = (n + kTileKeys - 1) / kTileKeys;
     const int passes = seg.molecules > 1u ? radixPassesFor(seg.molecules) : kRadixPasses;
-    // sort counters and look-back
-    const uint32_t zeroWords = uint32_t(passes) * 256u + uint32_t(passes) + uint32_t(passes) * tiles * 256u;
+    // sort counters and look-back (each pass's tiles x 256 words)
+    size_t lbWords = 0;
+    for (int p = 0; p < passes; p++) lbWords += size_t(passTiles(n, p)) * 256u;
+    const uint32_t zeroWords = uint32_t(passes) * 256u + uint32_t(passes) + uint32_t(lbWords);
     if (seg.molecules > 1u) {
         hipLaunchKernelGGL(k_scene_segments, dim3(seg.molecules), dim3(256), 0, st, sitesIn, seg.offsets, b.scene,
                            b.sortScratch, zeroWords);
@@ -903,13 +1090,19 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     uint32_t* vin = b.idxA;
     uint64_t* kout = b.codesB;
     uint32_t* vout = b.idxB;
+    size_t lbOff = 0;
     for (int p = 0; p < passes; p++) {
-        if (p < kRadixPasses)
+        const uint32_t tiles = passTiles(n, p);
+        if (widePass(n, p))
+            hipLaunchKernelGGL(k_onesweep_wide, dim3(tiles), dim3(kWideThreads), 0, st, kin, vin, kout, vout, n, p,
+                               hist + p * 256, lookback + lbOff, tileCounters + p, tiles, b.err);
+        else if (p < kRadixPasses)
             hipLaunchKernelGGL(k_onesweep<false>, dim3(tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n, p, seg,
-                               hist + p * 256, lookback + size_t(p) * tiles * 256u, tileCounters + p, tiles, b.err);
+                               hist + p * 256, lookback + lbOff, tileCounters + p, tiles, b.err);
         else
             hipLaunchKernelGGL(k_onesweep<true>, dim3(tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n, p, seg,
-                               hist + p * 256, lookback + size_t(p) * tiles * 256u, tileCounters + p, tiles, b.err);
+                               hist + p * 256, lookback + lbOff, tileCounters + p, tiles, b.err);
+        lbOff += size_t(tiles) * 256u;
         uint64_t* tk = kin; kin = kout; kout = tk;
         uint32_t* tv = vin; vin = vout; vout = tv;
     }

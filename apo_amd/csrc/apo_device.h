@@ -111,6 +111,23 @@ __device__ __forceinline__ bool intersectD(D4 a, D4 b) { return ltD(sphereDistD(
 struct Box { F3 mn, mx; };
 __device__ __forceinline__ Box selBox(bool c, const Box& a, const Box& b) { return {selF3(c, a.mn, b.mn), selF3(c, a.mx, b.mx)}; }
 
+// Can the box hold the centre of a site that the bisector search's
+// cylinder test (distance to the axis through o along unit u, plus the
+// site's radius, below cylR) accepts?  A conservative bound for radii >= 0:
+// the distance of the box's bounding sphere to the axis (a cross product, no
+// cancellation) against cylR plus 2e-3 of the farthest box point's distance
+// to o, twice what the FP32 rounding of the leaf test's distance can lose
+// (its a - v^2 cancels: at most sqrt(15 x 2^-24) of that distance).
+__device__ __forceinline__ bool boxNearAxis(const Box& b, F3 o, F3 u, float cylR) {
+    const F3 c{(b.mn.x + b.mx.x) * .5f, (b.mn.y + b.mx.y) * .5f, (b.mn.z + b.mx.z) * .5f};
+    const F3 hv{(b.mx.x - b.mn.x) * .5f, (b.mx.y - b.mn.y) * .5f, (b.mx.z - b.mn.z) * .5f};
+    const float h = lenf(hv);
+    const F3 w = subf(c, o);
+    const F3 x{w.y * u.z - w.z * u.y, w.z * u.x - w.x * u.z, w.x * u.y - w.y * u.x};
+    const float lmax = lenf(w) + h;
+    return lenf(x) - h < cylR * 1.000001f + 2e-3f * lmax;
+}
+
 __device__ __forceinline__ F3 closestPoint(const Box& b, F4 s) {
     return {__builtin_fminf(__builtin_fmaxf(s.x, b.mn.x), b.mx.x),
             __builtin_fminf(__builtin_fmaxf(s.y, b.mn.y), b.mx.y),

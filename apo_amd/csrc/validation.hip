@@ -911,8 +911,13 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
 // One step of a lane's bisector traversal (algorithm.inl:1019-1176).
 // Returns the leaf's site when it is a candidate: other than i, bk, inside
 // the search sphere and the cylinder around the bisector axis.
+// With axisPrune (no negative radius in the scene) a subtree whose box
+// cannot hold a site passing the cylinder test is not entered (boxNearAxis):
+// no candidate is lost and the others are visited in the same order, so the
+// search returns the same site.
 __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, const VStack& stNode, uint32_t i, uint32_t bk,
-                                                 F3 fci, F3 fck, F4 bs, float cylR, Trav& t, uint32_t& flags) {
+                                                 F3 fci, F3 fck, F4 bs, float cylR, bool axisPrune, Trav& t,
+                                                 uint32_t& flags) {
     const uint32_t lane = laneId();
     const uint32_t n = c.n;
     const uint32_t node = t.node;
@@ -934,6 +939,11 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, const VStack&
             closeChild = lc ? L0 : R0;
             ef = minFar < 0.f;
             ec = minClose < 0.f;
+            if (axisPrune && (ef || ec)) {
+                const F3 u = normalizef(subf(fck, fci));
+                if (ef) ef = boxNearAxis(selBox(lc, rb, lb), fci, u, cylR);
+                if (ec) ec = boxNearAxis(selBox(lc, lb, rb), fci, u, cylR);
+            }
         }
         if (ec || ef) {
             t.node = ec ? closeChild : farChild;
@@ -977,7 +987,7 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, const VStack&
 __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorLds& S, const CellLds& L,
                                                      const WaveCell& w, uint32_t i, const D4& si, uint32_t root,
                                                      bool need, uint32_t bk, D4 sk, F3 fci, F3 fck, F4 bsph,
-                                                     float cylR, uint32_t* spill, uint32_t& flags) {
+                                                     float cylR, bool axisPrune, uint32_t* spill, uint32_t& flags) {
     const uint32_t lane = laneId();
     const VStack st{stackRing(S.cell), spill};
     S.res[lane] = kNone;
@@ -1052,7 +1062,8 @@ __device__ __forceinline__ uint32_t bisectorSearches(const CellCtx& c, BisectorL
             if (give) tr.sb++;
             waveSync();
         }
-        if (tr.live && cand == kNone) cand = bisectorStep(c, st, i, bk, fci, fck, bsph, cylR, tr, flags);
+        if (tr.live && cand == kNone)
+            cand = bisectorStep(c, st, i, bk, fci, fck, bsph, cylR, axisPrune, tr, flags);
     }
     waveSync();
     return need && found == kNone ? S.res[lane] : found;
@@ -1100,6 +1111,7 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
     const float kr = c.knownRadii[i];
     const D4 si = loadSite(c, i);
     const uint32_t root = rootNode(c, cellMolecule(c, i));
+    const bool axisPrune = c.scene[cellMolecule(c, i)].rmin >= 0.0;
     const F3 fci{float(si.x), float(si.y), float(si.z)};
     const float fri = float(si.w);
     bool didChange = false;
@@ -1131,7 +1143,7 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
         }
         // the first site of each lane's traversal whose trisector with i, bk
         // has two points, the second clear of the cell's neighbours
-        found = bisectorSearches(c, S, L, w, i, si, root, need, bk, sk, fci, fck, bsph, cylR, spill, flags);
+        found = bisectorSearches(c, S, L, w, i, si, root, need, bk, sk, fci, fck, bsph, cylR, axisPrune, spill, flags);
         if (need && found == kNone) S.bis[bid] = kNone;
         const uint64_t m = ballot(found != kNone);
         waveSync();

@@ -28,7 +28,7 @@ class OrcConfig(ctypes.Structure):
     _fields_ = [(name, ctypes.c_uint32) for name in (
         "k", "vertex_capacity", "edge_capacity", "new_edge_capacity", "bisector_capacity",
         "max_iterations", "max_restarts", "threads", "stage_mask", "edge_bound", "fallback_k", "knn_passes",
-        "complete", "lanes")]
+        "complete", "lanes", "bisector_bound")]
 
 
 def build() -> str:

@@ -113,6 +113,19 @@ inline bool intersectD(D4 a, D4 b) { return ltD(sphereDistD(a, b), 0.0); }
 // ---------------------------------------------------------------------------
 struct Box { F3 mn, mx; };
 
+// The bisector search's conservative subtree test (validation.hip,
+// boxNearAxis): can the box hold the centre of a site whose distance to the
+// axis through o along u, plus its radius (>= 0), is below cylR?
+inline bool boxNearAxis(const Box& b, F3 o, F3 u, float cylR) {
+    const F3 c{(b.mn.x + b.mx.x) * .5f, (b.mn.y + b.mx.y) * .5f, (b.mn.z + b.mx.z) * .5f};
+    const F3 hv{(b.mx.x - b.mn.x) * .5f, (b.mx.y - b.mn.y) * .5f, (b.mx.z - b.mn.z) * .5f};
+    const float h = lenf(hv);
+    const F3 w = subf(c, o);
+    const F3 x{w.y * u.z - w.z * u.y, w.z * u.x - w.x * u.z, w.x * u.y - w.y * u.x};
+    const float lmax = lenf(w) + h;
+    return lenf(x) - h < cylR * 1.000001f + 2e-3f * lmax;
+}
+
 inline F3 closestPoint(const Box& b, F4 s) {
     return {std::fmin(std::fmax(s.x, b.mn.x), b.mx.x), std::fmin(std::fmax(s.y, b.mn.y), b.mx.y),
             std::fmin(std::fmax(s.z, b.mn.z), b.mx.z)};
@@ -1258,7 +1271,7 @@ uint32_t edgeSearch(const Context& C, const Cell& cell, uint32_t i, const D4& si
 
 // Contributor search along a bisector (algorithm.inl:1019-1176)
 uint32_t bisectorSearch(const Context& C, const Cell& cell, uint32_t i, const D4& si, uint32_t bk, F3 fci, F3 fck,
-                        F4 bs, float cylR, uint32_t& flags) {
+                        F4 bs, float cylR, bool axisPrune, uint32_t& flags) {
     const Lbvh& b = *C.bvh;
     const uint32_t n = b.n;
     const D4 sk = C.S[bk];
@@ -1274,7 +1287,14 @@ uint32_t bisectorSearch(const Context& C, const Cell& cell, uint32_t i, const D4
             const uint32_t farChild = lc ? R : L;
             const float minClose = lc ? dL : dR;
             const uint32_t closeChild = lc ? L : R;
-            const bool ef = minFar < 0.f, ec = minClose < 0.f;
+            bool ef = minFar < 0.f, ec = minClose < 0.f;
+            // subtrees that cannot hold a cylinder candidate (no candidate
+            // is lost, the rest keep their order: the same result)
+            if (axisPrune && (ef || ec)) {
+                const F3 u = normalizef(subf(fck, fci));
+                if (ef) ef = boxNearAxis(b.box(farChild), fci, u, cylR);
+                if (ec) ec = boxNearAxis(b.box(closeChild), fci, u, cylR);
+            }
             if (!ef && !ec) break;
             node = ec ? closeChild : farChild;
             if (ec && ef) {
@@ -1497,6 +1517,10 @@ void stageEdgeValidation(Context& C, const std::vector<uint32_t>& tasks) {
 void stageBisectorValidation(Context& C, uint64_t& changedCells) {
     const int64_t n = C.n;
     std::atomic<uint64_t> changedCount{0};
+    // the subtree test assumes no negative radius (the scene's minimum)
+    double rmin = DBL_MAX;
+    for (int64_t j = 0; j < n; j++) rmin = std::fmin(rmin, C.S[j].w);
+    const bool axisPrune = C.cfg.bisector_bound != 0 && rmin >= 0.0;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t ii = 0; ii < n; ii++) {
         const uint32_t i = uint32_t(ii);
@@ -1543,7 +1567,7 @@ void stageBisectorValidation(Context& C, uint64_t& changedCells) {
                 const float radius = lenf(subf(center, fci)) + std::fmax(fri, frk);
                 const F4 bsph{center.x, center.y, center.z, radius};
                 const float cylR = std::fmax(fri, frk);
-                const uint32_t c = bisectorSearch(C, cell, i, si, bk, fci, fck, bsph, cylR, flags);
+                const uint32_t c = bisectorSearch(C, cell, i, si, bk, fci, fck, bsph, cylR, axisPrune, flags);
                 if (c == kNone) bis[bid] = kNone;
                 else found[nFound++] = c;
             }
@@ -1898,6 +1922,7 @@ void orc_default_config(orc_config* cfg) {
     cfg->knn_passes = 1;
     cfg->complete = 0;
     cfg->lanes = 64;
+    cfg->bisector_bound = 1;
 }
 
 static orc_diagram* buildDiagram(const float* xyzr, uint32_t n, const orc_config* cfgIn, const uint8_t* classes,

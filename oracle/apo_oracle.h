@@ -91,6 +91,9 @@ typedef struct orc_config {
     uint32_t lanes;             /* order of the validation loops: 64 (default) steps of 64 records with
                                    contributors applied lowest lane first, the HIP kernels' order; 32 the
                                    reference's 32-lane warps visited highest lane first (utils.cuh:17-32) */
+    uint32_t bisector_bound;    /* bisector searches skip subtrees whose box cannot hold a site passing the
+                                   cylinder test (1, default, as the HIP kernels) or search as the reference
+                                   (0); results are identical either way */
 } orc_config;
 
 void orc_default_config(orc_config* cfg);

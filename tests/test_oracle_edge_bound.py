@@ -43,3 +43,22 @@ def test_bounded_search_matches_reference_search(name, mode):
     got = oracle.Diagram(sites, threads=4, edge_bound=mode)
     _same(ref, got)
 
+
+
+@pytest.mark.parametrize("name", ["loguniform", "protein"])
+def test_bisector_bound_keeps_results(name):
+    """The bisector searches' subtree test (a box that cannot hold a site
+    within the cylinder is skipped) changes no result: every cell record,
+    count, flag and vertex equals the reference's unpruned search."""
+    sites = (datasets.loguniform_sites(20_000, seed=21) if name == "loguniform"
+             else datasets.protein_like_sites(20_000, seed=22))
+    a = oracle.Diagram(sites, bisector_bound=1)
+    b = oracle.Diagram(sites, bisector_bound=0)
+    ca, Va, Ea = a.all_cells()
+    cb, Vb, Eb = b.all_cells()
+    for key in ("status", "vertex_nb", "edge_nb", "flags"):
+        assert np.array_equal(ca[key], cb[key]), key
+    assert np.array_equal(Va, Vb) and np.array_equal(Ea, Eb)
+    xa, ia = a.vertices()
+    xb, ib = b.vertices()
+    assert np.array_equal(ia, ib) and np.array_equal(xa.view(np.uint64), xb.view(np.uint64))

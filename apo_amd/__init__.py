@@ -272,6 +272,13 @@ class AlgorithmGPU:
         _lib.check(self._ctx, _lib.lib().apo_cell_radii(self._ctx, _lib.fptr(out)))
         return out
 
+    def proposal_radii(self) -> np.ndarray:
+        """Search radii of the last build's sites (input order) when it ran
+        the range-query cell pass (proposal_radius, or radii given)."""
+        out = np.zeros(len(self.sites), np.float32)
+        _lib.check(self._ctx, _lib.lib().apo_proposal_radii(self._ctx, _lib.fptr(out)))
+        return out
+
     def proposal_classes(self) -> np.ndarray:
         """Classes the learned proposal gave each site in the last build."""
         out = np.zeros(len(self.sites), np.uint8)

@@ -27,7 +27,8 @@ class Config(ctypes.Structure):
                 ("proposal_cost", ctypes.c_float), ("graph", ctypes.c_uint32), ("cover_ratio", ctypes.c_float),
                 ("waves_cells", ctypes.c_uint32), ("waves_vertex", ctypes.c_uint32), ("waves_edge", ctypes.c_uint32),
                 ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32),
-                ("proposal_max_k", ctypes.c_uint32), ("knn_passes", ctypes.c_uint32), ("complete", ctypes.c_uint32)]
+                ("proposal_max_k", ctypes.c_uint32), ("knn_passes", ctypes.c_uint32), ("complete", ctypes.c_uint32),
+                ("proposal_radius", ctypes.c_uint32)]
 
 
 class Precision(ctypes.Structure):
@@ -100,12 +101,13 @@ SIGNATURES = {
                                              ctypes.c_uint64, _u64p]),
     "apo_get_cell_topology": (ctypes.c_int, [_vp, _u64p, _u32p, ctypes.c_uint64, _u64p, _u32p, ctypes.c_uint64]),
     "apo_proposal_labels": (ctypes.c_int, [_vp, _f32p, _u32, _f32p, _u8p]),
-    "apo_proposal_train": (ctypes.c_int, [_vp, _f32p, _u8p, ctypes.c_uint64, _u32, _u32, ctypes.c_float, _u32,
-                                          _f32p, _f32p, _f32p]),
-    "apo_proposal_gradient": (ctypes.c_int, [_vp, _f32p, _f32p, _u8p, _u32, _f32p, _f32p]),
+    "apo_proposal_train": (ctypes.c_int, [_vp, _f32p, _u8p, _f32p, ctypes.c_uint64, _u32, _u32, ctypes.c_float,
+                                          _u32, _f32p, _f32p, _f32p]),
+    "apo_proposal_gradient": (ctypes.c_int, [_vp, _f32p, _f32p, _u8p, _f32p, _u32, _f32p, _f32p]),
     "apo_proposal_set": (ctypes.c_int, [_vp, _f32p]),
     "apo_proposal_infer": (ctypes.c_int, [_vp, _f32p, _u32, _u8p, _f32p]),
     "apo_proposal_classes": (ctypes.c_int, [_vp, _u8p]),
+    "apo_proposal_radii": (ctypes.c_int, [_vp, _f32p]),
     "apo_proposal_set_classes": (ctypes.c_int, [_vp, _u8p, _u32]),
     "apo_proposal_set_radii": (ctypes.c_int, [_vp, _f32p, _u32]),
     "apo_cell_radii": (ctypes.c_int, [_vp, _f32p]),

@@ -161,7 +161,8 @@ struct apo_ctx {
     float* givenRadii = nullptr;  // search radii given by the caller (input order): range-query cell pass
     uint32_t givenRN = 0;
     uint32_t givenRCap = 0;
-    float* radiiSorted = nullptr;  // the same in sorted order, capacity capP
+    float* radiiSorted = nullptr;  // the same in sorted order (or the radius head's), capacity capP
+    float* d16 = nullptr;          // 16th-neighbour centre distance per sorted site, capacity capP
     uint8_t* givenCls = nullptr;  // classes given by the caller (input order), used instead of the MLP
     uint32_t givenN = 0;          // their site count (0: none given)
     uint32_t givenCap = 0;
@@ -288,6 +289,7 @@ apo_status ensureProposalBuffers(apo_ctx* ctx, uint32_t n) {
     APO_HIP(alloc(ctx->cls, n));
     APO_HIP(alloc(ctx->labels, n));
     APO_HIP(alloc(ctx->radiiSorted, n));
+    APO_HIP(alloc(ctx->d16, n));
     ctx->capP = n;
     return APO_OK;
 }
@@ -412,6 +414,18 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     if (ranged && (ctx->cfg.stage_mask & 1u)) {
         // each cell updated with its whole range list, the known radius its
         // range when the list holds fewer sites than its capacity
+        APO_HIP(mark(7));
+        APO_HIP(cellsLaunch(cc, ctx->knn, Kk, ctx->counters + kCellQueue,
+                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st, nullptr,
+                            ctx->cfg.proposal_fallback_k, false, ctx->radiiSorted));
+    } else if (learned && ctx->cfg.proposal_radius && !ctx->givenN && (ctx->cfg.stage_mask & 1u)) {
+        // the radius head: features of the 16 nearest, each site's search
+        // radius, then the range-query lists below it and the ranged cell pass
+        APO_HIP(featuresLaunch(cc, ctx->knn, Kk, ctx->feat, st, ctx->d16));
+        APO_HIP(mlpInferLaunch(ctx->feat, n, ctx->pw, ctx->cls, nullptr, ctx->cfg.proposal_cost,
+                               ctx->cfg.proposal_fallback_k, ctx->cfg.proposal_max_k, st, ctx->d16,
+                               ctx->radiiSorted));
+        APO_HIP(knnLaunch(cc, Kk, ctx->knn, st, 0u, ctx->radiiSorted));
         APO_HIP(mark(7));
         APO_HIP(cellsLaunch(cc, ctx->knn, Kk, ctx->counters + kCellQueue,
                             persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st, nullptr,
@@ -720,6 +734,7 @@ void apo_default_config(apo_config* cfg) {
     cfg->proposal_max_k = 64;
     cfg->knn_passes = 1;
     cfg->complete = 0;
+    cfg->proposal_radius = 0;
 }
 
 // pcg hash of apo/src/core/utils.cpp:6-12, scaled by 2^-32
@@ -791,7 +806,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
-                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->stuck, ctx->givenRadii, ctx->radiiSorted, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
+                    ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->stuck, ctx->givenRadii, ctx->radiiSorted, ctx->d16, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1600,9 +1615,9 @@ apo_status apo_proposal_labels(apo_ctx* ctx, const float* xyzr, uint32_t n, floa
     return APO_OK;
 }
 
-apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t* labels, uint64_t count,
-                              uint32_t epochs, uint32_t batch, float lr, uint32_t seed, const float* init,
-                              float* params, float* epoch_loss) {
+apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t* labels,
+                              const float* radius_targets, uint64_t count, uint32_t epochs, uint32_t batch, float lr,
+                              uint32_t seed, const float* init, float* params, float* epoch_loss) {
     if (!ctx || !features || !labels || !params || count == 0 || batch == 0) return APO_ERR_INVALID;
     APO_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
@@ -1615,9 +1630,11 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
     const uint32_t m = uint32_t(keep.size());
     std::vector<float> f(size_t(m) * kFeatures);
     std::vector<uint8_t> y(m);
+    std::vector<float> rt(radius_targets ? m : 0u);
     for (uint32_t q = 0; q < m; q++) {
         std::memcpy(&f[size_t(q) * kFeatures], features + size_t(keep[q]) * kFeatures, kFeatures * 4);
         y[q] = uint8_t(proposalClass(labels[keep[q]]));
+        if (radius_targets) rt[q] = radius_targets[keep[q]];
     }
     // parameters: given, or uniform(+-sqrt(6 / fan_in)) from the seed
     std::vector<float> w(kProposalParams, 0.f);
@@ -1634,7 +1651,7 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
         off += kFeatures * kHidden + kHidden;
         fill(off, kHidden, kHidden * kHidden);
         off += kHidden * kHidden + kHidden;
-        fill(off, kHidden, kHidden * kProposalClasses);
+        fill(off, kHidden, kHidden * kProposalOutputs);
     }
     std::vector<float> wq(kProposalParams);
     for (uint32_t q = 0; q < kProposalParams; q++) {
@@ -1644,11 +1661,15 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
         std::memcpy(&wq[q], &b, 4);
     }
     const uint32_t maxWaves = trainWaves(batch < m ? batch : m);
-    DeviceTemp<float> dF, dW, dWq, dM, dV, dPart, dLossPart, dLoss, dB1, dB2, dB3;
+    DeviceTemp<float> dF, dW, dWq, dM, dV, dPart, dLossPart, dLoss, dB1, dB2, dB3, dR;
     DeviceTemp<uint16_t> dW1, dW2, dW3;
     DeviceTemp<uint8_t> dY;
     DeviceTemp<uint32_t> dOrder;
     APO_HIP(dF.alloc(f.size()));
+    if (radius_targets) {
+        APO_HIP(dR.alloc(m));
+        APO_HIP(hipMemcpyAsync(dR.p, rt.data(), size_t(m) * 4, hipMemcpyHostToDevice, st));
+    }
     APO_HIP(dY.alloc(m));
     APO_HIP(dOrder.alloc(m));
     APO_HIP(dW.alloc(kProposalParams));
@@ -1671,8 +1692,8 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
     std::vector<float> zeros(kProposalParams, 0.f);
     APO_HIP(hipMemcpyAsync(dM.p, zeros.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
     APO_HIP(hipMemcpyAsync(dV.p, zeros.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
-    const TrainBuffers tb{dF.p, dY.p, dW.p, dWq.p, dM.p, dV.p, dPart.p, dLossPart.p, dLoss.p, maxWaves,
-                          ProposalWeights{dW1.p, dW2.p, dW3.p, dB1.p, dB2.p, dB3.p}};
+    const TrainBuffers tb{dF.p, dY.p, radius_targets ? dR.p : nullptr, dW.p, dWq.p, dM.p, dV.p, dPart.p,
+                          dLossPart.p, dLoss.p, maxWaves, ProposalWeights{dW1.p, dW2.p, dW3.p, dB1.p, dB2.p, dB3.p}};
     APO_HIP(packWeightsLaunch(dW.p, tb.pw, st));
     std::vector<uint32_t> order(m);
     for (uint32_t q = 0; q < m; q++) order[q] = q;
@@ -1697,7 +1718,7 @@ apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t
 }
 
 apo_status apo_proposal_gradient(apo_ctx* ctx, const float* params, const float* features, const uint8_t* classes,
-                                 uint32_t count, float* grad, float* loss) {
+                                 const float* radius_targets, uint32_t count, float* grad, float* loss) {
     if (!ctx || !params || !features || !classes || !grad || count == 0) return APO_ERR_INVALID;
     for (uint32_t q = 0; q < count; q++)
         if (classes[q] >= kProposalClasses) return fail(ctx, APO_ERR_INVALID, "class out of range");
@@ -1713,11 +1734,15 @@ apo_status apo_proposal_gradient(apo_ctx* ctx, const float* params, const float*
     std::vector<uint32_t> order(count);
     for (uint32_t q = 0; q < count; q++) order[q] = q;
     const uint32_t waves = trainWaves(count);
-    DeviceTemp<float> dF, dW, dWq, dPart, dLossPart, dB1, dB2, dB3;
+    DeviceTemp<float> dF, dW, dWq, dPart, dLossPart, dB1, dB2, dB3, dR;
     DeviceTemp<uint16_t> dW1, dW2, dW3;
     DeviceTemp<uint8_t> dY;
     DeviceTemp<uint32_t> dOrder;
     APO_HIP(dF.alloc(size_t(count) * kFeatures));
+    if (radius_targets) {
+        APO_HIP(dR.alloc(count));
+        APO_HIP(hipMemcpyAsync(dR.p, radius_targets, size_t(count) * 4, hipMemcpyHostToDevice, st));
+    }
     APO_HIP(dY.alloc(count));
     APO_HIP(dOrder.alloc(count));
     APO_HIP(dW.alloc(kProposalParams));
@@ -1735,8 +1760,8 @@ apo_status apo_proposal_gradient(apo_ctx* ctx, const float* params, const float*
     APO_HIP(hipMemcpyAsync(dOrder.p, order.data(), size_t(count) * 4, hipMemcpyHostToDevice, st));
     APO_HIP(hipMemcpyAsync(dW.p, params, kProposalParams * 4, hipMemcpyHostToDevice, st));
     APO_HIP(hipMemcpyAsync(dWq.p, wq.data(), kProposalParams * 4, hipMemcpyHostToDevice, st));
-    const TrainBuffers tb{dF.p, dY.p, dW.p, dWq.p, nullptr, nullptr, dPart.p, dLossPart.p, nullptr, waves,
-                          ProposalWeights{dW1.p, dW2.p, dW3.p, dB1.p, dB2.p, dB3.p}};
+    const TrainBuffers tb{dF.p, dY.p, radius_targets ? dR.p : nullptr, dW.p, dWq.p, nullptr, nullptr, dPart.p,
+                          dLossPart.p, nullptr, waves, ProposalWeights{dW1.p, dW2.p, dW3.p, dB1.p, dB2.p, dB3.p}};
     APO_HIP(packWeightsLaunch(dW.p, tb.pw, st));
     APO_HIP(trainGradLaunch(tb, dOrder.p, count, 1.f / float(count), st));
     std::vector<float> part(size_t(waves) * kProposalParams), lp(waves);
@@ -1789,12 +1814,12 @@ apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t coun
     DeviceTemp<uint8_t> dC;
     APO_HIP(dF.alloc(size_t(count) * kFeatures));
     APO_HIP(dC.alloc(count));
-    if (logits) APO_HIP(dL.alloc(size_t(count) * kProposalClasses));
+    if (logits) APO_HIP(dL.alloc(size_t(count) * kProposalOutputs));
     APO_HIP(hipMemcpyAsync(dF.p, features, size_t(count) * kFeatures * 4, hipMemcpyHostToDevice, st));
     APO_HIP(mlpInferLaunch(dF.p, count, ctx->pw, dC.p, logits ? dL.p : nullptr, ctx->cfg.proposal_cost,
                            ctx->cfg.proposal_fallback_k, ctx->cfg.proposal_max_k, st));
     APO_HIP(hipMemcpyAsync(classes, dC.p, count, hipMemcpyDeviceToHost, st));
-    if (logits) APO_HIP(hipMemcpyAsync(logits, dL.p, size_t(count) * kProposalClasses * 4, hipMemcpyDeviceToHost, st));
+    if (logits) APO_HIP(hipMemcpyAsync(logits, dL.p, size_t(count) * kProposalOutputs * 4, hipMemcpyDeviceToHost, st));
     APO_HIP(hipStreamSynchronize(st));
     return APO_OK;
 }
@@ -1862,6 +1887,19 @@ apo_status apo_proposal_set_classes(apo_ctx* ctx, const uint8_t* classes, uint32
     APO_HIP(hipMemcpyAsync(ctx->givenCls, classes, n, hipMemcpyHostToDevice, ctx->stream));
     APO_HIP(hipStreamSynchronize(ctx->stream));
     ctx->givenN = n;
+    return APO_OK;
+}
+
+apo_status apo_proposal_radii(apo_ctx* ctx, float* radii) {
+    if (!ctx || !radii) return APO_ERR_INVALID;
+    if (!ctx->built || !ctx->stats.proposal || !(ctx->givenRN || (ctx->cfg.proposal_radius && !ctx->givenN)))
+        return fail(ctx, APO_ERR_STATE, "no build with search radii");
+    const size_t n = ctx->n;
+    std::vector<uint32_t> idx(n);
+    std::vector<float> r(n);
+    APO_HIP(hipMemcpy(idx.data(), ctx->idxA, n * 4, hipMemcpyDeviceToHost));
+    APO_HIP(hipMemcpy(r.data(), ctx->radiiSorted, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) radii[idx[i]] = r[i];
     return APO_OK;
 }
 

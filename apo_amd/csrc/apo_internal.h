@@ -301,10 +301,16 @@ hipError_t closedEdgesWriteLaunch(const CellCtx& c, const OutputBuffers& o, cons
 // Vertex offsets of each molecule of a batch (molecules + 1 values).
 hipError_t moleculeVerticesLaunch(const CellCtx& c, const OutputBuffers& o, uint64_t* out, hipStream_t st);
 
-// Learned neighbour-count proposal (learned.hip)
+// Learned proposal (learned.hip): 8 neighbour-count class logits and one
+// search-radius output, log2 of the radius over d16 (the centre distance of
+// the 16th nearest neighbour)
 constexpr uint32_t kFeatures = 52, kHidden = 64, kProposalClasses = 8;
+constexpr uint32_t kProposalOutputs = kProposalClasses + 1;
+constexpr uint32_t kRadiusOutput = kProposalClasses;
 constexpr uint32_t kProposalParams = kFeatures * kHidden + kHidden + kHidden * kHidden + kHidden +
-                                     kHidden * kProposalClasses + kProposalClasses;
+                                     kHidden * kProposalOutputs + kProposalOutputs;
+// weight of the radius head's squared error beside the cross-entropy
+constexpr float kRadiusLossWeight = 0.25f;
 constexpr uint32_t kLabelBuried = 255;
 enum CellsMode : uint32_t { kProposal = 0, kLabels = 1 };
 // Neighbours a class asks for; the last class (not certified within 64)
@@ -332,6 +338,7 @@ struct ProposalWeights {
 struct TrainBuffers {
     const float* feat;
     const uint8_t* cls;
+    const float* radius;  // radius-head targets per row (NaN: none), or null
     float *w, *wq, *m, *v;
     float* partial;      // maxWaves x kProposalParams
     float* lossPartial;  // maxWaves
@@ -341,16 +348,21 @@ struct TrainBuffers {
 };
 hipError_t cellsKLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, const uint8_t* cls, uint8_t* labels,
                         uint32_t mode, uint32_t fallbackK, hipStream_t st);
-hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st);
+// d16 (may be null) receives each site's 16th-neighbour centre distance.
+hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st,
+                          float* d16 = nullptr);
 // cls[i] = given[indices[i]]: caller-given classes (input order) in sorted order.
 hipError_t gatherClassesLaunch(const uint32_t* indices, const uint8_t* given, uint32_t n, uint8_t* cls,
                                hipStream_t st);
 // radii[i] = given[indices[i]]: caller-given search radii (input order) in sorted order.
 hipError_t gatherRadiiLaunch(const uint32_t* indices, const float* given, uint32_t n, float* radii, hipStream_t st);
 // Classes of n sites; classes asking for more than maxK neighbours are
-// replaced by the last class (not certified within maxK).
+// replaced by the last class (not certified within maxK).  logits (may be
+// null): kProposalOutputs per site.  radii (may be null, with d16): the
+// radius head's search radius d16 x 2^out per site.
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
-                          float cost, uint32_t fallbackK, uint32_t maxK, hipStream_t st);
+                          float cost, uint32_t fallbackK, uint32_t maxK, hipStream_t st, const float* d16 = nullptr,
+                          float* radii = nullptr);
 hipError_t packWeightsLaunch(const float* params, const ProposalWeights& w, hipStream_t st);
 // Waves (and partial gradients) of a batch of `rows`.
 uint32_t trainWaves(uint32_t rows);

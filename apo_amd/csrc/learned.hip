@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64) void k_cells_k(CellCtx c, const uint32_t* __res
 //   [51]           (largest - smallest gap) / d16
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_features(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
-                                                  float* __restrict__ feat) {
+                                                  float* __restrict__ feat, float* __restrict__ d16Out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.n) return;
     const F4 s = c.sorted[i];
@@ -116,6 +116,7 @@ __global__ __launch_bounds__(256) void k_features(CellCtx c, const uint32_t* __r
             break;
         }
     }
+    if (d16Out) d16Out[i] = d16;
     const float inv = d16 > 0.f ? 1.f / d16 : 0.f;
     f[0] = s.w * inv;
     float larger = 0.f, rsum = 0.f, gmin = FLT_MAX, gmax = -FLT_MAX;
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(256) void k_features(CellCtx c, const uint32_t* __r
 // ---------------------------------------------------------------------------
 // Inference on the matrix cores.  Weights are transposed BF16 [out][in] with
 // in = 64 (layer 1 zero-padded from 52) and out = 64, 64, 16 (layer 3
-// zero-padded from 8).  v_mfma_f32_16x16x16_bf16 lane layout: A[row = l % 16]
+// zero-padded from 9: 8 class logits, then the radius head).  v_mfma_f32_16x16x16_bf16 lane layout: A[row = l % 16]
 // [k = 4 (l / 16) + 0..3], B[k = 4 (l / 16) + 0..3][col = l % 16],
 // D[row = 4 (l / 16) + 0..3][col = l % 16].
 // ---------------------------------------------------------------------------
@@ -201,7 +202,8 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
                                                   const float* b1, const uint16_t* w2, const float* b2,
                                                   const uint16_t* w3, const float* b3, uint8_t* cls,
                                                   float* logitsOut, float cost, uint32_t fallbackK,
-                                                  uint32_t maxK) {
+                                                  uint32_t maxK, const float* __restrict__ d16,
+                                                  float* __restrict__ radii) {
     __shared__ uint16_t X[16][kHidden];
     __shared__ uint16_t H[16][kHidden];
     __shared__ float Lg[16][16];
@@ -256,7 +258,10 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
             }
             cls[base + lane] = uint8_t(best);
             if (logitsOut)
-                for (uint32_t q = 0; q < kProposalClasses; q++) logitsOut[size_t(base + lane) * kProposalClasses + q] = Lg[lane][q];
+                for (uint32_t q = 0; q < kProposalOutputs; q++)
+                    logitsOut[size_t(base + lane) * kProposalOutputs + q] = Lg[lane][q];
+            // the radius head: log2 of the search radius over d16
+            if (radii) radii[base + lane] = d16[base + lane] * exp2f(Lg[lane][kRadiusOutput]);
         }
         waveSync();
     }
@@ -264,7 +269,10 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
 
 // ---------------------------------------------------------------------------
 // Training on the matrix cores.  Parameters, flat: W1 [52][64], b1 [64],
-// W2 [64][64], b2 [64], W3 [64][8], b3 [8] (weights [in][out]).
+// W2 [64][64], b2 [64], W3 [64][9], b3 [9] (weights [in][out]).  The loss
+// of a row: softmax cross-entropy over the 8 class logits, plus
+// kRadiusLossWeight x the squared error of the radius head against the
+// row's target (rows without a target, NaN, add none).
 //
 // The forward pass is the inference pass above (BF16 weights and
 // activations, FP32 accumulation on v_mfma_f32_16x16x16_bf16), so the
@@ -277,13 +285,14 @@ __global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ feat
 // ---------------------------------------------------------------------------
 constexpr uint32_t kTrainTiles = 8;
 constexpr uint32_t kOffB1 = kFeatures * kHidden, kOffW2 = kOffB1 + kHidden, kOffB2 = kOffW2 + kHidden * kHidden,
-                   kOffW3 = kOffB2 + kHidden, kOffB3 = kOffW3 + kHidden * kProposalClasses;
+                   kOffW3 = kOffB2 + kHidden, kOffB3 = kOffW3 + kHidden * kProposalOutputs;
 
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
 __global__ __launch_bounds__(64) void k_train_grad(const float* __restrict__ feat, const uint8_t* __restrict__ cls,
+                                                   const float* __restrict__ radiusTarget,
                                                    const uint32_t* __restrict__ order, uint32_t rows,
                                                    float invBatch, const uint16_t* w1t, const float* b1,
                                                    const uint16_t* w2t, const float* b2, const uint16_t* w3t,
@@ -339,19 +348,25 @@ __global__ __launch_bounds__(64) void k_train_grad(const float* __restrict__ fea
                 for (uint32_t q = 0; q < kProposalClasses; q++)
                     Z[lane][q] = (e[q] / sum - (q == y ? 1.f : 0.f)) * invBatch;
                 loss += (m + logf(sum)) - z[y < kProposalClasses ? y : 0u];
+                // the radius head: w (z - t)^2, dL/dz = 2 w (z - t) / batch
+                const float t = radiusTarget ? radiusTarget[order[row]] : __builtin_nanf("");
+                const float zr = Z[lane][kRadiusOutput];
+                const bool has = t == t;
+                Z[lane][kRadiusOutput] = has ? 2.f * kRadiusLossWeight * (zr - t) * invBatch : 0.f;
+                if (has) loss += kRadiusLossWeight * (zr - t) * (zr - t);
             } else {
-                for (uint32_t q = 0; q < kProposalClasses; q++) Z[lane][q] = 0.f;
+                for (uint32_t q = 0; q < kProposalOutputs; q++) Z[lane][q] = 0.f;
             }
-            for (uint32_t q = kProposalClasses; q < 16u; q++) Z[lane][q] = 0.f;
+            for (uint32_t q = kProposalOutputs; q < 16u; q++) Z[lane][q] = 0.f;
         }
         waveSync();
         // dH2 = dZ W3^T through the ReLU
         for (uint32_t cb = 0; cb < 4u; cb++) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             const uint32_t i = cb * 16u + lr16;
-            for (uint32_t ks = 0; ks < kProposalClasses / 4u; ks++) {
+            for (uint32_t ks = 0; ks < (kProposalOutputs + 3u) / 4u; ks++) {
                 const uint32_t o = ks * 4u + lq;
-                acc = mfma4(Z[lr16][o], W3[i * kProposalClasses + o], acc);
+                acc = mfma4(Z[lr16][o], o < kProposalOutputs ? W3[i * kProposalOutputs + o] : 0.f, acc);
             }
             for (int e = 0; e < 4; e++) {
                 const uint32_t r = lq * 4u + e;
@@ -390,7 +405,7 @@ __global__ __launch_bounds__(64) void k_train_grad(const float* __restrict__ fea
         for (uint32_t r = 0; r < 16u; r++) {
             gb1 += D1[r][lane];
             gb2 += D2[r][lane];
-            if (lane < kProposalClasses) gb3 += Z[r][lane];
+            if (lane < kProposalOutputs) gb3 += Z[r][lane];
         }
         waveSync();
     }
@@ -403,11 +418,11 @@ __global__ __launch_bounds__(64) void k_train_grad(const float* __restrict__ fea
                 if (in < kFeatures) p[in * kHidden + out] = g1[rb][cb][e];
                 p[kOffW2 + in * kHidden + out] = g2[rb][cb][e];
             }
-            if (lr16 < kProposalClasses) p[kOffW3 + in * kProposalClasses + lr16] = g3[rb][e];
+            if (lr16 < kProposalOutputs) p[kOffW3 + in * kProposalOutputs + lr16] = g3[rb][e];
         }
     p[kOffB1 + lane] = gb1;
     p[kOffB2 + lane] = gb2;
-    if (lane < kProposalClasses) p[kOffB3 + lane] = gb3;
+    if (lane < kProposalOutputs) p[kOffB3 + lane] = gb3;
     float tot = loss;
     for (int o = 32; o >= 1; o >>= 1) tot += __shfl_xor(tot, o, 64);
     if (lane == 0) lossPartial[blockIdx.x] = tot;
@@ -446,7 +461,7 @@ __global__ void k_pack_weights(const float* p, uint16_t* w1, float* b1, uint16_t
     const float* W2 = B1 + kHidden;
     const float* B2 = W2 + kHidden * kHidden;
     const float* W3 = B2 + kHidden;
-    const float* B3 = W3 + kHidden * kProposalClasses;
+    const float* B3 = W3 + kHidden * kProposalOutputs;
     if (idx < kHidden * kHidden) {
         const uint32_t o = idx / kHidden, in = idx % kHidden;
         w1[idx] = toBf16(in < kFeatures ? W1[in * kHidden + o] : 0.f);
@@ -454,13 +469,13 @@ __global__ void k_pack_weights(const float* p, uint16_t* w1, float* b1, uint16_t
     }
     if (idx < 16u * kHidden) {
         const uint32_t o = idx / kHidden, in = idx % kHidden;
-        w3[idx] = toBf16(o < kProposalClasses ? W3[in * kProposalClasses + o] : 0.f);
+        w3[idx] = toBf16(o < kProposalOutputs ? W3[in * kProposalOutputs + o] : 0.f);
     }
     if (idx < kHidden) {
         b1[idx] = B1[idx];
         b2[idx] = B2[idx];
     }
-    if (idx < 16u) b3[idx] = idx < kProposalClasses ? B3[idx] : 0.f;
+    if (idx < 16u) b3[idx] = idx < kProposalOutputs ? B3[idx] : 0.f;
 }
 
 // ---------------------------------------------------------------------------
@@ -499,18 +514,21 @@ hipError_t gatherRadiiLaunch(const uint32_t* indices, const float* given, uint32
     return hipGetLastError();
 }
 
-hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st) {
+hipError_t featuresLaunch(const CellCtx& c, const uint32_t* knn, uint32_t K, float* feat, hipStream_t st,
+                          float* d16) {
     if (K < 16) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_features, dim3((c.n + 255u) / 256u), dim3(256), 0, st, c, knn, K, feat);
+    hipLaunchKernelGGL(k_features, dim3((c.n + 255u) / 256u), dim3(256), 0, st, c, knn, K, feat, d16);
     return hipGetLastError();
 }
 
 hipError_t mlpInferLaunch(const float* feat, uint32_t n, const ProposalWeights& w, uint8_t* cls, float* logits,
-                          float cost, uint32_t fallbackK, uint32_t maxK, hipStream_t st) {
+                          float cost, uint32_t fallbackK, uint32_t maxK, hipStream_t st, const float* d16,
+                          float* radii) {
+    if (radii && !d16) return hipErrorInvalidValue;
     const uint32_t tiles = (n + 15u) / 16u;
     const uint32_t grid = tiles < 8192u ? tiles : 8192u;
     hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(64), 0, st, feat, n, w.w1, w.b1, w.w2, w.b2, w.w3, w.b3, cls,
-                       logits, cost, fallbackK, maxK);
+                       logits, cost, fallbackK, maxK, d16, radii);
     return hipGetLastError();
 }
 
@@ -526,7 +544,8 @@ hipError_t trainGradLaunch(const TrainBuffers& t, const uint32_t* order, uint32_
                            hipStream_t st) {
     const uint32_t waves = trainWaves(rows);
     if (waves > t.maxWaves) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_train_grad, dim3(waves), dim3(64), 0, st, t.feat, t.cls, order, rows, invBatch, t.pw.w1,
+    hipLaunchKernelGGL(k_train_grad, dim3(waves), dim3(64), 0, st, t.feat, t.cls, t.radius, order, rows, invBatch,
+                       t.pw.w1,
                        t.pw.b1, t.pw.w2, t.pw.b2, t.pw.w3, t.pw.b3, t.wq, t.partial, t.lossPartial);
     return hipGetLastError();
 }

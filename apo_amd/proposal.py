@@ -1,12 +1,17 @@
-"""Learned neighbour-count proposal: labels, training and inference.
+"""Learned proposal: labels, training and inference.
 
-An MLP (52 -> 64 -> 64 -> 8) predicts for each site how many nearest
+An MLP (52 -> 64 -> 64 -> 9) predicts for each site how many nearest
 neighbours its cell needs before the plain security-radius test certifies
-it; builds with the proposal run that many updates per cell and send the
-cells the test does not certify through the plain validation.  Labels are
-the reference's certified k (app/benchmarks/benchmark_contributing_nearest.
-cu:75-113), computed by the library's label pass on the GPU; training runs
-in HIP kernels (apo_proposal_train).
+it (8 class logits) and the search radius it needs (output 8: log2 of the
+radius over d16, the centre distance of its 16th nearest neighbour); builds
+with the proposal run that many updates per cell (or, with
+proposal_radius, every site below the radius through a range query) and
+send the cells the test does not certify through the plain validation.
+Class labels are the reference's certified k (app/benchmarks/
+benchmark_contributing_nearest.cu:75-113), computed by the library's label
+pass on the GPU; radius targets are twice each cell's final radius (the
+radius that certifies it) from a completed plain build; training runs in
+HIP kernels (apo_proposal_train).
 """
 from __future__ import annotations
 
@@ -18,7 +23,8 @@ from . import _lib
 
 FEATURES = 52
 CLASSES = 8
-PARAMS = 8072
+OUTPUTS = 9   # the class logits, then the radius head
+PARAMS = 8137
 # neighbours asked for by each class; the last (not certified within 64) runs 16
 CLASS_K = (20, 24, 28, 32, 40, 48, 64, 16)
 LABEL_BURIED = 255
@@ -57,35 +63,78 @@ def labels(sites: np.ndarray):
     return feat, lab
 
 
-def train(features: np.ndarray, labels_: np.ndarray, epochs: int = 8, batch: int = 2048, lr: float = 3e-3,
-          seed: int = 0, init: np.ndarray | None = None):
-    """Train on the GPU; returns (params (8072,), mean loss per epoch)."""
+# Radius-head targets are clipped to [RADIUS_LO, RADIUS_HI] (log2 of the
+# radius over d16): a range of 2 d16 already holds about 128 sites, more
+# than any neighbour list, so larger radii all mean "not certified by a
+# list" and need no finer prediction.
+RADIUS_LO, RADIUS_HI = -3.0, 1.0
+
+
+def radius_targets(sites: np.ndarray, margin: float = 1e-3) -> np.ndarray:
+    """Radius-head targets of a site set (input order): log2 of twice each
+    cell's final radius, (1 + margin) above, over d16, clipped to
+    [RADIUS_LO, RADIUS_HI]; NaN for buried sites.  One plain build with
+    completion (apo_cell_radii) and its 16 nearest."""
+    from . import AlgorithmGPU
+    s = np.ascontiguousarray(np.asarray(sites, np.float32).reshape(-1, 4))
+    g = AlgorithmGPU(s, k=16, complete=4).build()
+    try:
+        cr = g.cell_radii()
+        knn = g.knns()
+        ss, idx = g.sorted_sites()
+    finally:
+        g.close()
+    # d16: centre distance of each sorted site's last listed neighbour
+    have = knn < len(s)
+    last = np.where(have.any(axis=1), have.shape[1] - 1 - np.argmax(have[:, ::-1], axis=1), 0)
+    j = knn[np.arange(len(s)), last]
+    ok = have.any(axis=1)
+    d = np.zeros(len(s), np.float32)
+    d[ok] = np.linalg.norm(ss[j[ok], :3] - ss[ok, :3], axis=1)
+    d16 = np.zeros(len(s), np.float32)
+    d16[idx] = d
+    t = np.full(len(s), np.nan, np.float32)
+    live = (cr > -np.finfo(np.float32).max) & (d16 > 0)
+    t[live] = np.clip(np.log2(2.0 * np.maximum(cr[live].astype(np.float64), 1e-30) * (1.0 + margin) / d16[live]),
+                      RADIUS_LO, RADIUS_HI)
+    return t
+
+
+def train(features: np.ndarray, labels_: np.ndarray, radius: np.ndarray | None = None, epochs: int = 8,
+          batch: int = 2048, lr: float = 3e-3, seed: int = 0, init: np.ndarray | None = None):
+    """Train on the GPU; returns (params (8137,), mean loss per epoch).
+    radius: radius-head targets (radius_targets), or None for none."""
     f = np.ascontiguousarray(features, np.float32)
     y = np.ascontiguousarray(labels_, np.uint8)
+    r = None if radius is None else np.ascontiguousarray(radius, np.float32)
     params = np.zeros(PARAMS, np.float32)
     loss = np.zeros(epochs, np.float32)
     ini = None if init is None else np.ascontiguousarray(init, np.float32)
     ctx = _ctx()
     try:
-        _lib.check(ctx, _lib.lib().apo_proposal_train(ctx, _lib.fptr(f), _lib.u8ptr(y), len(y), epochs, batch, lr,
-                                                       seed, None if ini is None else _lib.fptr(ini),
+        _lib.check(ctx, _lib.lib().apo_proposal_train(ctx, _lib.fptr(f), _lib.u8ptr(y),
+                                                       None if r is None else _lib.fptr(r), len(y), epochs, batch,
+                                                       lr, seed, None if ini is None else _lib.fptr(ini),
                                                        _lib.fptr(params), _lib.fptr(loss)))
     finally:
         _lib.lib().apo_destroy(ctx)
     return params, loss
 
 
-def gradient(params: np.ndarray, features: np.ndarray, classes: np.ndarray):
-    """(gradient (8072,), mean loss) of the softmax cross-entropy over the
-    rows at `params`, from the training kernels."""
+def gradient(params: np.ndarray, features: np.ndarray, classes: np.ndarray, radius: np.ndarray | None = None):
+    """(gradient (8137,), mean loss) of the training loss over the rows at
+    `params` (cross-entropy, plus the radius head's weighted squared error
+    where `radius` has a target), from the training kernels."""
     p = np.ascontiguousarray(params, np.float32)
     f = np.ascontiguousarray(features, np.float32)
     c = np.ascontiguousarray(classes, np.uint8)
+    r = None if radius is None else np.ascontiguousarray(radius, np.float32)
     g = np.zeros(PARAMS, np.float32)
     loss = np.zeros(1, np.float32)
     ctx = _ctx()
     try:
-        _lib.check(ctx, _lib.lib().apo_proposal_gradient(ctx, _lib.fptr(p), _lib.fptr(f), _lib.u8ptr(c), len(c),
+        _lib.check(ctx, _lib.lib().apo_proposal_gradient(ctx, _lib.fptr(p), _lib.fptr(f), _lib.u8ptr(c),
+                                                          None if r is None else _lib.fptr(r), len(c),
                                                           _lib.fptr(g), _lib.fptr(loss)))
     finally:
         _lib.lib().apo_destroy(ctx)
@@ -93,11 +142,12 @@ def gradient(params: np.ndarray, features: np.ndarray, classes: np.ndarray):
 
 
 def infer(params: np.ndarray, features: np.ndarray):
-    """Classes and logits of feature rows through the matrix-core inference."""
+    """Classes and the 9 outputs (8 logits, radius head) of feature rows
+    through the matrix-core inference."""
     p = np.ascontiguousarray(params, np.float32)
     f = np.ascontiguousarray(features, np.float32)
     cls = np.zeros(len(f), np.uint8)
-    logits = np.zeros((len(f), CLASSES), np.float32)
+    logits = np.zeros((len(f), OUTPUTS), np.float32)
     ctx = _ctx()
     try:
         L = _lib.lib()

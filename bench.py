@@ -693,11 +693,20 @@ def main():
         ex64["speedup_vs_plain"] = ex64["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
         ex64["note"] = "as learned_exact_radius with at most 64 sites per list"
         out["learned_exact_radius_64"] = ex64
+        # the radius head of the same MLP (proposal_radius): each site's
+        # predicted search radius through the range-query cell pass
+        tr = dict(ltc, proposal_radius=1)
+        lr_ = run_learned(params, dev, n, tr, args.steps, args.warmup, ckeys, plain_rows=crows)
+        lr_["speedup_vs_plain"] = lr_["cells_per_s"] / out["complete"]["eager"]["cells_per_s"]
+        lr_["note"] = ("search radius from the MLP's radius head (d16 x 2^output), range query of at most "
+                       f"{max(tr.get('proposal_max_k', 64), tr.get('proposal_fallback_k', 0))} sites per list")
+        out["learned_radius"] = lr_
         say(f"completion: learned {lcc['ms_per_step']:.1f} ms (identical rows {lcc['identical_vertex_rows']}), "
             f"exact labels {exc['ms_per_step']:.1f} ms (identical rows {exc['identical_vertex_rows']}), "
             f"exact radii {exr['ms_per_step']:.1f} ms (identical rows {exr['identical_vertex_rows']}, "
             f"{exr['cells_certified_by_cell_pass']} certified by the cell pass), with 64-site lists "
-            f"{ex64['ms_per_step']:.1f} ms ({ex64['cells_certified_by_cell_pass']} certified)")
+            f"{ex64['ms_per_step']:.1f} ms ({ex64['cells_certified_by_cell_pass']} certified); radius head "
+            f"{lr_['ms_per_step']:.1f} ms (identical rows {lr_['identical_vertex_rows']})")
         if not args.no_c5:
             say("C5: the proposal on a 5M-site set")
             out["learned_5m"] = c5_evaluation(params, plain_tuning, ltune, say)

@@ -74,9 +74,14 @@ typedef struct apo_config {
                                  stage changed go through vertex and edge validation again (the reference
                                  leaves them unvalidated, :1551-1594); (c) up to `complete` consistency rounds:
                                  a cell holding fewer records of a quadruple (i, j, k, l) than another of its
-                                 four cells is updated with the smallest quadruple site it does not know, and
+                                 four cells is updated with the smallest quadruple site it does not know (a
+                                 cell that knows them all is rebuilt, its stuck records' sites first), and
                                  validated again.  The output then no longer depends on k or the learned
                                  proposal where the reference's does */
+    uint32_t proposal_radius; /* learned proposal: 0 (default) each cell runs its class's neighbour count;
+                                 1 each cell takes the sites below the MLP's search radius (the radius head)
+                                 through a range query, as with apo_proposal_set_radii.  No counterpart in
+                                 the reference */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,
@@ -277,17 +282,19 @@ apo_status apo_get_topology_sets(apo_ctx* ctx, uint32_t* bisectors, uint64_t bis
 apo_status apo_get_cell_topology(apo_ctx* ctx, uint64_t* vertex_offsets, uint32_t* vertices, uint64_t vertex_cap,
                                  uint64_t* edge_offsets, uint32_t* edges, uint64_t edge_cap);
 
-/* ---- learned neighbour-count proposal.  No counterpart in the reference's
- * construction: an MLP (52 -> 64 -> 64 -> 8, BF16 weights, FP32
- * accumulation on the matrix cores) predicts per site how many nearest
- * neighbours the cell pass runs before the plain certification test
- * (algorithm.inl:197-201); cells the test does not certify go through the
- * plain validation.  Classes ask for 20, 24, 28, 32, 40, 48, 64
- * neighbours; class 7 (not certified within 64) runs
- * config.proposal_fallback_k (16 by default). ---- */
+/* ---- learned proposal.  No counterpart in the reference's construction:
+ * an MLP (52 -> 64 -> 64 -> 9, BF16 weights, FP32 accumulation on the
+ * matrix cores) predicts per site how many nearest neighbours the cell pass
+ * runs before the plain certification test (algorithm.inl:197-201), as 8
+ * class logits, and the search radius its cell needs (output 8: log2 of the
+ * radius over d16, the centre distance of the 16th nearest neighbour);
+ * cells the test does not certify go through the plain validation.
+ * Classes ask for 20, 24, 28, 32, 40, 48, 64 neighbours; class 7 (not
+ * certified within 64) runs config.proposal_fallback_k (16 by default). ---- */
 #define APO_PROPOSAL_FEATURES 52
 #define APO_PROPOSAL_CLASSES 8
-#define APO_PROPOSAL_PARAMS 8072 /* W1[52][64] b1[64] W2[64][64] b2[64] W3[64][8] b3[8], weights [in][out] */
+#define APO_PROPOSAL_OUTPUTS 9 /* the 8 class logits, then the radius head */
+#define APO_PROPOSAL_PARAMS 8137 /* W1[52][64] b1[64] W2[64][64] b2[64] W3[64][9] b3[9], weights [in][out] */
 
 /* Label pass (app/benchmarks/benchmark_contributing_nearest.cu:75-113): for
  * every site, the number of neighbours in kNN order after which the cell is
@@ -297,23 +304,32 @@ apo_status apo_proposal_labels(apo_ctx* ctx, const float* xyzr, uint32_t n, floa
 /* Train the MLP on (features, labels) from apo_proposal_labels with HIP
  * kernels on the matrix cores (the inference pass forward, BF16 weights and
  * activations with FP32 accumulation; an FP32 backward pass; Adam on FP32
- * master weights; softmax cross-entropy over the 8 classes).  init: starting parameters or NULL (seeded uniform);
- * params receives APO_PROPOSAL_PARAMS floats, epoch_loss (may be NULL) the
- * mean loss of each epoch. */
-apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t* labels, uint64_t count,
-                              uint32_t epochs, uint32_t batch, float lr, uint32_t seed, const float* init,
-                              float* params, float* epoch_loss);
-/* Gradient of the mean softmax cross-entropy over `count` feature rows with
- * classes 0..7, at parameters `params`, computed by the training kernels
- * (BF16 forward pass as in inference, FP32 backward pass on the matrix
- * cores); grad receives APO_PROPOSAL_PARAMS floats, loss (may be NULL) the
- * mean loss.  For checking the training kernels. */
+ * master weights).  The loss: softmax cross-entropy over the 8 classes, plus
+ * 0.25 x the squared error of the radius head against radius_targets (per
+ * site, log2 of the wanted radius over d16; NaN or a NULL array: none).
+ * init: starting parameters or NULL (seeded uniform); params receives
+ * APO_PROPOSAL_PARAMS floats, epoch_loss (may be NULL) the mean loss of
+ * each epoch. */
+apo_status apo_proposal_train(apo_ctx* ctx, const float* features, const uint8_t* labels,
+                              const float* radius_targets, uint64_t count, uint32_t epochs, uint32_t batch, float lr,
+                              uint32_t seed, const float* init, float* params, float* epoch_loss);
+/* Gradient of the training loss (above) averaged over `count` feature rows
+ * with classes 0..7 and radius targets (may be NULL), at parameters
+ * `params`, computed by the training kernels (BF16 forward pass as in
+ * inference, FP32 backward pass on the matrix cores); grad receives
+ * APO_PROPOSAL_PARAMS floats, loss (may be NULL) the mean loss.  For
+ * checking the training kernels. */
 apo_status apo_proposal_gradient(apo_ctx* ctx, const float* params, const float* features, const uint8_t* classes,
-                                 uint32_t count, float* grad, float* loss);
+                                 const float* radius_targets, uint32_t count, float* grad, float* loss);
 /* Use these parameters in the following builds (NULL: the plain path). */
 apo_status apo_proposal_set(apo_ctx* ctx, const float* params);
-/* Classes (and optionally the 8 logits) the set MLP gives to count feature rows. */
+/* Classes (and optionally the APO_PROPOSAL_OUTPUTS outputs: 8 logits and
+ * the radius head) the set MLP gives to count feature rows. */
 apo_status apo_proposal_infer(apo_ctx* ctx, const float* features, uint32_t count, uint8_t* classes, float* logits);
+/* Search radii of the last build's sites (input order), when it ran the
+ * range-query cell pass: the radius head's (config.proposal_radius) or the
+ * ones given with apo_proposal_set_radii.  No counterpart in the reference. */
+apo_status apo_proposal_radii(apo_ctx* ctx, float* radii);
 /* Classes of the last build's sites, in input order. */
 apo_status apo_proposal_classes(apo_ctx* ctx, uint8_t* classes);
 /* Per-site classes (input order, values 0..7) for the following builds of n

@@ -102,8 +102,41 @@ __device__ __forceinline__ double sphereDistD(D4 a, D4 b) {
     const D3 d{a.x - b.x, a.y - b.y, a.z - b.z};
     return len(d) - (a.w + b.w);
 }
-__device__ __forceinline__ bool intersectF(F4 a, F4 b) { return ltF(sphereDistF(a, b), 0.f); }
-__device__ __forceinline__ bool intersectD(D4 a, D4 b) { return ltD(sphereDistD(a, b), 0.0); }
+// intersect (math.cuh:380-388): the rounded value sqrt(d2) - s, with d2 the
+// squared centre distance and s = ra + rb, below the tolerance T (1e-8).
+// Away from the boundary d2 = (s + T)^2 the square root decides nothing:
+// with s >= 0, d2 below (s + T)^2 (1 - g) puts the rooted value more than
+// g/4 (s + T) under T, more than its two roundings (square root, then
+// subtraction: one unit each, relative to at most s + T) can move it, and
+// d2 above (s + T)^2 (1 + g) puts the root above s + T, so the rounded
+// difference is at least T.  g = 2^-40 in FP64 and 2^-12 in FP32 dwarf those
+// units (2^-53, 2^-24) and the rounding of the threshold itself.  Only a
+// distance inside the band (or s < 0, or an operand out of range) takes the
+// rooted form, so every decision is the rooted form's bit for bit.
+__device__ __forceinline__ bool intersectF(F4 a, F4 b) {
+    const F3 d = subf(xyzf(a), xyzf(b));
+    const float d2 = dotf(d, d);
+    const float s = a.w + b.w;
+    const float t = s + 1e-8f;
+    const float t2 = t * t;
+    if (s >= 0.f && t2 < 1e30f) {
+        if (d2 < t2 * (1.f - 0x1p-12f)) return true;
+        if (d2 > t2 * (1.f + 0x1p-12f)) return false;
+    }
+    return ltF(__builtin_sqrtf(d2) - s, 0.f);
+}
+__device__ __forceinline__ bool intersectD(D4 a, D4 b) {
+    const D3 d{a.x - b.x, a.y - b.y, a.z - b.z};
+    const double d2 = dot(d, d);
+    const double s = a.w + b.w;
+    const double t = s + 1e-8;
+    const double t2 = t * t;
+    if (s >= 0.0 && t2 < 1e300) {
+        if (d2 < t2 * (1.0 - 0x1p-40)) return true;
+        if (d2 > t2 * (1.0 + 0x1p-40)) return false;
+    }
+    return ltD(__builtin_sqrt(d2) - s, 0.0);
+}
 
 // ---------------------------------------------------------------------------
 // boxes (apo/include/apo/gpu/math.inl:79-101), FP32
@@ -135,6 +168,18 @@ __device__ __forceinline__ F3 closestPoint(const Box& b, F4 s) {
 }
 __device__ __forceinline__ float boxDistance(const Box& b, F4 s) {
     return lenf(subf(xyzf(s), closestPoint(b, s))) - s.w;
+}
+// Can box b hold a site of radius up to rmax that reaches into the ball?
+// The edge cover's subtree test, boxDistance(b, ball) <= rmax, without the
+// square root: the squared distance against (ball radius + rmax)^2 with a
+// relative slack of 2^-16.  The rooted form's FP32 roundings (square root,
+// subtraction) move its decision by less than 2^-22 of the squared
+// distance, so every box it accepts is accepted here: a conservative
+// superset, which is all a pruning test must be.
+__device__ __forceinline__ bool boxNearBall(const Box& b, F4 ball, float rmax) {
+    const F3 d = subf(xyzf(ball), closestPoint(b, ball));
+    const float s = ball.w + rmax;
+    return s >= 0.f && dotf(d, d) <= s * s * (1.f + 0x1p-16f);
 }
 __device__ __forceinline__ float boxFarthestDistance(const Box& b, F4 s) {
     const F3 centroid{(b.mn.x + b.mx.x) * .5f, (b.mn.y + b.mx.y) * .5f, (b.mn.z + b.mx.z) * .5f};

@@ -594,8 +594,8 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, const VStack& stN
 #pragma unroll
                 for (int m = 0; m < kCoverBalls; m++) {
                     if (m < q.cover) {
-                        tf = tf || boxDistance(fb, q.ball[m]) <= fr;
-                        tc = tc || boxDistance(cb, q.ball[m]) <= cr;
+                        tf = tf || boxNearBall(fb, q.ball[m], fr);
+                        tc = tc || boxNearBall(cb, q.ball[m], cr);
                     }
                 }
                 ef = ef && tf;

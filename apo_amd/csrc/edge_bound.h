@@ -124,23 +124,32 @@ __device__ __forceinline__ double clearLimit(const D4& x, double margin, double 
     return -(margin + 1e-9 + 1e-7 * (__builtin_fabs(x.w) + tolBase));
 }
 
-// Some neighbour other than k, l has a gap below lim to the sphere at x (x,
-// lim uniform; the wave tests the neighbours in parallel).  With F <= 64
-// each lane's neighbour site is preloaded (site, id).
-__device__ __forceinline__ bool waveClearlyHit(const CellCtx& c, const CellLds& L, uint32_t F, const D4& x,
-                                               double lim, uint32_t k, uint32_t l, const D4& site, uint32_t id) {
-    const uint32_t lane = laneId();
-    if (F <= 64u) return ballot(lane < F && id != k && id != l && sphereDistD(site, x) < lim) != 0;
-    for (uint32_t b = 0; b < F; b += 64) {
-        const uint32_t q = b + lane;
-        bool h = false;
-        if (q < F) {
-            const uint32_t s = L.nb[q];
-            h = s != k && s != l && sphereDistD(nbSite(c, L, q, s), x) < lim;
+// Some neighbour other than k, l has a gap below lim to the sphere at x, a
+// lane's own point and limit: the midpoint test of one interval, one
+// interval per lane.  Each lane walks the neighbour set.  With F <= 64
+// neighbour q's site is lane q's preloaded `site`, read across the wave (a
+// uniform value: one scalar copy per neighbour for all lanes); beyond that
+// the sites are loaded.  The walk ends when every lane has found a hit.
+__device__ __forceinline__ bool laneClearlyHit(const CellCtx& c, const CellLds& L, uint32_t F, const D4& x,
+                                               double lim, uint32_t k, uint32_t l, const D4& site) {
+    bool hit = false;
+    if (F <= 64u) {
+        for (uint32_t q = 0; q < F; q++) {
+            const uint32_t s = __builtin_amdgcn_readfirstlane(L.nb[q]);
+            if (s == k || s == l) continue;
+            const D4 sq = readLaneD4(site, q);
+            hit = hit || sphereDistD(sq, x) < lim;
+            if (ballot(!hit) == 0) break;
         }
-        if (ballot(h) != 0) return true;
+        return hit;
     }
-    return false;
+    for (uint32_t q = 0; q < F; q++) {
+        const uint32_t s = __builtin_amdgcn_readfirstlane(L.nb[q]);
+        if (s == k || s == l) continue;
+        hit = hit || sphereDistD(nbSite(c, L, q, s), x) < lim;
+        if (ballot(!hit) == 0) break;
+    }
+    return hit;
 }
 
 // Wave-cooperative cover of edge (k,l) of the cell in L; si, k, l are
@@ -249,10 +258,8 @@ __device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L,
     uint8_t* keep = reinterpret_cast<uint8_t*>(B.roots);
     const double qn = len(t.Q);
     D4 nbS{0, 0, 0, 0};
-    uint32_t nbId = kNone;
     if (F <= 64u && lane < F) {
-        nbId = L.nb[lane];
-        nbS = nbSite(c, L, lane, nbId);
+        nbS = nbSite(c, L, lane, L.nb[lane]);
     }
     uint32_t nStart = 0, nEnd = 0;
     double wRun = -DBL_MAX;
@@ -299,12 +306,9 @@ __device__ __forceinline__ int waveEdgeCover(const CellCtx& c, const CellLds& L,
                 }
             }
             bool kp = false;
-            uint64_t am = ballot(act);
-            while (am) {
-                const uint32_t src = uint32_t(__builtin_ctzll(am));
-                am &= am - 1;
-                const bool hit = waveClearlyHit(c, L, F, readLaneD4(x, src), readLaneD(lim, src), k, l, nbS, nbId);
-                if (lane == src) {
+            if (ballot(act) != 0) {
+                const bool hit = act && laneClearlyHit(c, L, F, x, lim, k, l, nbS);
+                if (act) {
                     if (unbounded) {
                         if (!hit) fail = true;
                     } else {

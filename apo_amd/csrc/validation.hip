@@ -271,7 +271,8 @@ constexpr uint32_t kShareIdle = 4;
 // and gaps) when the vertex centre lies outside the box, and unbounded
 // below when it lies inside (a centre inside a site gives a negative gap),
 // so a box within `slack` of the centre is never pruned.  Leaves improve
-// the bound strictly, so the first of equal gaps in this order is kept.
+// the bound strictly, so the first of equal gaps in this order is kept.  A
+// leaf reached by a step's descent or pop is tested in the same step.
 __device__ __forceinline__ void vertexNearStep(const CellCtx& c, const CellLds& L, const VStack& stNode,
                                                uint32_t F, uint32_t i, uint4 vx, F4 cur, float slack, Trav& t,
                                                float& nd, uint32_t& nb, uint32_t& flags) {
@@ -306,13 +307,14 @@ __device__ __forceinline__ void vertexNearStep(const CellCtx& c, const CellLds& 
                     flags |= kFlagStackOverflow;
                 }
             }
-            return;
+        } else {
+            popNode(stNode, t);
         }
-        popNode(stNode, t);
-        return;
+        if (!t.live || t.node < c.internal) return;
     }
-    if (node < nodeCount(c)) {
-        const uint32_t j = node - c.internal;
+    const uint32_t leaf = t.node;
+    if (leaf < nodeCount(c)) {
+        const uint32_t j = leaf - c.internal;
         if (j != i && j != vx.x && j != vx.y && j != vx.z && c.status[j] != kStatusBuried) {
             const float d = sphereDistF(c.sorted[j], cur);
             if (d < nd && !knownLane(L, F, j)) {
@@ -543,8 +545,9 @@ struct EdgeQuery {
 
 
 // One step of a lane's contributor traversal (algorithm.inl:616-793): visit
-// one internal node (descend to the closer accepted child, stack the other)
-// or test one leaf.  Returns the leaf's site when it is a candidate: other
+// one internal node (descend to the closer accepted child, stack the other,
+// or take the stack's top), then, when the node reached is a leaf, test it
+// in the same step.  Returns the leaf's site when it is a candidate: other
 // than i, k, l, not buried, touching the search sphere and, for a
 // hyperbolic edge, on the right side of both tangent planes.  A lane walks
 // its traversal in the reference's order; the wave never waits for lanes
@@ -614,11 +617,12 @@ __device__ __forceinline__ uint32_t edgeStep(const CellCtx& c, const VStack& stN
         } else {
             popNode(stNode, t);
         }
-        return kNone;
+        if (!t.live || t.node < c.internal) return kNone;
     }
+    const uint32_t leaf = t.node;
     uint32_t cand = kNone;
-    if (node < nodeCount(c)) {
-        const uint32_t j = node - c.internal;
+    if (leaf < nodeCount(c)) {
+        const uint32_t j = leaf - c.internal;
         bool cc = j != i && j != k && j != l && c.status[j] != kStatusBuried;
         if (cc) {
             const F4 fsj = c.sorted[j];
@@ -908,7 +912,8 @@ __device__ __forceinline__ void edgeTask(const CellCtx& c, EdgeLds& S, uint32_t 
 // ---------------------------------------------------------------------------
 // Contributor search along a bisector (algorithm.inl:859-1233)
 // ---------------------------------------------------------------------------
-// One step of a lane's bisector traversal (algorithm.inl:1019-1176).
+// One step of a lane's bisector traversal (algorithm.inl:1019-1176); a leaf
+// reached by the step's descent or pop is tested in the same step.
 // Returns the leaf's site when it is a candidate: other than i, bk, inside
 // the search sphere and the cylinder around the bisector axis.
 // With axisPrune (no negative radius in the scene) a subtree whose box
@@ -957,11 +962,12 @@ __device__ __forceinline__ uint32_t bisectorStep(const CellCtx& c, const VStack&
         } else {
             popNode(stNode, t);
         }
-        return kNone;
+        if (!t.live || t.node < c.internal) return kNone;
     }
+    const uint32_t leaf = t.node;
     uint32_t cand = kNone;
-    if (node < nodeCount(c)) {
-        const uint32_t j = node - c.internal;
+    if (leaf < nodeCount(c)) {
+        const uint32_t j = leaf - c.internal;
         bool cc = j != i && j != bk;
         if (cc) {
             const F4 fsj = c.sorted[j];

@@ -7,7 +7,9 @@ time to produce it does:
   the persistent work-queue grids (the wave "tiles" each CU runs at once),
   from 1 to each kernel's full occupancy;
 * cover_ratio -- the known-radius multiple above which an edge search is
-  pruned by the cover of the edge's empty spheres (exact pruning).
+  pruned by the cover of the edge's empty spheres (exact pruning);
+* cells_per_wave -- the cell pass's tile: a whole wave64 per cell, or half
+  a wave per cell (two cells per wave, twice the LDS per wave).
 
 The learned path adds the neighbours per pass of its fallback class
 (proposal_fallback_k: the neighbour count of sites predicted not to
@@ -42,6 +44,7 @@ TUNED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned")
 
 # knob -> stats field it is scored by (None: the whole build)
 PLAIN_KNOBS = {
+    "cells_per_wave": "ms_cells",
     "waves_cells": "ms_cells",
     "waves_vertex": "ms_vertex_validation_kernel",
     "waves_edge": "ms_edge_validation_kernel",
@@ -193,8 +196,9 @@ def tune(sites, workload: str, offsets=None, proposal=None, repeats: int = 2,
     occ = run.occupancy()
     candidates = {"waves_cells": _wave_candidates(occ[0]), "waves_vertex": _wave_candidates(occ[1]),
                   "waves_edge": _wave_candidates(occ[2]), "waves_bisector": _wave_candidates(occ[3]),
-                  "cover_ratio": COVER_RATIOS}
-    start = {"waves_cells": 0, "waves_vertex": 0, "waves_edge": 0, "waves_bisector": 0, "cover_ratio": 2.0}
+                  "cover_ratio": COVER_RATIOS, "cells_per_wave": (1, 2)}
+    start = {"cells_per_wave": 1, "waves_cells": 0, "waves_vertex": 0, "waves_edge": 0, "waves_bisector": 0,
+             "cover_ratio": 2.0}
 
     def plain_score(cfg, knob):
         st = run.stats(cfg)

@@ -97,7 +97,7 @@ struct apo_ctx {
     uint32_t* offsets = nullptr;
     uint64_t* moleculeVertices = nullptr;
     // blocks per CU of the persistent kernels at full occupancy
-    uint32_t occCells = 1, occVertex = 1, occEdge = 1, occBisector = 1;
+    uint32_t occCells = 1, occCellsPair = 1, occVertex = 1, occEdge = 1, occBisector = 1;
     // the captured build (config.graph) and what it was captured for
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -327,6 +327,18 @@ uint32_t persistentBlocks(apo_ctx* ctx, uint32_t own, uint32_t occupancy) {
     return uint32_t(ctx->cus) * perCu;
 }
 
+// The cell pass runs two cells per wave when the config asks for it.
+bool cellPairs(const apo_ctx* ctx) { return ctx->cfg.cells_per_wave == 2u; }
+
+// Grid of the cell pass: the configured waves per CU, at most what fits a CU
+// when a wave holds two cells (their records take twice the LDS).
+uint32_t cellPassBlocks(apo_ctx* ctx) {
+    if (!cellPairs(ctx)) return persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells);
+    const uint32_t own = ctx->cfg.waves_cells ? ctx->cfg.waves_cells : ctx->cfg.waves_per_cu;
+    const uint32_t perCu = own && own < ctx->occCellsPair ? own : ctx->occCellsPair;
+    return uint32_t(ctx->cus) * perCu;
+}
+
 std::string guardText(uint32_t bits) {
     static const char* names[] = {"sort tile", "sort scatter", "sort look-back spin", "gather index",
                                   "refit parent", "refit child", "knn child", "cell site id",
@@ -416,8 +428,8 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
         // range when the list holds fewer sites than its capacity
         APO_HIP(mark(7));
         APO_HIP(cellsLaunch(cc, ctx->knn, Kk, ctx->counters + kCellQueue,
-                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st, nullptr,
-                            ctx->cfg.proposal_fallback_k, false, ctx->radiiSorted));
+                            cellPassBlocks(ctx), st, nullptr,
+                            ctx->cfg.proposal_fallback_k, false, ctx->radiiSorted, cellPairs(ctx)));
     } else if (learned && ctx->cfg.proposal_radius && !ctx->givenN && (ctx->cfg.stage_mask & 1u)) {
         // the radius head: features of the 16 nearest, each site's search
         // radius, then the range-query lists below it and the ranged cell pass
@@ -428,8 +440,8 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
         APO_HIP(knnLaunch(cc, Kk, ctx->knn, st, 0u, ctx->radiiSorted));
         APO_HIP(mark(7));
         APO_HIP(cellsLaunch(cc, ctx->knn, Kk, ctx->counters + kCellQueue,
-                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st, nullptr,
-                            ctx->cfg.proposal_fallback_k, false, ctx->radiiSorted));
+                            cellPassBlocks(ctx), st, nullptr,
+                            ctx->cfg.proposal_fallback_k, false, ctx->radiiSorted, cellPairs(ctx)));
     } else if (learned && (ctx->cfg.stage_mask & 1u)) {
         // features of the 16 nearest, proposal classes, then each cell with its
         // proposed neighbour count and the plain certification test
@@ -442,13 +454,14 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
         }
         APO_HIP(mark(7));
         APO_HIP(cellsLaunch(cc, ctx->knn, Kk, ctx->counters + kCellQueue,
-                            persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), st, ctx->cls,
-                            ctx->cfg.proposal_fallback_k));
+                            cellPassBlocks(ctx), st, ctx->cls,
+                            ctx->cfg.proposal_fallback_k, false, nullptr, cellPairs(ctx)));
     } else {
         APO_HIP(mark(7));
-        const uint32_t cellBlocks = persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells);
+        const uint32_t cellBlocks = cellPassBlocks(ctx);
         APO_HIP(cellsLaunch(cc, ctx->knn, (ctx->cfg.stage_mask & 1u) && passes > 0u ? K : 0u,
-                            ctx->counters + kCellQueue, cellBlocks, st));
+                            ctx->counters + kCellQueue, cellBlocks, st, nullptr, 16u, false, nullptr,
+                            cellPairs(ctx)));
         // knnValidationStep > 1 (algorithm.inl:1535-1549): the next K neighbours
         // of every site, then the cells still in validation continue with them
         for (uint32_t s = 1; s < passes; s++) {
@@ -457,7 +470,8 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
             APO_HIP(knnLaunch(cc, K, ctx->knn, st, s));
             if (tp) APO_HIP(hipEventRecord(ctx->passEv[s - 1][1], st));
             if (ctx->cfg.stage_mask & 1u)
-                APO_HIP(cellsLaunch(cc, ctx->knn, K, ctx->counters + kCellQueue, cellBlocks, st, nullptr, 16u, true));
+                APO_HIP(cellsLaunch(cc, ctx->knn, K, ctx->counters + kCellQueue, cellBlocks, st, nullptr, 16u, true,
+                                    nullptr, cellPairs(ctx)));
         }
     }
     APO_TIMED_STAGE("cells");
@@ -490,7 +504,7 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
         hipLaunchKernelGGL(k_fill_words, dim3((n + 255u) / 256u), dim3(256), 0, st, ctx->want, n, kNone);
         APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st, nullptr,
                                  ctx->counters + kCompletionTasks));
-        const uint32_t blocks = persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells);
+        const uint32_t blocks = cellPassBlocks(ctx);
         for (uint32_t r = 0; r < ctx->cfg.complete; r++) {
             APO_HIP(consistencyLaunch(cc, ctx->want, ctx->stuck, ctx->counters, r, blocks, st));
             APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st, nullptr,
@@ -735,6 +749,7 @@ void apo_default_config(apo_config* cfg) {
     cfg->knn_passes = 1;
     cfg->complete = 0;
     cfg->proposal_radius = 0;
+    cfg->cells_per_wave = 0;
 }
 
 // pcg hash of apo/src/core/utils.cpp:6-12, scaled by 2^-32
@@ -772,7 +787,7 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     const uint32_t k = ctx->cfg.k, fk = ctx->cfg.proposal_fallback_k, mk = ctx->cfg.proposal_max_k;
     auto listK = [](uint32_t v) { return v == 16 || v == 24 || v == 32 || v == 48 || v == 64; };
     if (k == 0 || k > 64 || ctx->cfg.max_iterations > kMaxRounds || !listK(fk) || !listK(mk) ||
-        ctx->cfg.complete > kMaxCompletionRounds) {
+        ctx->cfg.complete > kMaxCompletionRounds || ctx->cfg.cells_per_wave > 2) {
         delete ctx;
         return APO_ERR_INVALID;
     }
@@ -781,7 +796,7 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     if (e == hipSuccess) e = staticScratchAddress(ctx->device, &ctx->staticScratch);
-    if (e == hipSuccess) e = cellsOccupancy(&ctx->occCells);
+    if (e == hipSuccess) e = cellsOccupancy(&ctx->occCells, &ctx->occCellsPair);
     if (e == hipSuccess) e = validationOccupancy(&ctx->occVertex, &ctx->occEdge, &ctx->occBisector);
     for (int s = 0; s < 10 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
     for (uint32_t r = 0; r < kTimedRounds && e == hipSuccess; r++)
@@ -1831,7 +1846,7 @@ apo_status apo_cell_radii(apo_ctx* ctx, float* radii) {
     const uint32_t n = ctx->n;
     DeviceTemp<float> d;
     APO_HIP(d.alloc(n));
-    APO_HIP(cellRadiiLaunch(cellCtx(ctx), d.p, persistentBlocks(ctx, ctx->cfg.waves_cells, ctx->occCells), ctx->stream));
+    APO_HIP(cellRadiiLaunch(cellCtx(ctx), d.p, cellPassBlocks(ctx), ctx->stream));
     std::vector<float> sorted(n);
     std::vector<uint32_t> idx(n);
     APO_HIP(hipMemcpyAsync(sorted.data(), d.p, size_t(n) * 4, hipMemcpyDeviceToHost, ctx->stream));

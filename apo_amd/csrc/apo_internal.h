@@ -174,7 +174,8 @@ hipError_t knnLaunch(const CellCtx& c, uint32_t k, uint32_t* knn, hipStream_t st
                      const float* bound = nullptr);
 // No kNN pass: empty lists (n * k words), status in validation, flags clear.
 hipError_t cellsResetLaunch(const CellCtx& c, uint32_t* knn, uint32_t k, hipStream_t st);
-hipError_t cellsOccupancy(uint32_t* blocksPerCu);
+// Blocks per CU of the cell pass with one cell per wave and with two.
+hipError_t cellsOccupancy(uint32_t* blocksPerCu, uint32_t* pairBlocksPerCu);
 // cellRadius of every stored cell (sorted order), rounded up to FP32; -FLT_MAX when buried.
 hipError_t cellRadiiLaunch(const CellCtx& c, float* out, uint32_t blocks, hipStream_t st);
 // Persistent cell pass over all sites; queue = kXcdQueueWords heads.
@@ -185,9 +186,10 @@ hipError_t cellRadiiLaunch(const CellCtx& c, float* out, uint32_t blocks, hipStr
 // bound (sorted order, may be null): the lists are range queries below these
 // gaps (knnLaunch with bound); a cell whose list is not full knows every site
 // below its bound, which then serves as its known radius.
+// pair: two cells per wave (half a wave each), the same records.
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
                        hipStream_t st, const uint8_t* cls = nullptr, uint32_t fallbackK = 16u,
-                       bool resume = false, const float* bound = nullptr);
+                       bool resume = false, const float* bound = nullptr, bool pair = false);
 
 // Device counters of a build (words of one buffer): work-queue heads of the
 // persistent kernels (8 per queue, one range per XCD), the task count of

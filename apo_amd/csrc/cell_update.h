@@ -1,4 +1,4 @@
-// cell_update.h -- wave64 cell update shared by the cell and validation kernels.
+// cell_update.h -- cell update shared by the cell and validation kernels (a wave64, or a half wave, per cell).
 //
 // Reference: apo/include/apo/gpu/topology_update.inl:5-653 (updateCell,
 // cellRadius, the initial octahedron).  The cell's vertex and edge records
@@ -28,6 +28,45 @@ struct WaveCell {
     uint32_t V, E, F, flags;
 };
 
+// The lanes that work on one cell: the whole wave (W = 64) or one half of
+// it (W = 32, lanes 32..63 the second half, each half with a cell of its
+// own).  Every step of a cell update is written against these: a group's
+// ballot holds its own lanes only (bit 0 = its first lane), `below` counts
+// the group's set bits under the calling lane, `read` takes a value from a
+// lane of the group and `uni` a value that every lane of the group holds
+// (a scalar copy for the whole wave; the value itself for a half).
+template <uint32_t W>
+struct Lanes;
+template <>
+struct Lanes<64> {
+    __device__ static __forceinline__ uint32_t lane() { return laneId(); }
+    __device__ static __forceinline__ uint64_t vote(bool p) { return ballot(p); }
+    __device__ static __forceinline__ uint32_t below(uint64_t m) { return countBelow(m); }
+    __device__ static __forceinline__ uint32_t read(uint32_t v, uint32_t t) { return readLane(v, t); }
+    __device__ static __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+    __device__ static __forceinline__ double maxD(double v) { return waveMaxD(v); }
+};
+template <>
+struct Lanes<32> {
+    __device__ static __forceinline__ uint32_t half() { return laneId() & 32u; }
+    __device__ static __forceinline__ uint32_t lane() { return laneId() & 31u; }
+    __device__ static __forceinline__ uint64_t vote(bool p) {
+        const uint64_t m = ballot(p);
+        return half() ? (m >> 32) : (m & 0xffffffffull);
+    }
+    __device__ static __forceinline__ uint32_t below(uint64_t m) {
+        return uint32_t(__builtin_popcount(uint32_t(m) & ((1u << lane()) - 1u)));
+    }
+    __device__ static __forceinline__ uint32_t read(uint32_t v, uint32_t t) {
+        return uint32_t(__shfl(int(v), int(half() | t), 64));
+    }
+    __device__ static __forceinline__ uint32_t uni(uint32_t v) { return v; }
+    __device__ static __forceinline__ double maxD(double v) {
+        for (int o = 16; o >= 1; o >>= 1) v = __builtin_fmax(v, __shfl_xor(v, o, 64));
+        return v;
+    }
+};
+
 __device__ __forceinline__ D4 loadSite(const CellCtx& c, uint32_t id) {
     if (id < c.n) return toD(c.sorted[id]);
     const uint32_t a = id - c.n;
@@ -43,10 +82,12 @@ __device__ __forceinline__ D4 vertexPosition(const CellCtx& c, const D4& si, uin
 }
 
 // Distinct sites over the edges [0, E), in first-occurrence order.
+template <uint32_t W = 64>
 __device__ inline void buildNeighbours(const CellCtx& c, CellLds& L, WaveCell& w) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     uint32_t F = 0;
-    for (uint32_t b = 0; b < w.E; b += 64) {
+    for (uint32_t b = 0; b < w.E; b += W) {
         const uint32_t e = b + lane;
         const bool has = e < w.E;
         uint32_t x0 = kNone, x1 = kNone;
@@ -56,7 +97,7 @@ __device__ inline void buildNeighbours(const CellCtx& c, CellLds& L, WaveCell& w
             x1 = ed.y;
         }
         bool f0 = has, f1 = has && x1 != x0;
-        const uint32_t lim = (w.E - b) < 64u ? w.E : b + 64u;
+        const uint32_t lim = (w.E - b) < W ? w.E : b + W;
         // earlier edges, eight per step: independent LDS reads in flight
         uint32_t e2 = 0;
         for (; e2 + 8u <= lim; e2 += 8u) {
@@ -77,8 +118,8 @@ __device__ inline void buildNeighbours(const CellCtx& c, CellLds& L, WaveCell& w
                 if (y.x == x1 || y.y == x1) f1 = false;
             }
         }
-        const uint64_t m0 = ballot(f0), m1 = ballot(f1);
-        const uint32_t off = F + countBelow(m0) + countBelow(m1);
+        const uint64_t m0 = G::vote(f0), m1 = G::vote(f1);
+        const uint32_t off = F + G::below(m0) + G::below(m1);
         if (f0) L.nb[off] = x0;
         if (f1) L.nb[off + (f0 ? 1u : 0u)] = x1;
         F += popc64(m0) + popc64(m1);
@@ -98,16 +139,17 @@ __device__ __forceinline__ D4 nbSite(const CellCtx& c, const CellLds& L, uint32_
 // (topology_update.inl:578-592).  Per lane, serial over the neighbour set,
 // U neighbours per step: their site loads are in flight together (callers
 // with many live registers take fewer).
-template <uint32_t U = 4>
+template <uint32_t U = 4, uint32_t W = 64>
 __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLds& L, uint32_t F, const D4& cand,
                                                   uint32_t skip1, uint32_t skip2) {
+    using G = Lanes<W>;
     uint32_t q = 0;
     if constexpr (U > 1u) {
         for (; q + U <= F; q += U) {
             uint32_t id[U];
             D4 s[U];
 #pragma unroll
-            for (uint32_t u = 0; u < U; u++) id[u] = __builtin_amdgcn_readfirstlane(L.nb[q + u]);
+            for (uint32_t u = 0; u < U; u++) id[u] = G::uni(L.nb[q + u]);
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) s[u] = nbSite(c, L, q + u, id[u]);
             bool hit = false;
@@ -117,7 +159,7 @@ __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLd
         }
     }
     for (; q < F; q++) {
-        const uint32_t id = __builtin_amdgcn_readfirstlane(L.nb[q]);
+        const uint32_t id = G::uni(L.nb[q]);
         if (id == skip1 || id == skip2) continue;
         if (intersectD(nbSite(c, L, q, id), cand)) return false;
     }
@@ -128,10 +170,11 @@ __device__ __forceinline__ bool clearOfNeighbours(const CellCtx& c, const CellLd
 // the bits of `mask` (bit 0: x0, bit 1: x1) whose candidate is clear of
 // every neighbour except skip1/skip2.  Equal to two clearOfNeighbours calls;
 // each neighbour's site is loaded once for both.
-template <uint32_t U = 4>
+template <uint32_t U = 4, uint32_t W = 64>
 __device__ __forceinline__ uint32_t clearPairOfNeighbours(const CellCtx& c, const CellLds& L, uint32_t F,
                                                           const D4& x0, const D4& x1, uint32_t mask,
                                                           uint32_t skip1, uint32_t skip2) {
+    using G = Lanes<W>;
     uint32_t ok = mask;
     uint32_t q = 0;
     if constexpr (U > 1u) {
@@ -139,7 +182,7 @@ __device__ __forceinline__ uint32_t clearPairOfNeighbours(const CellCtx& c, cons
             uint32_t id[U];
             D4 s[U];
 #pragma unroll
-            for (uint32_t u = 0; u < U; u++) id[u] = __builtin_amdgcn_readfirstlane(L.nb[q + u]);
+            for (uint32_t u = 0; u < U; u++) id[u] = G::uni(L.nb[q + u]);
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) s[u] = nbSite(c, L, q + u, id[u]);
             bool h0 = false, h1 = false;
@@ -154,7 +197,7 @@ __device__ __forceinline__ uint32_t clearPairOfNeighbours(const CellCtx& c, cons
         }
     }
     for (; q < F && ok; q++) {
-        const uint32_t id = __builtin_amdgcn_readfirstlane(L.nb[q]);
+        const uint32_t id = G::uni(L.nb[q]);
         if (id == skip1 || id == skip2) continue;
         const D4 s = nbSite(c, L, q, id);
         if ((ok & 1u) && intersectD(s, x0)) ok &= ~1u;
@@ -164,19 +207,20 @@ __device__ __forceinline__ uint32_t clearPairOfNeighbours(const CellCtx& c, cons
 }
 
 // Stable in-place removal of records flagged by `drop`, step by step.
-template <typename Drop>
+template <uint32_t W = 64, typename Drop>
 __device__ __forceinline__ uint32_t compact(uint4* rec, uint32_t count, Drop drop) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     uint32_t removed = 0;
-    for (uint32_t b = 0; b < count; b += 64) {
+    for (uint32_t b = 0; b < count; b += W) {
         const uint32_t idx = b + lane;
         const bool has = idx < count;
         uint4 r = {0, 0, 0, 0};
         if (has) r = rec[idx];
         const bool d = has && drop(idx, r);
-        const uint64_t m = ballot(d);
+        const uint64_t m = G::vote(d);
         waveSync();
-        if (has && !d) rec[idx - removed - countBelow(m)] = r;
+        if (has && !d) rec[idx - removed - G::below(m)] = r;
         removed += popc64(m);
         waveSync();
     }
@@ -191,9 +235,11 @@ __device__ __forceinline__ void sort3(uint32_t& a, uint32_t& b, uint32_t& c) {
 }
 
 // One incremental update of cell i by site j (topology_update.inl:49-555).
+template <uint32_t W = 64>
 __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, const D4& si, uint32_t j,
                            const D4& sj) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     if (w.V == 0 && w.E == 0) return false;
     {
         const double ri = si.w, rj = sj.w;
@@ -207,11 +253,11 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
         }
     }
     const uint32_t baseV = w.V, baseE = w.E;
-    buildNeighbours(c, L, w);
+    buildNeighbours<W>(c, L, w);
 
     // (1) vertices whose empty sphere intersects s_j
     uint32_t nInv = 0;
-    for (uint32_t b = 0; b < baseV; b += 64) {
+    for (uint32_t b = 0; b < baseV; b += W) {
         const uint32_t v = b + lane;
         bool inv = false;
         if (v < baseV) {
@@ -219,13 +265,13 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
             inv = intersectD(vertexPosition(c, si, r), sj);
             if (inv) L.vtx[v].w = (r.w & 0xffu) | (0xffu << 8);
         }
-        const uint64_t m = ballot(inv);
-        if (inv) L.list[nInv + countBelow(m)] = v;
+        const uint64_t m = G::vote(inv);
+        if (inv) L.list[nInv + G::below(m)] = v;
         nInv += popc64(m);
     }
     waveSync();
     if (nInv) {
-        for (uint32_t b = 0; b < baseE; b += 64) {
+        for (uint32_t b = 0; b < baseE; b += W) {
             const uint32_t e = b + lane;
             if (e < baseE) {
                 const uint4 ed = L.edg[e];
@@ -241,12 +287,12 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
             }
         }
         waveSync();
-        w.V = compact(L.vtx, w.V, [&](uint32_t idx, uint4 r) { return ((r.w >> 8) & 0xffu) == 0xffu && idx < baseV; });
+        w.V = compact<W>(L.vtx, w.V, [&](uint32_t idx, uint4 r) { return ((r.w >> 8) & 0xffu) == 0xffu && idx < baseV; });
     }
 
     // (2) candidate vertices on every edge (k, l)
     uint32_t NE = 0, nClosed = 0;
-    for (uint32_t b = 0; b < baseE; b += 64) {
+    for (uint32_t b = 0; b < baseE; b += W) {
         const uint32_t e = b + lane;
         const bool has = e < baseE;
         uint4 ed = {0, 0, 0, 0};
@@ -266,7 +312,7 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
             trisectorAliased(si, sk, sl, t);
             if (intersectD(t, sj)) vnb = 0;
         }
-        const uint32_t valid = clearPairOfNeighbours(c, L, w.F, q0, q1, mask, k, l);
+        const uint32_t valid = clearPairOfNeighbours<4, W>(c, L, w.F, q0, q1, mask, k, l);
         const uint32_t nNew = uint32_t(__builtin_popcount(valid));
         if (has && mask != 0 && nNew > 0) {
             if (vnb == kNone) vnb = 0;
@@ -275,12 +321,12 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
         if (has && vnb != ed.z) L.edg[e].z = vnb;
 
         const bool closedCand = has && mask == 0;
-        const uint64_t mc = ballot(closedCand);
-        if (closedCand) L.list[nClosed + countBelow(mc)] = e;
+        const uint64_t mc = G::vote(closedCand);
+        if (closedCand) L.list[nClosed + G::below(mc)] = e;
         nClosed += popc64(mc);
 
-        const uint64_t b0 = ballot(nNew >= 1), b1 = ballot(nNew >= 2);
-        const uint32_t off = countBelow(b0) + countBelow(b1);
+        const uint64_t b0 = G::vote(nNew >= 1), b1 = G::vote(nNew >= 2);
+        const uint32_t off = G::below(b0) + G::below(b1);
         uint32_t jj = j, kk = k, ll = l;
         sort3(jj, kk, ll);
         uint32_t slot = w.V + off;
@@ -315,7 +361,7 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
     // is skipped, which yields the reference's sequential result.
     if (nClosed) {
         const uint32_t cands = 2 * nClosed;
-        for (uint32_t qb = 0; qb < cands; qb += 64) {
+        for (uint32_t qb = 0; qb < cands; qb += W) {
             const uint32_t q = qb + lane;
             const bool act = q < cands;
             uint32_t id = kNone;
@@ -329,19 +375,19 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
             if (ok) {
                 D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
                 const uint32_t cnt = trisector(si, sj, loadSite(c, id), x0, x1);
-                ok = cnt == 2 && clearOfNeighbours(c, L, w.F, x0, id, i);
+                ok = cnt == 2 && clearOfNeighbours<4, W>(c, L, w.F, x0, id, i);
             }
             // keep the first accepted lane per id
-            uint64_t m = ballot(ok);
+            uint64_t m = G::vote(ok);
             uint64_t mm = m;
             while (mm) {
                 const uint32_t t = uint32_t(__builtin_ctzll(mm));
                 mm &= mm - 1;
-                const uint32_t idt = readLane(id, t);
+                const uint32_t idt = G::read(id, t);
                 if (ok && lane > t && id == idt) ok = false;
             }
-            m = ballot(ok);
-            const uint32_t pos = NE + countBelow(m);
+            m = G::vote(ok);
+            const uint32_t pos = NE + G::below(m);
             if (ok) {
                 if (pos < kNewEdgeCap) L.ne[pos] = id | (1u << 31);
             }
@@ -355,12 +401,12 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
     }
 
     // (4) old edges whose vertex count dropped to zero leave the cell
-    w.E = compact(L.edg, w.E, [&](uint32_t idx, uint4 r) { return r.z == 0u && idx < baseE; });
+    w.E = compact<W>(L.edg, w.E, [&](uint32_t idx, uint4 r) { return r.z == 0u && idx < baseE; });
 
     // (5) one new edge (j, k) per distinct k, ascending, vertexNb = run length
     //     (or kNone for a closed trisector)
     // first-occurrence flags go to bit 30 of the entry
-    for (uint32_t ab = 0; ab < NE; ab += 64) {
+    for (uint32_t ab = 0; ab < NE; ab += W) {
         const uint32_t a = ab + lane;
         bool first = a < NE;
         uint32_t key = 0;
@@ -377,7 +423,7 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
         waveSync();
     }
     uint32_t D = 0;
-    for (uint32_t ab = 0; ab < NE; ab += 64) {
+    for (uint32_t ab = 0; ab < NE; ab += W) {
         const uint32_t a = ab + lane;
         const bool first = a < NE && L.list[a];
         if (first) {
@@ -397,7 +443,7 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
             if (slot < kEdgeCap)
                 L.edg[slot] = make_uint4(j < key ? j : key, j < key ? key : j, closed ? kNone : count, 0u);
         }
-        D += popc64(ballot(first));
+        D += popc64(G::vote(first));
     }
     w.E += D;
     if (w.E > kEdgeCap) {
@@ -409,14 +455,16 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
 }
 
 // cellRadius (topology_update.inl:595-653)
+template <uint32_t W = 64>
 __device__ inline double cellRadius(const CellCtx& c, const CellLds& L, const WaveCell& w, const D4& si) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     double maxR = 0.0;
-    for (uint32_t b = 0; b < w.V; b += 64) {
+    for (uint32_t b = 0; b < w.V; b += W) {
         const uint32_t v = b + lane;
         if (v < w.V) maxR = __builtin_fmax(maxR, vertexPosition(c, si, L.vtx[v]).w);
     }
-    for (uint32_t b = 0; b < w.E; b += 64) {
+    for (uint32_t b = 0; b < w.E; b += W) {
         const uint32_t e = b + lane;
         if (e < w.E) {
             const uint4 ed = L.edg[e];
@@ -425,13 +473,15 @@ __device__ inline double cellRadius(const CellCtx& c, const CellLds& L, const Wa
                 maxR = __builtin_fmax(maxR, __builtin_fmax(x0.w, x1.w));
         }
     }
-    return waveMaxD(maxR);
+    return G::maxD(maxR);
 }
 
 // The initial cell: octahedron of the 6 artificial sites (algorithm.inl:1472-1499)
 // of the cell's molecule, whose ids start at n.
+template <uint32_t W = 64>
 __device__ __forceinline__ void initCell(CellLds& L, WaveCell& w, uint32_t n) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     const uint8_t av[8][3] = {{0, 2, 4}, {0, 2, 5}, {0, 3, 4}, {0, 3, 5}, {1, 2, 4}, {1, 2, 5}, {1, 3, 4}, {1, 3, 5}};
     const uint8_t ae[12][2] = {{0, 2}, {2, 4}, {1, 2}, {2, 5}, {1, 4}, {1, 5},
                                {0, 5}, {0, 4}, {1, 3}, {3, 5}, {0, 3}, {3, 4}};
@@ -445,28 +495,32 @@ __device__ __forceinline__ void initCell(CellLds& L, WaveCell& w, uint32_t n) {
 }
 
 // j is an endpoint of a current edge (wave-wide; algorithm.inl:368-376)
+template <uint32_t W = 64>
 __device__ __forceinline__ bool knownSite(const CellLds& L, const WaveCell& w, uint32_t j) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     bool hit = false;
-    for (uint32_t b = 0; b < w.E; b += 64) {
+    for (uint32_t b = 0; b < w.E; b += W) {
         const uint32_t e = b + lane;
         if (e < w.E) {
             const uint4 ed = L.edg[e];
             hit = hit || ed.x == j || ed.y == j;
         }
     }
-    return ballot(hit) != 0;
+    return G::vote(hit) != 0;
 }
 
 // The stored records of cell i (its vertices too when withVertices).
+template <uint32_t W = 64>
 __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, bool withVertices) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     const uint32_t E = c.enb[i];
     const uint32_t V = withVertices ? c.vnb[i] : 0u;
     const uint4* ge = c.edg + size_t(i) * kEdgeCap;
     const uint4* gv = c.vtx + size_t(i) * kVertexCap;
-    for (uint32_t e = lane; e < E; e += 64) L.edg[e] = ge[e];
-    for (uint32_t v = lane; v < V; v += 64) L.vtx[v] = gv[v];
+    for (uint32_t e = lane; e < E; e += W) L.edg[e] = ge[e];
+    for (uint32_t v = lane; v < V; v += W) L.vtx[v] = gv[v];
     w.V = V;
     w.E = E;
     w.F = 0;
@@ -474,12 +528,14 @@ __device__ __forceinline__ void loadCell(const CellCtx& c, CellLds& L, WaveCell&
     waveSync();
 }
 
+template <uint32_t W = 64>
 __device__ __forceinline__ void storeCell(const CellCtx& c, const CellLds& L, const WaveCell& w, uint32_t i) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    const uint32_t lane = G::lane();
     uint4* gv = c.vtx + size_t(i) * kVertexCap;
     uint4* ge = c.edg + size_t(i) * kEdgeCap;
-    for (uint32_t v = lane; v < w.V; v += 64) gv[v] = L.vtx[v];
-    for (uint32_t e = lane; e < w.E; e += 64) ge[e] = L.edg[e];
+    for (uint32_t v = lane; v < w.V; v += W) gv[v] = L.vtx[v];
+    for (uint32_t e = lane; e < w.E; e += W) ge[e] = L.edg[e];
     if (lane == 0) {
         c.vnb[i] = w.V;
         c.enb[i] = w.E;

@@ -36,57 +36,73 @@ __device__ __forceinline__ double rangeKnownRadius(const CellCtx& c, uint32_t i,
 // first `use` of the K-long list), then the security-radius certification.
 // The first pass starts from the octahedron; a resumed pass (a later kNN
 // pass) continues the stored cell and leaves certified and buried cells as
-// they are.
+// they are.  W lanes work on the cell (a wave, or half a wave with the other
+// half on another cell); the list is read W entries per lane step.
+template <uint32_t W>
 __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uint32_t* __restrict__ knn, uint32_t K,
                                          uint32_t use, uint32_t i, bool resume, const float* __restrict__ bound) {
-    const uint32_t lane = laneId();
+    using G = Lanes<W>;
+    constexpr uint32_t kChunks = 64u / W;
+    const uint32_t lane = G::lane();
     const uint32_t n = c.n;
     WaveCell w;
     const uint8_t st = c.status[i];
     const bool active = !(st == kStatusCertified || st == kStatusBuried || K == 0);
     if (resume) {
         if (!active) return;
-        loadCell(c, L, w, i, true);
+        loadCell<W>(c, L, w, i, true);
     } else {
-        initCell(L, w, artificialBase(c, cellMolecule(c, i)));
+        initCell<W>(L, w, artificialBase(c, cellMolecule(c, i)));
     }
     float kr = -FLT_MAX;
     bool certified = false;
     uint64_t readBytes = 1;  // status
     if (active) {
         const D4 si = loadSite(c, i);
-        uint32_t myJ = kNone;
-        bool myBuried = false;
-        if (lane < K) {
-            myJ = knn[size_t(i) * K + lane];
-            if (myJ != kNone && myJ >= n) {
-                atomicOr(c.err, kErrCellNeighbour);
-                myJ = kNone;
+        uint32_t myJ[kChunks];
+        uint64_t buriedMask[kChunks], noneMask[kChunks];
+        uint32_t count = 64u;
+#pragma unroll
+        for (uint32_t ch = 0; ch < kChunks; ch++) {
+            const uint32_t slot = ch * W + lane;
+            uint32_t j = kNone;
+            bool buried = false;
+            if (slot < K) {
+                j = knn[size_t(i) * K + slot];
+                if (j != kNone && j >= n) {
+                    atomicOr(c.err, kErrCellNeighbour);
+                    j = kNone;
+                }
+                if (j != kNone) buried = c.status[j] == kStatusBuried;
             }
-            if (myJ != kNone) myBuried = c.status[myJ] == kStatusBuried;
+            myJ[ch] = j;
+            buriedMask[ch] = G::vote(buried);
+            noneMask[ch] = G::vote(j == kNone);
+            // neighbours are used up to the first empty slot
+            if (count == 64u && noneMask[ch]) count = ch * W + uint32_t(__builtin_ctzll(noneMask[ch]));
         }
-        const uint64_t buriedMask = ballot(myBuried);
-        const uint64_t noneMask = ballot(myJ == kNone);
-        // neighbours are used up to the first empty slot
-        const uint32_t count = noneMask ? uint32_t(__builtin_ctzll(noneMask)) : 64u;
         const uint32_t limit = count < use ? count : use;
         double knownRadius = -DBL_MAX;
+        uint32_t used = 0;
         for (uint32_t t = 0; t < limit; t++) {
-            if ((buriedMask >> t) & 1u) continue;
-            const uint32_t j = readLane(myJ, t);
+            const uint32_t ch = t / W, tl = t % W;
+            const uint64_t bm = kChunks == 1u ? buriedMask[0] : (ch ? buriedMask[kChunks - 1u] : buriedMask[0]);
+            if ((bm >> tl) & 1u) continue;
+            const uint32_t src = kChunks == 1u ? myJ[0] : (ch ? myJ[kChunks - 1u] : myJ[0]);
+            const uint32_t j = G::read(src, tl);
             const D4 sj = loadSite(c, j);
-            updateCell(c, L, w, i, si, j, sj);
+            updateCell<W>(c, L, w, i, si, j, sj);
             knownRadius = __builtin_fmax(knownRadius, sphereDistD(si, sj));
+            used++;
         }
         if (bound && count < K) knownRadius = __builtin_fmax(knownRadius, rangeKnownRadius(c, i, bound[i], si));
-        const double maxR = cellRadius(c, L, w, si);
+        const double maxR = cellRadius<W>(c, L, w, si);
         certified = gtD(knownRadius, 2.0 * maxR);
         kr = float(knownRadius);
         // own site, K neighbour ids and their status, the sites used
-        const uint64_t lim = limit < 64u ? ((uint64_t(1) << limit) - 1u) : ~uint64_t(0);
-        readBytes += 16u + 5u * uint64_t(K) + 16u * uint64_t(popc64(lim & ~buriedMask));
+        readBytes += 16u + 5u * uint64_t(K) + 16u * uint64_t(used);
     }
-    storeCell(c, L, w, i);
+    storeCell<W>(c, L, w, i);
     if (lane == 0) {
         if (certified) c.status[i] = kStatusCertified;
         c.knownRadii[i] = kr;
@@ -95,10 +111,18 @@ __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uin
     }
 }
 
+// Neighbour count of cell i: the list length K, or with cls (the learned
+// proposal) the count of the cell's class, at most K.
+__device__ __forceinline__ uint32_t cellUse(const uint8_t* __restrict__ cls, uint32_t i, uint32_t K,
+                                            uint32_t fallbackK) {
+    if (!cls) return K;
+    const uint32_t q = cls[i];
+    const uint32_t use = q < kProposalClasses ? proposalK(q, fallbackK) : K;
+    return use > K ? K : use;
+}
+
 // Persistent cell pass: each wave takes cells from the XCD-ranged queue
-// (Morton order within each range) until every cell is done.  With cls
-// (the learned proposal), cell i runs the neighbour count of its class,
-// at most the list length K.
+// (Morton order within each range) until every cell is done.
 __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
                                               uint32_t* queue, const uint8_t* __restrict__ cls, uint32_t fallbackK,
                                               uint32_t resume, const float* __restrict__ bound) {
@@ -106,13 +130,27 @@ __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restr
     for (;;) {
         const uint32_t i = popWork(queue, c.n);
         if (i == kNone) break;
-        uint32_t use = K;
-        if (cls) {
-            const uint32_t q = cls[i];
-            use = q < kProposalClasses ? proposalK(q, fallbackK) : K;
-            if (use > K) use = K;
-        }
-        cellTask(c, L, knn, K, use, i, resume != 0u, bound);
+        cellTask<64>(c, L, knn, K, cellUse(cls, i, K, fallbackK), i, resume != 0u, bound);
+        waveSync();
+    }
+}
+
+// The same pass with two cells per wave, one per half: a cell update keeps
+// fewer than 32 records busy on most of its steps, so each half of the
+// wave's instructions works on a cell of its own.  Each half takes its own
+// cell from the queue; the records are those of k_cells.
+__global__ __launch_bounds__(64) void k_cells_pair(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
+                                                   uint32_t* queue, const uint8_t* __restrict__ cls,
+                                                   uint32_t fallbackK, uint32_t resume,
+                                                   const float* __restrict__ bound) {
+    __shared__ CellLds L[2];
+    const uint32_t half = laneId() >> 5;
+    for (;;) {
+        const uint32_t i0 = popWork(queue, c.n);
+        if (i0 == kNone) break;
+        const uint32_t i1 = popWork(queue, c.n);
+        const uint32_t i = half ? i1 : i0;
+        if (i != kNone) cellTask<32>(c, L[half], knn, K, cellUse(cls, i, K, fallbackK), i, resume != 0u, bound);
         waveSync();
     }
 }
@@ -167,19 +205,26 @@ hipError_t cellRadiiLaunch(const CellCtx& c, float* out, uint32_t blocks, hipStr
     return hipGetLastError();
 }
 
-hipError_t cellsOccupancy(uint32_t* blocksPerCu) {
-    int b = 0;
-    const hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cells, 64, 0);
+hipError_t cellsOccupancy(uint32_t* blocksPerCu, uint32_t* pairBlocksPerCu) {
+    int b = 0, p = 0;
+    hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_cells, 64, 0);
+    if (r == hipSuccess) r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_cells_pair, 64, 0);
     *blocksPerCu = uint32_t(b > 0 ? b : 1);
+    *pairBlocksPerCu = uint32_t(p > 0 ? p : 1);
     return r;
 }
 
 hipError_t cellsLaunch(const CellCtx& c, const uint32_t* knn, uint32_t k, uint32_t* queue, uint32_t blocks,
-                       hipStream_t st, const uint8_t* cls, uint32_t fallbackK, bool resume, const float* bound) {
+                       hipStream_t st, const uint8_t* cls, uint32_t fallbackK, bool resume, const float* bound,
+                       bool pair) {
     if (k > 64) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, queue, kXcdQueueWords, 0u);
-    hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue, cls, fallbackK,
-                       resume ? 1u : 0u, bound);
+    if (pair)
+        hipLaunchKernelGGL(k_cells_pair, dim3(blocks), dim3(64), 0, st, c, knn, k, queue, cls, fallbackK,
+                           resume ? 1u : 0u, bound);
+    else
+        hipLaunchKernelGGL(k_cells, dim3(blocks), dim3(64), 0, st, c, knn, k, queue, cls, fallbackK,
+                           resume ? 1u : 0u, bound);
     return hipGetLastError();
 }
 

@@ -82,6 +82,9 @@ typedef struct apo_config {
                                  1 each cell takes the sites below the MLP's search radius (the radius head)
                                  through a range query, as with apo_proposal_set_radii.  No counterpart in
                                  the reference */
+    uint32_t cells_per_wave;  /* cell pass: cells a wave64 works on at once, 1 (the whole wave on one cell) or
+                                 2 (half a wave each, twice the LDS per wave); 0 = default (1).  The records
+                                 are the same either way; an autotuner knob */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,

@@ -235,7 +235,10 @@ __device__ __forceinline__ void sort3(uint32_t& a, uint32_t& b, uint32_t& c) {
 }
 
 // One incremental update of cell i by site j (topology_update.inl:49-555).
-template <uint32_t W = 64>
+// Fuse: solve steps (1) and (2) together when they fit the group (the cell
+// pass, whose cells are small); the validation kernels, whose cells rarely
+// fit, keep the registers.
+template <uint32_t W = 64, bool Fuse = false>
 __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uint32_t i, const D4& si, uint32_t j,
                            const D4& sj) {
     using G = Lanes<W>;
@@ -255,19 +258,60 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
     const uint32_t baseV = w.V, baseE = w.E;
     buildNeighbours<W>(c, L, w);
 
+    // Steps (1) and (2) solve one quadrisector per record: each vertex's
+    // position (1), each edge's candidate vertices with s_j (2).  When the
+    // vertices and edges fit the group together, both run as one solve:
+    // edge e on lane e, vertex v on lane W - baseV + v (the same arithmetic
+    // per record, one pass instead of two).
+    const bool fused = Fuse && baseV + baseE <= W;
+    const bool vLane = fused && baseV != 0u && lane >= W - baseV;
+    D4 fq0{0, 0, 0, 0}, fq1{0, 0, 0, 0};
+    uint32_t fmask = 0;
+    if (fused) {
+        D4 a{0, 0, 0, 0}, b{0, 0, 0, 0}, d{0, 0, 0, 0};
+        bool act = false;
+        if (lane < baseE) {
+            const uint4 ed = L.edg[lane];
+            a = sj;
+            b = loadSite(c, ed.x);
+            d = loadSite(c, ed.y);
+            act = true;
+        } else if (vLane) {
+            const uint4 r = L.vtx[lane - (W - baseV)];
+            a = loadSite(c, r.x);
+            b = loadSite(c, r.y);
+            d = loadSite(c, r.z);
+            act = true;
+        }
+        if (act) fmask = quadrisector(si, a, b, d, fq0, fq1);
+    }
+
     // (1) vertices whose empty sphere intersects s_j
     uint32_t nInv = 0;
-    for (uint32_t b = 0; b < baseV; b += W) {
-        const uint32_t v = b + lane;
+    if (fused) {
+        const uint32_t v = lane - (W - baseV);
         bool inv = false;
-        if (v < baseV) {
+        if (vLane) {
             const uint4 r = L.vtx[v];
-            inv = intersectD(vertexPosition(c, si, r), sj);
+            inv = intersectD(selD4((r.w & 0xffu) != 0u, fq1, fq0), sj);
             if (inv) L.vtx[v].w = (r.w & 0xffu) | (0xffu << 8);
         }
         const uint64_t m = G::vote(inv);
-        if (inv) L.list[nInv + G::below(m)] = v;
-        nInv += popc64(m);
+        if (inv) L.list[G::below(m)] = v;
+        nInv = popc64(m);
+    } else {
+        for (uint32_t b = 0; b < baseV; b += W) {
+            const uint32_t v = b + lane;
+            bool inv = false;
+            if (v < baseV) {
+                const uint4 r = L.vtx[v];
+                inv = intersectD(vertexPosition(c, si, r), sj);
+                if (inv) L.vtx[v].w = (r.w & 0xffu) | (0xffu << 8);
+            }
+            const uint64_t m = G::vote(inv);
+            if (inv) L.list[nInv + G::below(m)] = v;
+            nInv += popc64(m);
+        }
     }
     waveSync();
     if (nInv) {
@@ -304,7 +348,13 @@ __device__ inline bool updateCell(const CellCtx& c, CellLds& L, WaveCell& w, uin
         if (has) {
             sk = loadSite(c, k);
             sl = loadSite(c, l);
-            mask = quadrisector(si, sj, sk, sl, q0, q1);
+            if (fused) {
+                q0 = fq0;
+                q1 = fq1;
+                mask = fmask;
+            } else {
+                mask = quadrisector(si, sj, sk, sl, q0, q1);
+            }
         }
         uint32_t vnb = ed.z;
         if (has && mask == 0 && vnb == kNone) {

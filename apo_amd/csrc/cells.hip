@@ -91,7 +91,7 @@ __device__ __forceinline__ void cellTask(const CellCtx& c, CellLds& L, const uin
             const uint32_t src = kChunks == 1u ? myJ[0] : (ch ? myJ[kChunks - 1u] : myJ[0]);
             const uint32_t j = G::read(src, tl);
             const D4 sj = loadSite(c, j);
-            updateCell<W>(c, L, w, i, si, j, sj);
+            updateCell<W, true>(c, L, w, i, si, j, sj);
             knownRadius = __builtin_fmax(knownRadius, sphereDistD(si, sj));
             used++;
         }

@@ -28,7 +28,8 @@ class Config(ctypes.Structure):
                 ("waves_cells", ctypes.c_uint32), ("waves_vertex", ctypes.c_uint32), ("waves_edge", ctypes.c_uint32),
                 ("waves_bisector", ctypes.c_uint32), ("proposal_fallback_k", ctypes.c_uint32),
                 ("proposal_max_k", ctypes.c_uint32), ("knn_passes", ctypes.c_uint32), ("complete", ctypes.c_uint32),
-                ("proposal_radius", ctypes.c_uint32), ("cells_per_wave", ctypes.c_uint32)]
+                ("proposal_radius", ctypes.c_uint32), ("cells_per_wave", ctypes.c_uint32),
+                ("bisector_per_cell", ctypes.c_uint32)]
 
 
 class Precision(ctypes.Structure):

@@ -97,7 +97,8 @@ struct apo_ctx {
     uint32_t* offsets = nullptr;
     uint64_t* moleculeVertices = nullptr;
     // blocks per CU of the persistent kernels at full occupancy
-    uint32_t occCells = 1, occCellsPair = 1, occVertex = 1, occEdge = 1, occBisector = 1;
+    uint32_t occCells = 1, occCellsPair = 1, occVertex = 1, occEdge = 1, occBisector = 1, occBisectorPlan = 1,
+             occBisectorLanes = 1;
     // the captured build (config.graph) and what it was captured for
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -137,6 +138,11 @@ struct apo_ctx {
     uint32_t spillBlocks = 0;
     uint32_t* err = nullptr;        // device guard bits
     uint32_t* tasks = nullptr;
+    // the bisector stage split by search (BisectorSplit); null in static builds
+    uint32_t* bisSearch = nullptr;
+    uint32_t* bisNb = nullptr;
+    uint32_t* bisCount = nullptr;
+    uint8_t* bisMark = nullptr;
     // output
     uint32_t* cellCount = nullptr;
     uint32_t* blockSums = nullptr;
@@ -269,6 +275,10 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     APO_HIP(alloc(ctx->counters, kCounterWords));
     APO_HIP(alloc(ctx->err, 1));
     APO_HIP(alloc(ctx->tasks, cap));
+    APO_HIP(alloc(ctx->bisSearch, size_t(cap) * kBisSearchSlots));
+    APO_HIP(alloc(ctx->bisNb, size_t(cap) * kBisNeighbourSlots));
+    APO_HIP(alloc(ctx->bisCount, cap));
+    APO_HIP(alloc(ctx->bisMark, cap));
     APO_HIP(alloc(ctx->cellCount, cap));
     APO_HIP(alloc(ctx->blockSums, (cap + 255) / 256 + 2));
     APO_HIP(alloc(ctx->total, 1));
@@ -487,14 +497,18 @@ apo_status enqueueBuild(apo_ctx* ctx, const F4* sitesDev, uint32_t n, bool learn
     const ValidationGrid vg{std::min(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex), ctx->spillBlocks),
                             std::min(persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge), ctx->spillBlocks),
                             std::min(persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector),
-                                     ctx->spillBlocks)};
+                                     ctx->spillBlocks),
+                            persistentBlocks(ctx, 0u, ctx->occBisectorPlan),
+                            std::min(persistentBlocks(ctx, 0u, ctx->occBisectorLanes), ctx->spillBlocks)};
+    const BisectorSplit bsplit{ctx->bisSearch, ctx->bisNb, ctx->bisCount, ctx->bisMark};
     if (ctx->cfg.stage_mask & 2u)
         APO_HIP(validationLaunch(cc, vb, vg, ctx->cfg.max_iterations, ctx->cfg.max_restarts, false, st,
                                  timed ? &ctx->vt : nullptr));
     APO_TIMED_STAGE("vertex/edge validation");
     APO_HIP(mark(4));
     if (ctx->cfg.stage_mask & 4u)
-        APO_HIP(validationLaunch(cc, vb, vg, 1, ctx->cfg.max_restarts, true, st, nullptr));
+        APO_HIP(validationLaunch(cc, vb, vg, 1, ctx->cfg.max_restarts, true, st, nullptr, nullptr,
+                                 ctx->cfg.bisector_per_cell ? nullptr : &bsplit));
     APO_TIMED_STAGE("bisector validation");
     APO_HIP(mark(5));
     if (ctx->cfg.complete && (ctx->cfg.stage_mask & 2u)) {
@@ -750,6 +764,7 @@ void apo_default_config(apo_config* cfg) {
     cfg->complete = 0;
     cfg->proposal_radius = 0;
     cfg->cells_per_wave = 0;
+    cfg->bisector_per_cell = 0;
 }
 
 // pcg hash of apo/src/core/utils.cpp:6-12, scaled by 2^-32
@@ -787,7 +802,7 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     const uint32_t k = ctx->cfg.k, fk = ctx->cfg.proposal_fallback_k, mk = ctx->cfg.proposal_max_k;
     auto listK = [](uint32_t v) { return v == 16 || v == 24 || v == 32 || v == 48 || v == 64; };
     if (k == 0 || k > 64 || ctx->cfg.max_iterations > kMaxRounds || !listK(fk) || !listK(mk) ||
-        ctx->cfg.complete > kMaxCompletionRounds || ctx->cfg.cells_per_wave > 2) {
+        ctx->cfg.complete > kMaxCompletionRounds || ctx->cfg.cells_per_wave > 2 || ctx->cfg.bisector_per_cell > 1) {
         delete ctx;
         return APO_ERR_INVALID;
     }
@@ -797,7 +812,9 @@ apo_status apo_create(apo_ctx** out, const apo_config* cfg) {
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ctx->cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
     if (e == hipSuccess) e = staticScratchAddress(ctx->device, &ctx->staticScratch);
     if (e == hipSuccess) e = cellsOccupancy(&ctx->occCells, &ctx->occCellsPair);
-    if (e == hipSuccess) e = validationOccupancy(&ctx->occVertex, &ctx->occEdge, &ctx->occBisector);
+    if (e == hipSuccess)
+        e = validationOccupancy(&ctx->occVertex, &ctx->occEdge, &ctx->occBisector, &ctx->occBisectorPlan,
+                                &ctx->occBisectorLanes);
     for (int s = 0; s < 10 && e == hipSuccess; s++) e = hipEventCreate(&ctx->ev[s]);
     for (uint32_t r = 0; r < kTimedRounds && e == hipSuccess; r++)
         for (int q = 0; q < 3 && e == hipSuccess; q++) e = hipEventCreate(&ctx->vt.ev[r][q]);
@@ -821,6 +838,7 @@ void apo_destroy(apo_ctx* ctx) {
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
+                    ctx->bisSearch, ctx->bisNb, ctx->bisCount, ctx->bisMark,
                     ctx->params, ctx->feat, ctx->cls, ctx->labels, ctx->givenCls, ctx->cavRatios, ctx->spill, ctx->want, ctx->stuck, ctx->givenRadii, ctx->radiiSorted, ctx->d16, ctx->pw.w1, ctx->pw.w2, ctx->pw.w3,
                     ctx->pw.b1, ctx->pw.b2, ctx->pw.b3};
     for (void* p : ptrs)
@@ -875,9 +893,10 @@ apo_status prepareBuild(apo_ctx* ctx, uint32_t n, const uint32_t* offsets, uint3
     ctx->cavValid = false;
     // per-wave device regions of the vertex, edge and bisector validation
     // grids (the kernels run one after the other on the stream and share them)
-    const uint32_t vblocks = std::max(std::max(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
-                                               persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge)),
-                                      persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector));
+    const uint32_t vblocks = std::max(std::max(std::max(persistentBlocks(ctx, ctx->cfg.waves_vertex, ctx->occVertex),
+                                                        persistentBlocks(ctx, ctx->cfg.waves_edge, ctx->occEdge)),
+                                               persistentBlocks(ctx, ctx->cfg.waves_bisector, ctx->occBisector)),
+                                      persistentBlocks(ctx, 0u, ctx->occBisectorLanes));
     if (vblocks > ctx->spillBlocks) {
         dropGraph(ctx);
         if (ctx->spill) APO_HIP(hipFree(ctx->spill));
@@ -952,7 +971,7 @@ namespace {
     X(scene) X(boxPartials) X(boxTicket) X(sortScratch) X(range) X(startLists) X(codesA) X(codesB) X(idxA) X(idxB) \
     X(sorted) X(nodes) X(parent) X(sizes) X(refitCounter) X(rootBox) X(status) X(knownRadii) X(flags) X(vtx)       \
     X(vnb) X(edg) X(enb) X(knn) X(counters) X(spill) X(err) X(tasks) X(cellCount) X(blockSums) X(total)            \
-    X(outXyzw) X(outIds) X(bytes) X(want) X(stuck)
+    X(outXyzw) X(outIds) X(bytes) X(want) X(stuck) X(bisSearch) X(bisNb) X(bisCount) X(bisMark)
 
 struct BuildBuffers {
 #define APO_DECLARE(f) decltype(apo_ctx::f) f;
@@ -1037,6 +1056,11 @@ bool carveStatic(void* base, uint32_t n, uint32_t K, double* xyzw, uint32_t* ids
     c.take(b.stuck, kStuckCap);
     c.take(b.spill, size_t(kStaticWaveRegions) * waveScratchWords());
     b.spillBlocks = kStaticWaveRegions;
+    // the bisector stage runs per cell (no room for the split stage's lists)
+    b.bisSearch = nullptr;
+    b.bisNb = nullptr;
+    b.bisCount = nullptr;
+    b.bisMark = nullptr;
     b.outXyzw = reinterpret_cast<D4*>(xyzw);
     b.outIds = reinterpret_cast<uint4*>(ids);
     b.outCap = capacity;

@@ -214,7 +214,9 @@ constexpr uint32_t kCompletionUpdated = kRoundTasks + kMaxRounds;
 constexpr uint32_t kCompletionStuck = kCompletionUpdated + kMaxCompletionRounds;
 constexpr uint32_t kCompletionTasks = kCompletionStuck + kMaxCompletionRounds;
 constexpr uint32_t kCompletionRebuilt = kCompletionTasks + (1 + kMaxCompletionRounds) * kMaxRounds;
-constexpr uint32_t kCounterWords = kCompletionRebuilt + 1;
+// the bisector stage split by search: cells marked for the per-cell stage
+constexpr uint32_t kBisMarked = kCompletionRebuilt + 1;
+constexpr uint32_t kCounterWords = kBisMarked + 1;
 // A round's list of stuck records (cell, the record's other three sites):
 // a round that finds more rebuilds no cell.  want[] codes besides site ids:
 constexpr uint32_t kStuckCap = 16384;
@@ -252,6 +254,45 @@ __device__ __forceinline__ uint32_t popWork(uint32_t* heads, uint32_t count) {
     return __builtin_amdgcn_readfirstlane(r);
 }
 
+// A wave's claim on a work queue: items [cur, end) taken but not yet served.
+struct WorkClaim {
+    uint32_t cur = 0, end = 0;
+};
+
+// popWork taking up to `batch` consecutive items of a range per atomic: the
+// queue's heads are device-scope atomics shared by every XCD, whose
+// throughput, not the kernels, bounds a grid that takes one cell at a time
+// (a per-cell launch of a million cheap cells spent most of its time here).
+// Returns the next item of the wave's claim, claiming more when it is
+// served; kNone when every range is drained.
+__device__ __forceinline__ uint32_t popWorkBatch(uint32_t* heads, uint32_t count, WorkClaim& w, uint32_t batch) {
+    if (w.cur < w.end) return w.cur++;
+    constexpr uint32_t kXcd = kXcdQueueWords;
+    uint32_t first = kNone, stop = kNone;
+    if (laneId() == 0) {
+        const uint32_t home = blockIdx.x % kXcd;
+        const uint32_t base = count / kXcd, extra = count % kXcd;
+        for (uint32_t k = 0; k < kXcd; k++) {
+            const uint32_t x = (home + k) % kXcd;
+            const uint32_t len = base + (x < extra ? 1u : 0u);
+            if (__hip_atomic_load(&heads[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= len) continue;
+            const uint32_t t = atomicAdd(&heads[x], batch);
+            if (t < len) {
+                const uint32_t start = x * base + (x < extra ? x : extra);
+                first = start + t;
+                stop = start + (len - t < batch ? len : t + batch);
+                break;
+            }
+        }
+    }
+    first = __builtin_amdgcn_readfirstlane(first);
+    stop = __builtin_amdgcn_readfirstlane(stop);
+    if (first == kNone) return kNone;
+    w.cur = first + 1u;
+    w.end = stop;
+    return first;
+}
+
 // Validation (algorithm.inl:208-1233).  Task lists live on the device.
 struct ValidationBuffers {
     uint32_t* tasks;       // n
@@ -262,9 +303,23 @@ struct ValidationBuffers {
 // Words of device memory per validation wave (deep stack slots of the vertex
 // and edge searches, then the edge cover buffers).
 uint32_t waveScratchWords();
-// Persistent grid sizes (blocks of one wave).
+// Persistent grid sizes (blocks of one wave); bisectorPlan and
+// bisectorLanes: the grids of the split bisector stage (bisectorLanes 0: the
+// stage runs per cell only).
 struct ValidationGrid {
-    uint32_t vertex, edge, bisector;
+    uint32_t vertex, edge, bisector, bisectorPlan, bisectorLanes;
+};
+// Slots of the bisector stage split by search (validationLaunch), per cell:
+// its searches' bisector sites, its distinct neighbours, the counts (searches
+// | neighbours << 16) and a mark (1: the per-cell stage runs it).  A cell
+// with more searches or neighbours than slots is marked.
+constexpr uint32_t kBisSearchSlots = 32;
+constexpr uint32_t kBisNeighbourSlots = 64;
+struct BisectorSplit {
+    uint32_t* search;  // n * kBisSearchSlots
+    uint32_t* nb;      // n * kBisNeighbourSlots
+    uint32_t* count;   // n
+    uint8_t* mark;     // n
 };
 // Events around each vertex / edge validation launch (rounds beyond
 // kTimedRounds run untimed).
@@ -273,11 +328,15 @@ struct ValidationTiming {
     hipEvent_t ev[kTimedRounds][3];  // before vertex, between, after edge
     uint32_t rounds;                 // rounds with events
 };
-hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks);
-// roundTasks: one word per round for its task count (vertex + edge rounds)
+hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks,
+                               uint32_t* bisectorPlanBlocks, uint32_t* bisectorLaneBlocks);
+// roundTasks: one word per round for its task count (vertex + edge rounds).
+// bisector: the bisector stage instead, split by search when `split` holds
+// buffers and g.bisectorLanes is set, else one wave per cell.
 hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const ValidationGrid& g,
                             uint32_t iterations, uint32_t maxRestarts, bool bisector, hipStream_t st,
-                            ValidationTiming* tm, uint32_t* roundTasks = nullptr);
+                            ValidationTiming* tm, uint32_t* roundTasks = nullptr,
+                            const BisectorSplit* split = nullptr);
 // One consistency round of the completion mode (complete.hip): want holds n
 // words (kNone or kWantRebuilt between rounds, kNone at the build's start),
 // stuck kStuckCap records; round < kMaxCompletionRounds.

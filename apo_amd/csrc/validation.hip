@@ -1177,6 +1177,267 @@ __device__ __forceinline__ void bisectorTask(const CellCtx& c, BisectorLds& S, u
 }
 
 // ---------------------------------------------------------------------------
+// The bisector stage split by search.  A cell runs a handful of bisector
+// searches (about five), so a wave that serves one cell keeps most of its
+// lanes idle and waits on its few traversals.  The split stage gives each
+// lane a search of its own, from many cells, and leaves the per-cell stage
+// (bisectorTask) only the cells where a search finds a contributor:
+//  * k_bisector_plan, a wave per cell: the cell's sorted distinct bisectors
+//    above i (algorithm.inl:903-968), less those inside its known radius
+//    (:993-1006), go to the cell's search slots, and its distinct
+//    neighbours to its neighbour slots; the cell's status becomes validated
+//    (a cell with more searches or neighbours than slots is marked);
+//  * k_bisector_lanes, a search per lane: waves take runs of consecutive
+//    cells and hand their searches to idle lanes; each runs the reference's
+//    traversal (bisectorStep) and candidate test (not a neighbour, a
+//    two-point trisector whose second point no neighbour other than bk
+//    intersects) against the neighbour slots; a contributor marks its cell;
+//  * the per-cell stage over the marked cells, updates included.
+// Until a cell's first update, every search of the reference's stage sees
+// the cell as stored, which is what the lanes search against; a cell none of
+// whose searches finds a contributor is never updated, so the reference
+// leaves it validated with the flags of its searches (stack overflow at the
+// reference's depth: the lanes walk the same order) and of its bisector list.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bisectorPlanTask(const CellCtx& c, BisectorLds& S, uint32_t i,
+                                                 const BisectorSplit& sp) {
+    CellLds& L = S.cell;
+    const uint32_t lane = laneId();
+    if (c.status[i] == kStatusBuried) {
+        if (lane == 0) {
+            sp.count[i] = 0u;
+            sp.mark[i] = 0u;
+        }
+        return;
+    }
+    WaveCell w;
+    loadCell(c, L, w, i, false);
+    uint32_t flags = 0;
+    buildNeighbours(c, L, w);
+    const uint32_t F = w.F;
+    // sorted distinct neighbours above i, as bisectorTask lists them
+    uint32_t B = 0;
+    for (uint32_t b = 0; b < F; b += 64) {
+        const uint32_t q = b + lane;
+        uint32_t id = kNone;
+        bool keep = false;
+        if (q < F) {
+            id = L.nb[q];
+            keep = id < c.n && id > i;
+        }
+        uint32_t rank = 0;
+        if (keep)
+            for (uint32_t q2 = 0; q2 < F; q2++) {
+                const uint32_t o = L.nb[q2];
+                if (o < c.n && o > i && o < id) rank++;
+            }
+        if (keep && rank < kBisectorCap) S.bis[rank] = id;
+        B += popc64(ballot(keep));
+    }
+    if (B > kBisectorCap) {
+        B = kBisectorCap;
+        flags |= kFlagBisectorOverflow;
+    }
+    waveSync();
+    // the searches the reference runs: bisectors outside the known radius
+    const float kr = c.knownRadii[i];
+    const D4 si = loadSite(c, i);
+    const F3 fci{float(si.x), float(si.y), float(si.z)};
+    const float fri = float(si.w);
+    uint32_t T = 0;
+    for (uint32_t b = 0; b < B; b += 64) {
+        const uint32_t q = b + lane;
+        uint32_t bk = kNone;
+        bool run = false;
+        if (q < B) {
+            bk = S.bis[q];
+            const D4 sk = loadSite(c, bk);
+            const F3 fck{float(sk.x), float(sk.y), float(sk.z)};
+            const float dist = lenf(subf(fci, fck)) + fri + float(sk.w);
+            run = !ltF(dist, kr);
+        }
+        const uint64_t m = ballot(run);
+        waveSync();
+        if (run) S.bis[T + countBelow(m)] = bk;
+        T += popc64(m);
+        waveSync();
+    }
+    const bool fits = T <= kBisSearchSlots && F <= kBisNeighbourSlots;
+    if (fits) {
+        if (lane < T) sp.search[size_t(i) * kBisSearchSlots + lane] = S.bis[lane];
+        if (lane < F) sp.nb[size_t(i) * kBisNeighbourSlots + lane] = L.nb[lane];
+    }
+    if (lane == 0) {
+        sp.count[i] = fits ? (T | (F << 16)) : 0u;
+        sp.mark[i] = fits ? 0u : 1u;
+        c.status[i] = kStatusCertified;
+        if (flags) c.flags[i] |= flags;
+        // status, known radius, counts and edges read; searches, neighbours,
+        // counts, mark and status written
+        countBytes(c, kBytesBisectorVal, 25u + 16u * uint64_t(w.E) + 4u * (T + F) + 10u);
+    }
+}
+
+// Each wave plans a contiguous share of the cells (every cell costs about the
+// same here, and a work queue's atomics would cost more than the planning).
+__global__ __launch_bounds__(64) void k_bisector_plan(CellCtx c, BisectorSplit sp) {
+    __shared__ BisectorLds S;
+    const uint32_t per = (c.n + gridDim.x - 1u) / gridDim.x;
+    const uint32_t lo = blockIdx.x * per, hi = lo + per < c.n ? lo + per : c.n;
+    for (uint32_t i = lo; i < hi; i++) {
+        bisectorPlanTask(c, S, i, sp);
+        waveSync();
+    }
+}
+
+// Neighbour id j is among the cell's F neighbour slots (four per step, their
+// loads in flight together).
+__device__ __forceinline__ bool slotKnown(const uint32_t* __restrict__ nb, uint32_t F, uint32_t j) {
+    uint32_t q = 0;
+    for (; q + 4u <= F; q += 4u) {
+        const uint32_t a = nb[q], b = nb[q + 1u], d = nb[q + 2u], e = nb[q + 3u];
+        if (a == j || b == j || d == j || e == j) return true;
+    }
+    for (; q < F; q++)
+        if (nb[q] == j) return true;
+    return false;
+}
+
+// No neighbour other than skip intersects the sphere x (clearOfNeighbours
+// over the neighbour slots).
+__device__ __forceinline__ bool slotClear(const CellCtx& c, const uint32_t* __restrict__ nb, uint32_t F, const D4& x,
+                                          uint32_t skip) {
+    for (uint32_t q = 0; q < F; q++) {
+        const uint32_t id = nb[q];
+        if (id == skip) continue;
+        if (intersectD(loadSite(c, id), x)) return false;
+    }
+    return true;
+}
+
+// Cells per run a wave takes from the queue, and the idle lanes that make
+// the wave hand out searches (or all lanes).
+constexpr uint32_t kBisRunCells = 32;
+constexpr uint32_t kLaneRefill = 16;
+
+__global__ __launch_bounds__(64) void k_bisector_lanes(CellCtx c, uint32_t* counters, BisectorSplit sp,
+                                                       uint32_t* spill) {
+    __shared__ uint32_t ring[kVertexLdsDepth][64];
+    __shared__ uint32_t runEnd[kBisRunCells];  // inclusive prefix of the run's search counts
+    uint32_t* mySpill = spill + size_t(blockIdx.x) * kWaveScratchWords;
+    const VStack st{ring, mySpill};
+    const uint32_t lane = laneId();
+    const uint32_t runs = (c.n + kBisRunCells - 1u) / kBisRunCells;
+    // the wave's current run: cells [run0, run0 + 32), searches [next, total)
+    uint32_t run0 = 0, next = 0, total = 0;
+    bool drained = false;
+    Trav tr{0u, 0u, false};
+    uint32_t i = kNone, bk = kNone, cand = kNone, flags = 0, F = 0;
+    const uint32_t* nb = nullptr;
+    D4 si{0, 0, 0, 0}, sk{0, 0, 0, 0};
+    F3 fci{0, 0, 0}, fck{0, 0, 0};
+    F4 bsph{0, 0, 0, 0};
+    float cylR = 0.f;
+    bool axisPrune = false;
+    for (;;) {
+        uint64_t idle = ballot(!tr.live && cand == kNone);
+        if (!drained && (popc64(idle) >= kLaneRefill || idle == ~uint64_t(0))) {
+            if (next == total) {
+                // the next run of cells and the prefix of their search counts
+                const uint32_t r = popWork(counters + kBisectorQueue, runs);
+                if (r == kNone) {
+                    drained = true;
+                } else {
+                    run0 = r * kBisRunCells;
+                    const uint32_t cell = run0 + lane;
+                    const uint32_t cnt = lane < kBisRunCells && cell < c.n ? (sp.count[cell] & 0xffffu) : 0u;
+                    const uint32_t incl = waveInclusiveSum(cnt);
+                    if (lane < kBisRunCells) runEnd[lane] = incl;
+                    total = readLane(incl, kBisRunCells - 1u);
+                    next = 0;
+                    waveSync();
+                }
+            }
+            if (next < total) {
+                // idle lanes take the run's next searches in order
+                const uint32_t rank = countBelow(idle);
+                const uint32_t k = next + rank;
+                if (((idle >> lane) & 1u) && k < total) {
+                    uint32_t lo = 0;
+                    while (runEnd[lo] <= k) lo++;  // the run's cell holding search k
+                    i = run0 + lo;
+                    const uint32_t first = lo ? runEnd[lo - 1u] : 0u;
+                    const uint32_t cnt = sp.count[i];
+                    F = cnt >> 16;
+                    nb = sp.nb + size_t(i) * kBisNeighbourSlots;
+                    bk = sp.search[size_t(i) * kBisSearchSlots + (k - first)];
+                    si = loadSite(c, i);
+                    sk = loadSite(c, bk);
+                    fci = F3{float(si.x), float(si.y), float(si.z)};
+                    fck = F3{float(sk.x), float(sk.y), float(sk.z)};
+                    const float fri = float(si.w), frk = float(sk.w);
+                    const F3 center = mulf(addf(fci, fck), .5f);
+                    const float radius = lenf(subf(center, fci)) + __builtin_fmaxf(fri, frk);
+                    bsph = F4{center.x, center.y, center.z, radius};
+                    cylR = __builtin_fmaxf(fri, frk);
+                    const uint32_t mol = cellMolecule(c, i);
+                    axisPrune = c.scene[mol].rmin >= 0.0;
+                    tr = Trav{rootNode(c, mol), 0u, true};
+                    flags = 0;
+                }
+                const uint32_t given = popc64(idle);
+                next = next + given < total ? next + given : total;
+            }
+        }
+        const uint64_t stepping = ballot(tr.live && cand == kNone), waiting = ballot(cand != kNone);
+        if (stepping == 0 && waiting == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (waiting != 0 && popc64(waiting) >= popc64(stepping)) {
+            // the waiting candidates together (bisectorSearches' test)
+            if (cand != kNone) {
+                bool ok = !slotKnown(nb, F, cand);
+                if (ok) {
+                    D4 x0{0, 0, 0, 0}, x1{0, 0, 0, 0};
+                    const uint32_t cnt = trisector(si, loadSite(c, cand), sk, x0, x1);
+                    ok = cnt == 2u && slotClear(c, nb, F, x1, bk);
+                }
+                if (ok) {
+                    sp.mark[i] = 1u;
+                    tr.live = false;
+                }
+                cand = kNone;
+                if (!tr.live && flags) {
+                    atomicOr(&c.flags[i], flags);
+                    flags = 0;
+                }
+            }
+            continue;
+        }
+        if (tr.live && cand == kNone) {
+            cand = bisectorStep(c, st, i, bk, fci, fck, bsph, cylR, axisPrune, tr, flags);
+            if (!tr.live && cand == kNone && flags) {
+                atomicOr(&c.flags[i], flags);
+                flags = 0;
+            }
+        }
+    }
+}
+
+// Cells marked for the per-cell stage, appended in wave order.
+__global__ __launch_bounds__(256) void k_compact_marked(const uint8_t* __restrict__ mark, uint32_t n,
+                                                        uint32_t* list, uint32_t* count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool take = i < n && mark[i] != 0u;
+    const uint64_t m = ballot(take);
+    uint32_t base = 0;
+    if (laneId() == 0 && m) base = atomicAdd(count, popc64(m));
+    base = __shfl(base, 0, 64);
+    if (take) list[base + countBelow(m)] = i;
+}
+
+// ---------------------------------------------------------------------------
 // Persistent launches: every wave takes cells from a device work queue
 // until it is drained, so the grid is sized to the chip, not to the task
 // count, and no launch depends on a count read back by the host.
@@ -1211,14 +1472,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     }
 }
 
+// Every cell, or (list) the listed cells: the per-cell bisector stage.
 __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t* counters, uint32_t maxRestarts,
-                                                            uint32_t* spill) {
+                                                            uint32_t* spill, const uint32_t* __restrict__ list) {
     __shared__ BisectorLds S;
     uint32_t* mySpill = spill + size_t(blockIdx.x) * kWaveScratchWords;
+    const uint32_t count = list ? readFirst(counters[kBisMarked]) : c.n;
     for (;;) {
-        const uint32_t i = popWork(counters + kBisectorQueue, c.n);
-        if (i == kNone) break;
-        bisectorTask(c, S, i, maxRestarts, mySpill);
+        const uint32_t t = popWork(counters + kBisectorQueue, count);
+        if (t == kNone) break;
+        bisectorTask(c, S, list ? list[t] : t, maxRestarts, mySpill);
         waveSync();
     }
 }
@@ -1243,26 +1506,47 @@ __global__ void k_round_reset(uint32_t* counters) {
 
 uint32_t waveScratchWords() { return kWaveScratchWords; }
 
-hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks) {
-    int v = 0, e = 0, b = 0;
+hipError_t validationOccupancy(uint32_t* vertexBlocks, uint32_t* edgeBlocks, uint32_t* bisectorBlocks,
+                               uint32_t* bisectorPlanBlocks, uint32_t* bisectorLaneBlocks) {
+    int v = 0, e = 0, b = 0, p = 0, l = 0;
     hipError_t r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, k_vertex_validation, 64, 0);
     if (r == hipSuccess) r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&e, k_edge_validation, 64, 0);
     if (r == hipSuccess) r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_bisector_validation, 64, 0);
+    if (r == hipSuccess) r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, k_bisector_plan, 64, 0);
+    if (r == hipSuccess) r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&l, k_bisector_lanes, 64, 0);
     *vertexBlocks = uint32_t(v > 0 ? v : 1);
     *edgeBlocks = uint32_t(e > 0 ? e : 1);
     *bisectorBlocks = uint32_t(b > 0 ? b : 1);
+    *bisectorPlanBlocks = uint32_t(p > 0 ? p : 1);
+    *bisectorLaneBlocks = uint32_t(l > 0 ? l : 1);
     return r;
 }
 
 hipError_t validationLaunch(const CellCtx& c, const ValidationBuffers& v, const ValidationGrid& g,
                             uint32_t iterations, uint32_t maxRestarts, bool bisector, hipStream_t st,
-                            ValidationTiming* tm, uint32_t* roundTasks) {
+                            ValidationTiming* tm, uint32_t* roundTasks, const BisectorSplit* split) {
     if (!roundTasks) roundTasks = v.counters + kRoundTasks;
     if (bisector) {
-        if (g.bisector > v.spillBlocks) return hipErrorInvalidValue;
+        if (g.bisector > v.spillBlocks || g.bisectorLanes > v.spillBlocks) return hipErrorInvalidValue;
+        const uint32_t* list = nullptr;
+        if (split && split->search && g.bisectorLanes) {
+            // plan, search per lane, then the marked cells per cell (a list
+            // in the validation task buffer, free until the next round)
+            hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, v.counters + kBisectorQueue, kXcdQueueWords,
+                               0u);
+            hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, v.counters + kBisMarked, 1u, 0u);
+            hipLaunchKernelGGL(k_bisector_plan, dim3(g.bisectorPlan), dim3(64), 0, st, c, *split);
+            hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, v.counters + kBisectorQueue, kXcdQueueWords,
+                               0u);
+            hipLaunchKernelGGL(k_bisector_lanes, dim3(g.bisectorLanes), dim3(64), 0, st, c, v.counters, *split,
+                               v.spill);
+            hipLaunchKernelGGL(k_compact_marked, dim3((c.n + 255) / 256), dim3(256), 0, st, split->mark, c.n, v.tasks,
+                               v.counters + kBisMarked);
+            list = v.tasks;
+        }
         hipLaunchKernelGGL(k_fill_words, dim3(1), dim3(64), 0, st, v.counters + kBisectorQueue, kXcdQueueWords, 0u);
         hipLaunchKernelGGL(k_bisector_validation, dim3(g.bisector), dim3(64), 0, st, c, v.counters, maxRestarts,
-                           v.spill);
+                           v.spill, list);
         return hipGetLastError();
     }
     if (iterations > kMaxRounds || g.vertex > v.spillBlocks || g.edge > v.spillBlocks) return hipErrorInvalidValue;

@@ -85,6 +85,9 @@ typedef struct apo_config {
     uint32_t cells_per_wave;  /* cell pass: cells a wave64 works on at once, 1 (the whole wave on one cell) or
                                  2 (half a wave each, twice the LDS per wave); 0 = default (1).  The records
                                  are the same either way; an autotuner knob */
+    uint32_t bisector_per_cell; /* bisector validation: 0 (default) plans each cell's searches, runs them one per
+                                   lane across cells and runs the per-cell stage only where a search finds a
+                                   contributor; 1 runs every cell's stage per cell.  The records are the same */
 } apo_config;
 
 /* Per-build statistics (the stage split of AlgorithmGPU::Sample,

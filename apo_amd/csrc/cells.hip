@@ -121,14 +121,24 @@ __device__ __forceinline__ uint32_t cellUse(const uint8_t* __restrict__ cls, uin
     return use > K ? K : use;
 }
 
+// Consecutive cells a wave of the cell pass claims per queue atomic: up to 8,
+// while every wave still makes at least 32 claims (short passes keep claims
+// of one cell, so the last cells spread over the waves).
+__device__ __forceinline__ uint32_t cellBatch(uint32_t cells) {
+    const uint32_t perWave = cells / (gridDim.x * 32u);
+    return perWave < 1u ? 1u : (perWave > 8u ? 8u : perWave);
+}
+
 // Persistent cell pass: each wave takes cells from the XCD-ranged queue
 // (Morton order within each range) until every cell is done.
 __global__ __launch_bounds__(64) void k_cells(CellCtx c, const uint32_t* __restrict__ knn, uint32_t K,
                                               uint32_t* queue, const uint8_t* __restrict__ cls, uint32_t fallbackK,
                                               uint32_t resume, const float* __restrict__ bound) {
     __shared__ CellLds L;
+    WorkClaim claim;
+    const uint32_t batch = cellBatch(c.n);
     for (;;) {
-        const uint32_t i = popWork(queue, c.n);
+        const uint32_t i = popWorkBatch(queue, c.n, claim, batch);
         if (i == kNone) break;
         cellTask<64>(c, L, knn, K, cellUse(cls, i, K, fallbackK), i, resume != 0u, bound);
         waveSync();
@@ -145,10 +155,12 @@ __global__ __launch_bounds__(64) void k_cells_pair(CellCtx c, const uint32_t* __
                                                    const float* __restrict__ bound) {
     __shared__ CellLds L[2];
     const uint32_t half = laneId() >> 5;
+    WorkClaim claim;
+    const uint32_t batch = cellBatch(c.n);
     for (;;) {
-        const uint32_t i0 = popWork(queue, c.n);
+        const uint32_t i0 = popWorkBatch(queue, c.n, claim, batch);
         if (i0 == kNone) break;
-        const uint32_t i1 = popWork(queue, c.n);
+        const uint32_t i1 = popWorkBatch(queue, c.n, claim, batch);
         const uint32_t i = half ? i1 : i0;
         if (i != kNone) cellTask<32>(c, L[half], knn, K, cellUse(cls, i, K, fallbackK), i, resume != 0u, bound);
         waveSync();

@@ -1437,6 +1437,11 @@ __global__ __launch_bounds__(256) void k_compact_marked(const uint8_t* __restric
     if (take) list[base + countBelow(m)] = i;
 }
 
+// Tasks a validation wave claims per queue atomic: one (the costs of cells
+// vary, and neighbouring cells cost alike, so longer claims unbalance the
+// waves).
+constexpr uint32_t kTaskBatch = 1;
+
 // ---------------------------------------------------------------------------
 // Persistent launches: every wave takes cells from a device work queue
 // until it is drained, so the grid is sized to the chip, not to the task
@@ -1451,8 +1456,9 @@ __global__ __launch_bounds__(64) void k_vertex_validation(CellCtx c, const uint3
     uint32_t* mySpill = spill + size_t(blockIdx.x) * kWaveScratchWords;
     const uint32_t count = __builtin_amdgcn_readfirstlane(counters[kTaskCount]);
     if (blockIdx.x == 0 && laneId() == 0) *roundTasks = count;
+    WorkClaim claim;
     for (;;) {
-        const uint32_t t = popWork(counters + kVertexQueue, count);
+        const uint32_t t = popWorkBatch(counters + kVertexQueue, count, claim, kTaskBatch);
         if (t == kNone) break;
         vertexTask(c, S, tasks[t], maxRestarts, mySpill);
         waveSync();
@@ -1464,8 +1470,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     __shared__ EdgeLds S;
     uint32_t* myScratch = scratch + size_t(blockIdx.x) * kWaveScratchWords;
     const uint32_t count = __builtin_amdgcn_readfirstlane(counters[kTaskCount]);
+    WorkClaim claim;
     for (;;) {
-        const uint32_t t = popWork(counters + kEdgeQueue, count);
+        const uint32_t t = popWorkBatch(counters + kEdgeQueue, count, claim, kTaskBatch);
         if (t == kNone) break;
         edgeTask(c, S, tasks[t], maxRestarts, myScratch);
         waveSync();
@@ -1478,8 +1485,9 @@ __global__ __launch_bounds__(64) void k_bisector_validation(CellCtx c, uint32_t*
     __shared__ BisectorLds S;
     uint32_t* mySpill = spill + size_t(blockIdx.x) * kWaveScratchWords;
     const uint32_t count = list ? readFirst(counters[kBisMarked]) : c.n;
+    WorkClaim claim;
     for (;;) {
-        const uint32_t t = popWork(counters + kBisectorQueue, count);
+        const uint32_t t = popWorkBatch(counters + kBisectorQueue, count, claim, kTaskBatch);
         if (t == kNone) break;
         bisectorTask(c, S, list ? list[t] : t, maxRestarts, mySpill);
         waveSync();

@@ -196,8 +196,12 @@ __device__ __forceinline__ uint32_t moleculeDigit(uint32_t m, int p) { return (m
 // the frame of its molecule m, and m's low 4 bits go to key bits 60-63.
 __global__ __launch_bounds__(256) void k_morton(const F4* __restrict__ sites, uint32_t n, Segments seg, int passes,
                                                 const Scene* __restrict__ scene, uint64_t* __restrict__ codes,
-                                                uint32_t* __restrict__ indices, uint32_t* __restrict__ hist) {
+                                                uint32_t* __restrict__ indices, uint32_t* __restrict__ hist,
+                                                RefitStarts rs) {
     __shared__ uint32_t sh[kMaxRadixPasses][256];
+    // the refit's start lists begin empty (the hierarchy kernel fills them)
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < refitStartWords(n); w += gridDim.x * blockDim.x)
+        rs.list[w] = ~0u;
     for (uint32_t t = threadIdx.x; t < kMaxRadixPasses * 256; t += blockDim.x) (&sh[0][0])[t] = 0u;
     __syncthreads();
     float fmn[3], scale[3];
@@ -594,97 +598,6 @@ __global__ __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(4)
 }
 
 // ---------------------------------------------------------------------------
-// Karras 2012 hierarchy (lbvh.cu:144-223) + gather of the sorted sites
-// (setLeaves, lbvh.cu:88-114).  Thread i handles internal node i and leaf i.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kKarrasBlock = 256;
-constexpr uint32_t kKarrasHalo = 256;  // codes staged in LDS on each side of the block's own
-
-__global__ __launch_bounds__(kKarrasBlock) void k_karras(const uint64_t* __restrict__ codes,
-                                                         const uint32_t* __restrict__ indices,
-                                                         const F4* __restrict__ sitesIn, uint32_t n, Segments seg,
-                                                         F4* __restrict__ sorted, Node* __restrict__ nodes,
-                                                         int32_t* __restrict__ parent,
-                                                         uint32_t* __restrict__ refitCounter,
-                                                         uint32_t* __restrict__ sizes, uint32_t* __restrict__ range,
-                                                         RefitStarts rs, uint32_t* err) {
-    // most comparisons of a node touch codes a few positions away: the block
-    // stages its codes and a halo on each side in LDS
-    __shared__ uint64_t wc[kKarrasBlock + 2 * kKarrasHalo];
-    const uint32_t blo = blockIdx.x * kKarrasBlock;
-    const uint32_t wlo = blo > kKarrasHalo ? blo - kKarrasHalo : 0u;
-    const uint32_t whi = n - blo > kKarrasBlock + kKarrasHalo ? blo + kKarrasBlock + kKarrasHalo : n;
-    for (uint32_t t = threadIdx.x; t < whi - wlo; t += kKarrasBlock) wc[t] = codes[wlo + t];
-    __syncthreads();
-    const uint32_t gi = blo + threadIdx.x;
-    if (gi >= n) return;
-    if (gi < refitStartWords(n)) rs.list[gi] = ~0u;
-    const uint32_t src = indices[gi];
-    if (src < n) {
-        sorted[gi] = sitesIn[src];
-    } else {
-        sorted[gi] = F4{0.f, 0.f, 0.f, 0.f};
-        atomicOr(err, kErrGatherIndex);
-    }
-    const uint32_t internal = n - seg.molecules;
-    sizes[internal + gi] = 1u;
-    // the molecule's segment: Karras runs on local indices, so each
-    // molecule's tree is the one a build of that molecule alone emits
-    uint32_t m = 0, base = 0, count = n;
-    if (seg.molecules > 1u) {
-        m = moleculeOf(seg.offsets, seg.molecules, gi);
-        base = seg.offsets[m];
-        count = seg.offsets[m + 1] - base;
-    }
-    const int i = int(gi - base);
-    const int ni = int(count);
-    if (i >= ni - 1) return;
-    const uint64_t ci = wc[gi - wlo];
-    // delta(i, j): common prefix of the codes, or 64 + that of the indices
-    // when the codes are equal; -1 outside the molecule
-    auto delta = [&](int j) -> int {
-        if (j < 0 || j >= ni) return -1;
-        const uint32_t gp = base + uint32_t(j);
-        const uint64_t cj = gp - wlo < whi - wlo ? wc[gp - wlo] : codes[gp];
-        if (ci != cj) return __builtin_clzll(ci ^ cj);
-        return __builtin_clzll(uint64_t(int64_t(i ^ j))) + 64;
-    };
-    const uint32_t nodeBase = base - m;  // the molecule's first internal node (its root)
-    const uint32_t g = nodeBase + uint32_t(i);
-    refitCounter[g] = 0u;
-    const int dd = delta(i + 1) - delta(i - 1);
-    const int d = (dd > 0) - (dd < 0);
-    const int deltaMin = delta(i - d);
-    int lmax = 2;
-    while (delta(i + lmax * d) > deltaMin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta(i + (l + t) * d) > deltaMin) l += t;
-    const int j = i + l * d;
-    const int deltaNode = delta(j);
-    int s = 0, k = 2, t;
-    do {
-        t = (l + k - 1) / k;
-        k <<= 1;
-        if (delta(i + (s + t) * d) > deltaNode) s += t;
-    } while (t > 1);
-    const int gamma = i + s * d + (d < 0 ? d : 0);
-    // a child whose range is one leaf is that leaf, else the internal node
-    // with the same local index
-    const uint32_t lc = (i < j ? i : j) == gamma ? internal + base + uint32_t(gamma) : nodeBase + uint32_t(gamma);
-    const uint32_t rc = (i > j ? i : j) == gamma + 1 ? internal + base + uint32_t(gamma) + 1u
-                                                     : nodeBase + uint32_t(gamma) + 1u;
-    reinterpret_cast<int2*>(&nodes[g].d)[0] = make_int2(int32_t(lc), int32_t(rc));
-    parent[lc] = int32_t(g);
-    parent[rc] = int32_t(g);
-    if (i == 0) parent[g] = -1;
-    // leaf range (a node's range starts or ends at its own local index)
-    const uint32_t first = base + uint32_t(i < j ? i : j), last = base + uint32_t(i > j ? i : j);
-    reinterpret_cast<uint2*>(range)[g] = make_uint2(first, last);
-    sizes[g] = last - first + 1u;
-}
-
-// ---------------------------------------------------------------------------
 // Bottom-up refit (lbvh.cu:225-272): one thread per leaf climbs; the second
 // thread to reach a node unions both child boxes.  Child boxes and max radii
 // are written into the parent's 64-byte record.
@@ -758,138 +671,11 @@ __device__ __forceinline__ void recordStart(const RefitStarts& rs, uint32_t p, u
     rs.list[refitSlotOffset(rs.n, uint32_t(level)) + slot] = p;
 }
 
-// Phase 0: one block per 2^kRefitShift consecutive leaves, one thread per
-// leaf, climbs through the nodes whose leaves all lie in its range.  Their
-// parents and leaf ranges are staged in LDS first: within a molecule a
-// node's index is one end of its leaf range minus the molecule index, so
-// the block's nodes form one window of at most 2^kRefitShift indices.  A
-// child is its parent's left child iff both ranges start at the same leaf.
-// Each arrival at a node inside the block leaves its box in LDS under the
-// node's split (the last leaf of its left child), one slot per side, and
-// counts itself there; the second arrival writes the node's whole record
-// and climbs on.  A node whose leaves leave the block only receives its
-// child's slot here (in the node record), and the child's finisher records
-// the node as a climb start of its phase when due.  Trees of a batch never
-// share a node, so each climb ends at its molecule's root.
-__global__ __launch_bounds__(kRefitLeaves) void k_refit_local(const F4* __restrict__ sorted, uint32_t n,
-                                                              uint32_t internal, Segments seg, Node* nodes,
-                                                              const int32_t* __restrict__ parent,
-                                                              const uint32_t* __restrict__ range, RefitStarts rs,
-                                                              float* rootBox, uint32_t* err) {
-    __shared__ int32_t wParent[kRefitLeaves];
-    __shared__ uint2 wRange[kRefitLeaves];
-    __shared__ uint32_t arrivals[kRefitLeaves];
-    __shared__ float xch[2][7][kRefitLeaves];  // [side][min xyz, max xyz, max radius][split - lo]
-    const uint32_t lo = blockIdx.x * kRefitLeaves;
-    const uint32_t hi = (n - lo) < kRefitLeaves ? n : lo + kRefitLeaves;
-    uint32_t wb = lo, we = hi;
-    if (seg.molecules > 1u) {
-        wb = lo - moleculeOf(seg.offsets, seg.molecules, lo);
-        we = hi - moleculeOf(seg.offsets, seg.molecules, hi - 1u);
-    }
-    we = we < internal ? we : internal;
-    const uint32_t wn = we > wb ? we - wb : 0u;
-    const uint32_t tid = threadIdx.x;
-    if (tid < wn) {
-        wParent[tid] = parent[wb + tid];
-        wRange[tid] = reinterpret_cast<const uint2*>(range)[wb + tid];
-    }
-    arrivals[tid] = 0u;
-    __syncthreads();
-    const uint32_t i = lo + tid;
-    if (i >= hi) return;
-    const bool single = internal + 1u == n;
-    float mn[3], mx[3];
-    const F4 s = sorted[i];
-    leafBox(s, mn, mx);
-    float maxr = s.w;
-    uint32_t node = internal + i, first = i, last = i;
-    int p = parent[internal + i];
-    for (uint32_t climb = 0; climb < n; climb++) {
-        if (p < 0) {
-            if (node >= internal || (single && node != 0u)) {
-                atomicOr(err, kErrRefitParent);
-            } else if (single) {
-                for (int c = 0; c < 3; c++) {
-                    rootBox[c] = mn[c];
-                    rootBox[3 + c] = mx[c];
-                }
-                rootBox[6] = maxr;
-            }
-            break;
-        }
-        if (uint32_t(p) >= internal) {
-            atomicOr(err, kErrRefitParent);
-            break;
-        }
-        const uint32_t w = uint32_t(p) - wb;
-        const bool inWindow = w < wn;
-        uint32_t pf, pl;
-        if (inWindow) {
-            const uint2 r = wRange[w];
-            pf = r.x;
-            pl = r.y;
-        } else {
-            pf = range[2 * p];
-            pl = range[2 * p + 1];
-        }
-        const bool left = pf == first;
-        if (left ? pl <= last : (pl != last || pf >= first)) {
-            atomicOr(err, kErrRefitParent);
-            break;
-        }
-        if (pf < lo || pl >= hi) {  // finished by a later phase
-            writeSlot(nodes + p, left, mn, mx, maxr);
-            if (left) recordStart(rs, uint32_t(p), pf, pl, last + 1u, err);
-            break;
-        }
-        if (!inWindow) {
-            atomicOr(err, kErrRefitParent);
-            break;
-        }
-        const uint32_t split = (left ? last : first - 1u) - lo;
-        const uint32_t side = left ? 0u : 1u;
-        for (int c = 0; c < 3; c++) {
-            xch[side][c][split] = mn[c];
-            xch[side][3 + c][split] = mx[c];
-        }
-        xch[side][6][split] = maxr;
-        // (LDS-only ordering: the node records written here are read by later
-        // kernels, so no wait for global stores)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        const uint32_t old = atomicAdd(&arrivals[split], 1u);
-        if (old == 0u) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-        float lmn[3], lmx[3], rmn[3], rmx[3];
-        for (int c = 0; c < 3; c++) {
-            lmn[c] = xch[0][c][split];
-            lmx[c] = xch[0][3 + c][split];
-            rmn[c] = xch[1][c][split];
-            rmx[c] = xch[1][3 + c][split];
-        }
-        const float lr = xch[0][6][split], rr = xch[1][6][split];
-        Node* P = nodes + p;
-        P->a = make_float4(lmn[0], lmn[1], lmn[2], lmx[0]);
-        P->b = make_float4(lmx[1], lmx[2], rmn[0], rmn[1]);
-        P->c = make_float4(rmn[2], rmx[0], rmx[1], rmx[2]);
-        reinterpret_cast<int2*>(&P->d)[1] = make_int2(__float_as_int(lr), __float_as_int(rr));
-        for (int c = 0; c < 3; c++) {
-            mn[c] = __builtin_fminf(lmn[c], rmn[c]);
-            mx[c] = __builtin_fmaxf(lmx[c], rmx[c]);
-        }
-        maxr = __builtin_fmaxf(lr, rr);
-        node = uint32_t(p);
-        first = pf;
-        last = pl;
-        p = wParent[w];
-    }
-}
-
 // Phase p >= 1: one wave per block of 2^refitShift(p) leaves.  Lane t
 // starts at the node recorded for the block's t-th boundary of level p-1
 // (if any) and climbs while the parent stays inside the block.  A parent
 // with a second child of level p-1 is continued by the later of the two
-// (arrival counter in refitCounter, zeroed by k_karras); a parent whose
+// (arrival counter in refitCounter, zeroed by k_hierarchy); a parent whose
 // other child finished in an earlier phase is continued at once.
 __global__ __launch_bounds__(64) void k_refit_phase(uint32_t n, uint32_t internal, uint32_t phase, Node* nodes,
                                                     const int32_t* __restrict__ parent,
@@ -947,6 +733,277 @@ __global__ __launch_bounds__(64) void k_refit_phase(uint32_t n, uint32_t interna
         node = uint32_t(p);
         first = pf;
         last = pl;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hierarchy and the first refit phase in one kernel: block b takes leaves
+// [512 b, 512 b + 512) and the internal nodes whose index lies in the same
+// window (a node's index is one end of its leaf range, minus the molecule
+// index).  Each thread first runs Karras 2012 for its node (lbvh.cu:144-223)
+// and gathers its leaf's site (setLeaves, lbvh.cu:88-114), then climbs from
+// its leaf as in the refit (lbvh.cu:225-272) through the nodes whose leaf
+// ranges lie inside the block.  Those nodes' parents, ranges and children
+// are known in the block (LDS), and the second arrival at a node writes its
+// whole 64-byte record (child boxes, child ids, max radii) once.  A node
+// whose parent was computed by another block is appended to `pending`
+// (k_refit_pending writes its slot after the kernel boundary); a node whose
+// parent was computed here but whose range leaves the block gets its slot
+// written now.  Child ids of the nodes whose ranges leave the block are
+// written by the Karras step.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kHierHalo = 256;  // codes staged in LDS on each side of the block's own
+constexpr int32_t kParentUnknown = -2;
+
+__global__ __launch_bounds__(kRefitLeaves) void k_hierarchy(const uint64_t* __restrict__ codes,
+                                                            const uint32_t* __restrict__ indices,
+                                                            const F4* __restrict__ sitesIn, uint32_t n, Segments seg,
+                                                            F4* __restrict__ sorted, Node* __restrict__ nodes,
+                                                            int32_t* __restrict__ parent,
+                                                            uint32_t* __restrict__ refitCounter,
+                                                            uint32_t* __restrict__ sizes, uint32_t* __restrict__ range,
+                                                            RefitStarts rs, uint32_t* __restrict__ pending,
+                                                            uint32_t* pendingCount, float* rootBox, uint32_t* err) {
+    // the staged codes and the climb's exchange slots share one buffer
+    __shared__ uint64_t pool[7 * kRefitLeaves];
+    __shared__ int32_t wParent[kRefitLeaves];      // parent of window node w, kParentUnknown if not computed here
+    __shared__ int32_t wLeafParent[kRefitLeaves];  // parent of the block's leaf t
+    __shared__ uint2 wRange[kRefitLeaves];
+    __shared__ uint32_t arrivals[kRefitLeaves];
+    __shared__ uint32_t pendLocal, pendBase;
+    uint64_t* wc = pool;
+    float(*xch)[7][kRefitLeaves] = reinterpret_cast<float(*)[7][kRefitLeaves]>(pool);  // [side][min, max, r][split - lo]
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lo = blockIdx.x * kRefitLeaves;
+    const uint32_t hi = (n - lo) < kRefitLeaves ? n : lo + kRefitLeaves;
+    const uint32_t internal = n - seg.molecules;
+    uint32_t wb = lo, we = hi;
+    if (seg.molecules > 1u) {
+        wb = lo - moleculeOf(seg.offsets, seg.molecules, lo);
+        we = hi - moleculeOf(seg.offsets, seg.molecules, hi - 1u);
+    }
+    we = we < internal ? we : internal;
+    const uint32_t wn = we > wb ? we - wb : 0u;
+    const uint32_t wlo = lo > kHierHalo ? lo - kHierHalo : 0u;
+    const uint32_t whi = n - hi > kHierHalo ? hi + kHierHalo : n;
+    for (uint32_t t = tid; t < whi - wlo; t += kRefitLeaves) wc[t] = codes[wlo + t];
+    wParent[tid] = kParentUnknown;
+    wLeafParent[tid] = kParentUnknown;
+    arrivals[tid] = 0u;
+    if (tid == 0) pendLocal = 0u;
+    __syncthreads();
+
+    const uint32_t gi = lo + tid;
+    const bool live = gi < hi;
+    F4 site{0.f, 0.f, 0.f, 0.f};
+    uint32_t m = 0, base = 0, count = n;
+    if (live) {
+        const uint32_t src = indices[gi];
+        if (src < n) site = sitesIn[src];
+        else atomicOr(err, kErrGatherIndex);
+        sorted[gi] = site;
+        sizes[internal + gi] = 1u;
+        if (seg.molecules > 1u) {
+            m = moleculeOf(seg.offsets, seg.molecules, gi);
+            base = seg.offsets[m];
+            count = seg.offsets[m + 1] - base;
+        }
+    }
+    const int i = int(gi - base);
+    const int ni = int(count);
+    const uint32_t nodeBase = base - m;  // the molecule's first internal node (its root)
+    if (live && i < ni - 1) {
+        const uint64_t ci = wc[gi - wlo];
+        // delta(i, j): common prefix of the codes, or 64 + that of the indices
+        // when the codes are equal; -1 outside the molecule
+        auto delta = [&](int j) -> int {
+            if (j < 0 || j >= ni) return -1;
+            const uint32_t gp = base + uint32_t(j);
+            const uint64_t cj = gp - wlo < whi - wlo ? wc[gp - wlo] : codes[gp];
+            if (ci != cj) return __builtin_clzll(ci ^ cj);
+            return __builtin_clzll(uint64_t(int64_t(i ^ j))) + 64;
+        };
+        const uint32_t g = nodeBase + uint32_t(i);
+        refitCounter[g] = 0u;
+        const int dd = delta(i + 1) - delta(i - 1);
+        const int d = (dd > 0) - (dd < 0);
+        const int deltaMin = delta(i - d);
+        int lmax = 2;
+        while (delta(i + lmax * d) > deltaMin) lmax <<= 1;
+        int l = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+            if (delta(i + (l + t) * d) > deltaMin) l += t;
+        const int j = i + l * d;
+        const int deltaNode = delta(j);
+        int s = 0, k = 2, t;
+        do {
+            t = (l + k - 1) / k;
+            k <<= 1;
+            if (delta(i + (s + t) * d) > deltaNode) s += t;
+        } while (t > 1);
+        const int gamma = i + s * d + (d < 0 ? d : 0);
+        // a child whose range is one leaf is that leaf, else the internal node
+        // with the same local index
+        const bool leftLeaf = (i < j ? i : j) == gamma, rightLeaf = (i > j ? i : j) == gamma + 1;
+        const uint32_t lc = leftLeaf ? internal + base + uint32_t(gamma) : nodeBase + uint32_t(gamma);
+        const uint32_t rc = rightLeaf ? internal + base + uint32_t(gamma) + 1u : nodeBase + uint32_t(gamma) + 1u;
+        parent[lc] = int32_t(g);
+        parent[rc] = int32_t(g);
+        if (i == 0) {
+            parent[g] = -1;
+            wParent[g - wb] = -1;
+        }
+        const uint32_t first = base + uint32_t(i < j ? i : j), last = base + uint32_t(i > j ? i : j);
+        reinterpret_cast<uint2*>(range)[g] = make_uint2(first, last);
+        sizes[g] = last - first + 1u;
+        wRange[g - wb] = make_uint2(first, last);
+        // the children's parent, for the climbs that reach them in this block
+        const uint32_t ls = base + uint32_t(gamma);  // global index of the split leaf
+        if (leftLeaf) {
+            if (ls - lo < kRefitLeaves) wLeafParent[ls - lo] = int32_t(g);
+        } else if (lc - wb < wn) {
+            wParent[lc - wb] = int32_t(g);
+        }
+        if (rightLeaf) {
+            if (ls + 1u - lo < kRefitLeaves) wLeafParent[ls + 1u - lo] = int32_t(g);
+        } else if (rc - wb < wn) {
+            wParent[rc - wb] = int32_t(g);
+        }
+        // a node finished in a later phase gets its child ids now; the others
+        // with their whole record at the end of their climb
+        if (first < lo || last >= hi) reinterpret_cast<int2*>(&nodes[g].d)[0] = make_int2(int32_t(lc), int32_t(rc));
+    }
+    __syncthreads();
+
+    uint32_t pend = ~0u, pendAt = 0u;
+    if (live) {
+        const bool single = internal + 1u == n;
+        float mn[3], mx[3];
+        leafBox(site, mn, mx);
+        float maxr = site.w;
+        uint32_t node = internal + gi, first = gi, last = gi;
+        int p = wLeafParent[tid];
+        for (uint32_t climb = 0; climb < n; climb++) {
+            if (p == kParentUnknown) {  // parent computed by another block: its slot is written later
+                pend = node;
+                pendAt = atomicAdd(&pendLocal, 1u);
+                break;
+            }
+            if (p < 0) {
+                if (node >= internal || (single && node != 0u)) {
+                    atomicOr(err, kErrRefitParent);
+                } else if (single) {
+                    for (int c = 0; c < 3; c++) {
+                        rootBox[c] = mn[c];
+                        rootBox[3 + c] = mx[c];
+                    }
+                    rootBox[6] = maxr;
+                }
+                break;
+            }
+            const uint32_t w = uint32_t(p) - wb;
+            if (uint32_t(p) >= internal || w >= wn) {
+                atomicOr(err, kErrRefitParent);
+                break;
+            }
+            const uint2 r = wRange[w];
+            const uint32_t pf = r.x, pl = r.y;
+            const bool left = pf == first;
+            if (left ? pl <= last : (pl != last || pf >= first)) {
+                atomicOr(err, kErrRefitParent);
+                break;
+            }
+            if (pf < lo || pl >= hi) {  // finished by a later phase
+                writeSlot(nodes + p, left, mn, mx, maxr);
+                if (left) recordStart(rs, uint32_t(p), pf, pl, last + 1u, err);
+                break;
+            }
+            const uint32_t sg = left ? last : first - 1u;  // split leaf (global)
+            const uint32_t split = sg - lo;
+            const uint32_t side = left ? 0u : 1u;
+            for (int c = 0; c < 3; c++) {
+                xch[side][c][split] = mn[c];
+                xch[side][3 + c][split] = mx[c];
+            }
+            xch[side][6][split] = maxr;
+            // (LDS-only ordering: the node records written here are read by
+            // later kernels, so no wait for global stores)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            const uint32_t old = atomicAdd(&arrivals[split], 1u);
+            if (old == 0u) break;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+            float lmn[3], lmx[3], rmn[3], rmx[3];
+            for (int c = 0; c < 3; c++) {
+                lmn[c] = xch[0][c][split];
+                lmx[c] = xch[0][3 + c][split];
+                rmn[c] = xch[1][c][split];
+                rmx[c] = xch[1][3 + c][split];
+            }
+            const float lr = xch[0][6][split], rr = xch[1][6][split];
+            // the Karras children of a node split after leaf sg
+            const uint32_t lc = pf == sg ? internal + sg : sg - m;
+            const uint32_t rc = pl == sg + 1u ? internal + sg + 1u : sg + 1u - m;
+            Node* P = nodes + p;
+            P->a = make_float4(lmn[0], lmn[1], lmn[2], lmx[0]);
+            P->b = make_float4(lmx[1], lmx[2], rmn[0], rmn[1]);
+            P->c = make_float4(rmn[2], rmx[0], rmx[1], rmx[2]);
+            P->d = make_int4(int32_t(lc), int32_t(rc), __float_as_int(lr), __float_as_int(rr));
+            for (int c = 0; c < 3; c++) {
+                mn[c] = __builtin_fminf(lmn[c], rmn[c]);
+                mx[c] = __builtin_fmaxf(lmx[c], rmx[c]);
+            }
+            maxr = __builtin_fmaxf(lr, rr);
+            node = uint32_t(p);
+            first = pf;
+            last = pl;
+            p = wParent[w];
+        }
+    }
+    __syncthreads();
+    if (tid == 0 && pendLocal) pendBase = atomicAdd(pendingCount, pendLocal);
+    __syncthreads();
+    if (pend != ~0u) pending[pendBase + pendAt] = pend;
+}
+
+// The nodes k_hierarchy appended to `pending` (their parent was computed by
+// another block, so its range leaves theirs): each writes its box into its
+// side of the parent's record, and the left child records the parent as a
+// climb start of the parent's phase when due.
+__global__ __launch_bounds__(256) void k_refit_pending(const uint32_t* __restrict__ pending,
+                                                       const uint32_t* __restrict__ pendingCount, uint32_t internal,
+                                                       const F4* __restrict__ sorted, Node* nodes,
+                                                       const int32_t* __restrict__ parent,
+                                                       const uint32_t* __restrict__ range, RefitStarts rs,
+                                                       uint32_t* err) {
+    const uint32_t count = *pendingCount;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+        const uint32_t c = pending[k];
+        float mn[3], mx[3], maxr;
+        uint32_t first, last;
+        if (c >= internal) {
+            first = last = c - internal;
+            const F4 s = sorted[first];
+            leafBox(s, mn, mx);
+            maxr = s.w;
+        } else {
+            nodeUnion(nodes + c, mn, mx, maxr);
+            first = range[2 * c];
+            last = range[2 * c + 1];
+        }
+        const int p = parent[c];
+        if (p < 0 || uint32_t(p) >= internal) {
+            atomicOr(err, kErrRefitParent);
+            continue;
+        }
+        const uint32_t pf = range[2 * p], pl = range[2 * p + 1];
+        const bool left = pf == first;
+        if (left ? pl <= last : (pl != last || pf >= first)) {
+            atomicOr(err, kErrRefitParent);
+            continue;
+        }
+        writeSlot(nodes + p, left, mn, mx, maxr);
+        if (left) recordStart(rs, uint32_t(p), pf, pl, last + 1u, err);
     }
 }
 
@@ -1062,7 +1119,8 @@ uint32_t passTiles(uint32_t n, int p) {
 
 size_t sortScratchWords(uint32_t n) {
     const size_t tiles = (n + kTileKeys - 1) / kTileKeys;  // the most any pass has
-    return size_t(kMaxRadixPasses) * 256u + kMaxRadixPasses + size_t(kMaxRadixPasses) * tiles * 256u;
+    // (+1: the hierarchy's pending count)
+    return size_t(kMaxRadixPasses) * 256u + kMaxRadixPasses + size_t(kMaxRadixPasses) * tiles * 256u + 1u;
 }
 
 hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const Segments& seg, hipStream_t st) {
@@ -1070,7 +1128,7 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     // sort counters and look-back (each pass's tiles x 256 words)
     size_t lbWords = 0;
     for (int p = 0; p < passes; p++) lbWords += size_t(passTiles(n, p)) * 256u;
-    const uint32_t zeroWords = uint32_t(passes) * 256u + uint32_t(passes) + uint32_t(lbWords);
+    const uint32_t zeroWords = uint32_t(passes) * 256u + uint32_t(passes) + uint32_t(lbWords) + 1u;
     if (seg.molecules > 1u) {
         hipLaunchKernelGGL(k_scene_segments, dim3(seg.molecules), dim3(256), 0, st, sitesIn, seg.offsets, b.scene,
                            b.sortScratch, zeroWords);
@@ -1082,10 +1140,11 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     uint32_t* hist = b.sortScratch;
     uint32_t* tileCounters = hist + passes * 256u;
     uint32_t* lookback = tileCounters + passes;
+    uint32_t* pendingCount = lookback + lbWords;
     // (few blocks: each flushes passes x 256 histogram counts with global atomics)
     const uint32_t mortonBlocks = (n + 255u) / 256u < 512u ? (n + 255u) / 256u : 512u;
     hipLaunchKernelGGL(k_morton, dim3(mortonBlocks), dim3(256), 0, st, sitesIn, n, seg, passes, b.scene, b.codesA,
-                       b.idxA, hist);
+                       b.idxA, hist, RefitStarts{b.startLists, n});
     uint64_t* kin = b.codesA;
     uint32_t* vin = b.idxA;
     uint64_t* kout = b.codesB;
@@ -1112,11 +1171,18 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     }
     const uint32_t internal = n - seg.molecules;
     const RefitStarts rs{b.startLists, n};
-    hipLaunchKernelGGL(k_karras, dim3((n + kKarrasBlock - 1u) / kKarrasBlock), dim3(kKarrasBlock), 0, st, b.codesA, b.idxA, sitesIn, n, seg, b.sorted, b.nodes,
-                       b.parent, b.refitCounter, b.sizes, b.range, rs, b.err);
+    // the second key buffer is free once the sort is done: it holds the
+    // hierarchy's pending nodes (at most one per node and leaf)
+    uint32_t* pending = reinterpret_cast<uint32_t*>(b.codesB);
     const uint32_t refitBlocks = (n + kRefitLeaves - 1u) / kRefitLeaves;
-    hipLaunchKernelGGL(k_refit_local, dim3(refitBlocks), dim3(kRefitLeaves), 0, st, b.sorted, n, internal, seg,
-                       b.nodes, b.parent, b.range, rs, b.rootBox, b.err);
+    hipLaunchKernelGGL(k_hierarchy, dim3(refitBlocks), dim3(kRefitLeaves), 0, st, b.codesA, b.idxA, sitesIn, n, seg,
+                       b.sorted, b.nodes, b.parent, b.refitCounter, b.sizes, b.range, rs, pending, pendingCount,
+                       b.rootBox, b.err);
+    if (refitBlocks > 1u) {
+        const uint32_t pendingBlocks = refitBlocks < 1024u ? refitBlocks : 1024u;
+        hipLaunchKernelGGL(k_refit_pending, dim3(pendingBlocks), dim3(256), 0, st, pending, pendingCount, internal,
+                           b.sorted, b.nodes, b.parent, b.range, rs, b.err);
+    }
     for (uint32_t phase = 1; phase <= refitPhases(n); phase++)
         hipLaunchKernelGGL(k_refit_phase, dim3((refitSlots(n, phase - 1u) + 63u) / 64u), dim3(64), 0, st, n, internal,
                            phase, b.nodes, b.parent, b.range, rs, b.refitCounter, b.rootBox, b.err);

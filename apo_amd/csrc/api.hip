@@ -121,7 +121,6 @@ struct apo_ctx {
     F4* sorted = nullptr;
     Node* nodes = nullptr;
     int32_t* parent = nullptr;
-    uint32_t* sizes = nullptr;
     uint32_t* refitCounter = nullptr;
     float* rootBox = nullptr;
     // cells
@@ -261,7 +260,6 @@ apo_status ensureCapacity(apo_ctx* ctx, uint32_t n, uint32_t K) {
     APO_HIP(alloc(ctx->sorted, cap));
     APO_HIP(alloc(ctx->nodes, cap));
     APO_HIP(alloc(ctx->parent, nodes));
-    APO_HIP(alloc(ctx->sizes, nodes));
     APO_HIP(alloc(ctx->refitCounter, cap));
     APO_HIP(alloc(ctx->rootBox, 8));
     APO_HIP(alloc(ctx->status, cap));
@@ -381,7 +379,7 @@ apo_status stageCheck(apo_ctx* ctx, const char* stage) {
 LbvhBuffers lbvhBuffers(apo_ctx* ctx) {
     return LbvhBuffers{ctx->scene,  ctx->boxPartials, ctx->boxTicket,    ctx->sortScratch, ctx->codesA,
                        ctx->codesB, ctx->idxA,        ctx->idxB,         ctx->sorted,      ctx->nodes,
-                       ctx->parent, ctx->sizes,       ctx->refitCounter, ctx->rootBox,     ctx->err,
+                       ctx->parent, ctx->refitCounter, ctx->rootBox,     ctx->err,
                        ctx->range,  ctx->startLists};
 }
 
@@ -834,7 +832,7 @@ void apo_destroy(apo_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     dropGraph(ctx);
     void* ptrs[] = {ctx->offsets, ctx->moleculeVertices, ctx->sitesIn, ctx->scene, ctx->boxPartials, ctx->boxTicket, ctx->sortScratch, ctx->codesA,
-                    ctx->codesB, ctx->idxA, ctx->idxB, ctx->sorted, ctx->nodes, ctx->parent, ctx->sizes,
+                    ctx->codesB, ctx->idxA, ctx->idxB, ctx->sorted, ctx->nodes, ctx->parent,
                     ctx->refitCounter, ctx->rootBox, ctx->status, ctx->knownRadii, ctx->flags, ctx->vtx, ctx->vnb,
                     ctx->edg, ctx->enb, ctx->knn, ctx->counters, ctx->tasks, ctx->cellCount, ctx->blockSums,
                     ctx->total, ctx->outXyzw, ctx->outIds, ctx->err, ctx->bytes, ctx->range, ctx->startLists,
@@ -969,7 +967,7 @@ namespace {
 // context's own afterwards.
 #define APO_BUILD_BUFFERS(X)                                                                                     \
     X(scene) X(boxPartials) X(boxTicket) X(sortScratch) X(range) X(startLists) X(codesA) X(codesB) X(idxA) X(idxB) \
-    X(sorted) X(nodes) X(parent) X(sizes) X(refitCounter) X(rootBox) X(status) X(knownRadii) X(flags) X(vtx)       \
+    X(sorted) X(nodes) X(parent) X(refitCounter) X(rootBox) X(status) X(knownRadii) X(flags) X(vtx)       \
     X(vnb) X(edg) X(enb) X(knn) X(counters) X(spill) X(err) X(tasks) X(cellCount) X(blockSums) X(total)            \
     X(outXyzw) X(outIds) X(bytes) X(want) X(stuck) X(bisSearch) X(bisNb) X(bisCount) X(bisMark)
 
@@ -1034,7 +1032,6 @@ bool carveStatic(void* base, uint32_t n, uint32_t K, double* xyzw, uint32_t* ids
     c.take(b.sorted, n);
     c.take(b.nodes, n);
     c.take(b.parent, nodes);
-    c.take(b.sizes, nodes);
     c.take(b.refitCounter, n);
     c.take(b.rootBox, 8);
     c.take(b.status, n);
@@ -1442,7 +1439,13 @@ apo_status apo_get_lbvh(apo_ctx* ctx, uint64_t* sorted_codes, int32_t* parent, i
     std::vector<int32_t> par(nodes);
     APO_HIP(hipMemcpy(par.data(), ctx->parent, nodes * 4, hipMemcpyDeviceToHost));
     if (parent) std::memcpy(parent, par.data(), nodes * 4);
-    if (sizes) APO_HIP(hipMemcpy(sizes, ctx->sizes, nodes * 4, hipMemcpyDeviceToHost));
+    if (sizes) {
+        // leaf counts: one per leaf, last - first + 1 per internal node
+        std::vector<uint32_t> rg(size_t(internal) * 2);
+        APO_HIP(hipMemcpy(rg.data(), ctx->range, rg.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < internal; i++) sizes[i] = rg[2 * size_t(i) + 1] - rg[2 * size_t(i)] + 1u;
+        for (size_t i = internal; i < nodes; i++) sizes[i] = 1u;
+    }
     std::vector<Node> nd(internal);
     APO_HIP(hipMemcpy(nd.data(), ctx->nodes, size_t(internal) * sizeof(Node), hipMemcpyDeviceToHost));
     // unpack the child-in-parent records into per-node arrays; a root's box

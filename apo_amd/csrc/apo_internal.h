@@ -77,7 +77,6 @@ struct LbvhBuffers {
     F4* sorted;            // sites in Morton order
     Node* nodes;           // n-1 internal nodes
     int32_t* parent;       // 2n-1
-    uint32_t* sizes;       // 2n-1
     uint32_t* refitCounter;// n-1
     float* rootBox;        // 7 floats: min, max, max radius
     uint32_t* err;         // device guard bits

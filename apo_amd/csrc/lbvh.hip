@@ -94,6 +94,8 @@ __device__ __forceinline__ void blockExtent(double* mn, double* mx, double* r) {
         }
 }
 
+constexpr uint32_t kStreamUnroll = 8;  // site loads in flight per thread in the streaming kernels
+
 __global__ __launch_bounds__(256) void k_scene_box(const F4* __restrict__ sites, uint32_t n, Scene* scene,
                                                    double* partials, uint32_t* ticket, uint32_t* zero,
                                                    uint32_t zeroWords) {
@@ -101,8 +103,15 @@ __global__ __launch_bounds__(256) void k_scene_box(const F4* __restrict__ sites,
     double mn[4] = {DBL_MAX, DBL_MAX, DBL_MAX, DBL_MAX};
     double mx[4] = {-DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        accumulateExtent(sites[i], mn, mx);
+    // kStreamUnroll loads in flight per thread before their use
+    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += kStreamUnroll * stride) {
+        F4 sv[kStreamUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kStreamUnroll; u++)
+            if (i0 + u * stride < n) sv[u] = sites[i0 + u * stride];
+#pragma unroll
+        for (uint32_t u = 0; u < kStreamUnroll; u++)
+            if (i0 + u * stride < n) accumulateExtent(sv[u], mn, mx);
     }
     // clear the scratch words that later kernels accumulate into
     for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < zeroWords; w += stride) zero[w] = 0u;
@@ -211,20 +220,29 @@ __global__ __launch_bounds__(256) void k_morton(const F4* __restrict__ sites, ui
     }
     const bool batch = seg.molecules > 1u;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t m = 0;
-        if (batch) {
-            m = moleculeOf(seg.offsets, seg.molecules, i);
-            for (int a = 0; a < 3; a++) {
-                fmn[a] = scene[m].fmn[a];
-                scale[a] = scene[m].scale[a];
+    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += kStreamUnroll * stride) {
+        F4 sv[kStreamUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kStreamUnroll; u++)
+            if (i0 + u * stride < n) sv[u] = sites[i0 + u * stride];
+#pragma unroll
+        for (uint32_t u = 0; u < kStreamUnroll; u++) {
+            const uint32_t i = i0 + u * stride;
+            if (i >= n) break;
+            uint32_t m = 0;
+            if (batch) {
+                m = moleculeOf(seg.offsets, seg.molecules, i);
+                for (int a = 0; a < 3; a++) {
+                    fmn[a] = scene[m].fmn[a];
+                    scale[a] = scene[m].scale[a];
+                }
             }
+            const uint64_t code = mortonCode(sv[u], fmn, scale) | (uint64_t(m & 15u) << 60);
+            codes[i] = code;
+            indices[i] = i;
+            for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p][(code >> (8 * p)) & 255u], 1u);
+            for (int p = kRadixPasses; p < passes; p++) atomicAdd(&sh[p][moleculeDigit(m, p)], 1u);
         }
-        const uint64_t code = mortonCode(sites[i], fmn, scale) | (uint64_t(m & 15u) << 60);
-        codes[i] = code;
-        indices[i] = i;
-        for (int p = 0; p < kRadixPasses; p++) atomicAdd(&sh[p][(code >> (8 * p)) & 255u], 1u);
-        for (int p = kRadixPasses; p < passes; p++) atomicAdd(&sh[p][moleculeDigit(m, p)], 1u);
     }
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < uint32_t(passes) * 256u; t += blockDim.x) {
@@ -761,7 +779,7 @@ __global__ __launch_bounds__(kRefitLeaves) void k_hierarchy(const uint64_t* __re
                                                             F4* __restrict__ sorted, Node* __restrict__ nodes,
                                                             int32_t* __restrict__ parent,
                                                             uint32_t* __restrict__ refitCounter,
-                                                            uint32_t* __restrict__ sizes, uint32_t* __restrict__ range,
+                                                            uint32_t* __restrict__ range,
                                                             RefitStarts rs, uint32_t* __restrict__ pending,
                                                             uint32_t* pendingCount, float* rootBox, uint32_t* err) {
     // the staged codes and the climb's exchange slots share one buffer
@@ -803,7 +821,6 @@ __global__ __launch_bounds__(kRefitLeaves) void k_hierarchy(const uint64_t* __re
         if (src < n) site = sitesIn[src];
         else atomicOr(err, kErrGatherIndex);
         sorted[gi] = site;
-        sizes[internal + gi] = 1u;
         if (seg.molecules > 1u) {
             m = moleculeOf(seg.offsets, seg.molecules, gi);
             base = seg.offsets[m];
@@ -825,7 +842,6 @@ __global__ __launch_bounds__(kRefitLeaves) void k_hierarchy(const uint64_t* __re
             return __builtin_clzll(uint64_t(int64_t(i ^ j))) + 64;
         };
         const uint32_t g = nodeBase + uint32_t(i);
-        refitCounter[g] = 0u;
         const int dd = delta(i + 1) - delta(i - 1);
         const int d = (dd > 0) - (dd < 0);
         const int deltaMin = delta(i - d);
@@ -856,7 +872,6 @@ __global__ __launch_bounds__(kRefitLeaves) void k_hierarchy(const uint64_t* __re
         }
         const uint32_t first = base + uint32_t(i < j ? i : j), last = base + uint32_t(i > j ? i : j);
         reinterpret_cast<uint2*>(range)[g] = make_uint2(first, last);
-        sizes[g] = last - first + 1u;
         wRange[g - wb] = make_uint2(first, last);
         // the children's parent, for the climbs that reach them in this block
         const uint32_t ls = base + uint32_t(gamma);  // global index of the split leaf
@@ -870,9 +885,13 @@ __global__ __launch_bounds__(kRefitLeaves) void k_hierarchy(const uint64_t* __re
         } else if (rc - wb < wn) {
             wParent[rc - wb] = int32_t(g);
         }
-        // a node finished in a later phase gets its child ids now; the others
-        // with their whole record at the end of their climb
-        if (first < lo || last >= hi) reinterpret_cast<int2*>(&nodes[g].d)[0] = make_int2(int32_t(lc), int32_t(rc));
+        // a node finished in a later phase gets its child ids and a cleared
+        // arrival counter now; the others their whole record at the end of
+        // their climb
+        if (first < lo || last >= hi) {
+            reinterpret_cast<int2*>(&nodes[g].d)[0] = make_int2(int32_t(lc), int32_t(rc));
+            refitCounter[g] = 0u;
+        }
     }
     __syncthreads();
 
@@ -1176,7 +1195,7 @@ hipError_t lbvhBuild(const LbvhBuffers& b, const F4* sitesIn, uint32_t n, const 
     uint32_t* pending = reinterpret_cast<uint32_t*>(b.codesB);
     const uint32_t refitBlocks = (n + kRefitLeaves - 1u) / kRefitLeaves;
     hipLaunchKernelGGL(k_hierarchy, dim3(refitBlocks), dim3(kRefitLeaves), 0, st, b.codesA, b.idxA, sitesIn, n, seg,
-                       b.sorted, b.nodes, b.parent, b.refitCounter, b.sizes, b.range, rs, pending, pendingCount,
+                       b.sorted, b.nodes, b.parent, b.refitCounter, b.range, rs, pending, pendingCount,
                        b.rootBox, b.err);
     if (refitBlocks > 1u) {
         const uint32_t pendingBlocks = refitBlocks < 1024u ? refitBlocks : 1024u;

@@ -514,8 +514,10 @@ def main():
     # Each step is bracketed by device synchronisations; per-kernel times
     # and byte counts are read between steps, outside the timed spans, and
     # summed over all timed steps.
+    # (the bisector entry is the whole stage: the plan, search and per-cell
+    # kernels, timed and counted together)
     kern = {"k_edge_validation": [0.0, 0, 0], "k_vertex_validation": [0.0, 0, 0], "k_cells": [0.0, 0, 0],
-            "k_bisector_validation": [0.0, 0, 0]}
+            "bisector_stage": [0.0, 0, 0]}
     s = _lib.Stats()
     wall = 0.0
     # every build launches each validation kernel once per round (rounds
@@ -532,7 +534,7 @@ def main():
                 ("k_edge_validation", s.ms_edge_validation_kernel, s.bytes_edge_validation, rounds),
                 ("k_vertex_validation", s.ms_vertex_validation_kernel, s.bytes_vertex_validation, rounds),
                 ("k_cells", s.ms_cells, s.bytes_cells, 1),
-                ("k_bisector_validation", s.ms_bisector, s.bytes_bisector_validation, 1)):
+                ("bisector_stage", s.ms_bisector, s.bytes_bisector_validation, 1)):
             kern[name][0] += ms
             kern[name][1] += b
             kern[name][2] += launches

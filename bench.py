@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--proposal-cost", type=float, default=None, help="override the tuned proposal cost")
     ap.add_argument("--no-c5", action="store_true", help="skip the 5M-site learned evaluation")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-molecules workload")
+    ap.add_argument("--lbvh-only", action="store_true",
+                    help="only the LBVH-alone timings at 1M, 4M and 16M sites (for kernel traces)")
     ap.add_argument("--batch-molecules", type=int, default=512)
     return ap.parse_args()
 
@@ -485,6 +487,10 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.lbvh_only:
+        print(json.dumps({"lbvh_sizes": lbvh_sizes(lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))}),
+              flush=True)
+        return
     n = args.n
     sites = datasets.loguniform_sites(n, seed=3)
     dev = torch.from_numpy(sites).to("cuda")

@@ -436,21 +436,22 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const uint64_t* __res
 }
 
 // ---------------------------------------------------------------------------
-// The same pass over key digits with tiles of 6144 keys (8 waves x 12
+// The same pass over key digits with tiles of 8192 keys (8 waves x 16
 // rounds x 64 lanes) for large sorts: fewer tiles, so a shorter look-back
 // chain and longer runs per digit in the scatter.  The keys are staged in
 // LDS by output position and written; each slot then takes its output index
-// (low word) and value (high word), and the values are written: one 48 KB
+// (low word) and value (high word), and the values are written: one 64 KB
 // staging buffer for both, so two blocks fit a CU.  Same ranking, so the
 // same stable order.  Below kWideMinKeys keys the 4096-key tiles fill the
-// chip better (1M keys: 0.35 against 0.37 ms for the LBVH; 16M keys: 3.04
-// against 3.28 ms).
+// chip better (1M keys: 0.35 against 0.37 ms for the LBVH with 6144-key
+// tiles); from 2M keys the wide ones win (4M: 0.81 -> 0.78 ms; 16M: 2.70 ->
+// 2.62 ms against 6144-key tiles from 4M keys).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kWideWaves = 8;
-constexpr uint32_t kWideRounds = 12;
+constexpr uint32_t kWideRounds = 16;
 constexpr uint32_t kWideThreads = 64u * kWideWaves;
 constexpr uint32_t kWideKeys = kWideThreads * kWideRounds;
-constexpr uint32_t kWideMinKeys = 1u << 22;
+constexpr uint32_t kWideMinKeys = 1u << 21;
 
 __global__ __launch_bounds__(kWideThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_onesweep_wide(const uint64_t* __restrict__ keysIn,
                                                                const uint32_t* __restrict__ valsIn,

@@ -28,8 +28,10 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
                                             const float* __restrict__ bound) {
     __shared__ uint32_t hKey[KT][64];
     __shared__ float hVal[KT][64];
+    // stacked entries: parent node << 1 | the stacked child's side; its gap
+    // is recomputed from the parent's copy of the child box when popped
+    // (the FP32 value the reference stacks), so the stack takes half the LDS
     __shared__ uint32_t stNode[kStackDepth][64];
-    __shared__ float stDist[kStackDepth][64];
 
     const uint32_t lane = laneId();
     const uint32_t base = blockIdx.x * 64u;
@@ -80,11 +82,11 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
                 const bool exploreFar = minFar < root && maxFar > startingDistance;
                 const bool exploreClose = minClose < root && maxClose > startingDistance;
                 if (!exploreFar && !exploreClose) break;
+                const uint32_t here = node;
                 node = exploreClose ? closeChild : farChild;
                 if (exploreClose && exploreFar) {
                     if (sp < kStackDepth) {
-                        stNode[sp][lane] = farChild;
-                        stDist[sp][lane] = minFar;
+                        stNode[sp][lane] = (here << 1) | (leftClosest ? 1u : 0u);
                         sp++;
                     } else {
                         overflow = 1;
@@ -122,8 +124,13 @@ __global__ __launch_bounds__(64) void k_knn(CellCtx c, uint32_t* __restrict__ kn
             bool found = false;
             while (sp > 0) {
                 sp--;
-                if (stDist[sp][lane] > hVal[0][lane]) continue;
-                node = stNode[sp][lane];
+                const uint32_t e = stNode[sp][lane];
+                const Node pn = c.nodes[e >> 1];
+                Box lb, rb;
+                nodeBoxes(pn, lb, rb);
+                const bool right = (e & 1u) != 0u;
+                if (boxDistance(selBox(right, rb, lb), s) > hVal[0][lane]) continue;
+                node = uint32_t(right ? pn.d.y : pn.d.x);
                 found = true;
                 break;
             }
